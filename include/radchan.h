@@ -1,0 +1,174 @@
+/*
+ * radchan.h -- C ABI of the MI355X radiative-transfer library
+ * (narrow-band correlated-k OERM Monte Carlo, a drop-in for RadChan's
+ * src/montecarlo path on one GPU).
+ *
+ * Two groups of entry points:
+ *
+ *  1. The reference's own entry points, same symbols and by-reference
+ *     signatures, called from Fortran through implicit F77 interfaces
+ *     (src/dns/channel.f:242,251,255 -> gfortran mangles MC_GPU/GET_RESULTS
+ *     to mc_gpu_/get_results_).  Parameters come from rc_set_params (defaults:
+ *     the values of src/param/param.h and src/param/NarrowBand.h); tables are
+ *     read from "<table dir>/" (default "properties", as read_tables.cpp:26,45,68).
+ *
+ *  2. Extensions (no counterpart in the reference): runtime parameters, error
+ *     query, explicit tables, a device-resident session API for repeated
+ *     solves and batches of temperature snapshots, and a per-ray trace used by
+ *     the parity tests.
+ *
+ * All functions are host functions.  Integer returns: 0 on success, a
+ * negative code on failure with a message available from rc_last_error().
+ * The two reference entry points return void; on failure they print the
+ * message to stderr and leave the outputs untouched (the reference aborts
+ * the process, param.h:102-124), and rc_last_error() reports it.
+ */
+#ifndef RADCHAN_H
+#define RADCHAN_H
+
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+#define RC_MAXLEV 9
+
+/* Runtime form of src/param/param.h + src/param/NarrowBand.h. */
+typedef struct rc_params {
+    int32_t imax, jmax, kmax;        /* param.h:10-12, global sizes            */
+    int32_t p_col;                   /* param.h:9, z slabs (nodes); 1 = whole  */
+    float   Lx, Ly, Lz;              /* param.h:13-15                          */
+    int32_t nVar;                    /* param.h:18                             */
+    double  fact_mesh;               /* param.h:16                             */
+    double  chanHeight;              /* param.h:20                             */
+    int32_t photmax;                 /* param.h:19                             */
+    int32_t grid_num;                /* param.h:24                             */
+    int32_t srt, grey, calcwalls;    /* param.h:57-61                          */
+    int32_t grey_true;               /* grey only: 1 band, Ib = sigma T^4/pi   */
+    int32_t maxi[RC_MAXLEV], maxj[RC_MAXLEV], maxk[RC_MAXLEV]; /* param.h:27-29 */
+    int32_t maxs[RC_MAXLEV];         /* param.h:31-49                          */
+    int32_t bd[6];                   /* W,E,S,N,B,T faces, param.h:95-100      */
+    int32_t adiabw, adiabe;          /* param.h:82-83                          */
+    float   eps[6];                  /* param.h:68-73                          */
+    float   Tw[6];                   /* param.h:75-80                          */
+    float   toll, tollS;             /* param.h:86-87                          */
+    int32_t nT, nQ, nB;              /* NarrowBand.h:10-12 (grey :35-37)       */
+    float   abscoeff, Tmins, wvmax;  /* NarrowBand.h:38-40                     */
+    uint64_t seed;                   /* Philox key (no reference counterpart)  */
+} rc_params;
+
+/* Spectral tables in the reference's original band order, as read by readT
+ * (src/montecarlo/read_tables.cpp:21-95).  Caller-owned arrays:
+ * Tnb[nT], kP[nT] (scaled by chanHeight), wvl/wvr/wvc/kavg[nB],
+ * kq[nB][nT][nQ] (scaled), prob[nT][nB], probg[nT][nB][nQ], wq[nQ]. */
+typedef struct rc_tables {
+    int32_t nB, nT, nQ, pad;
+    float *Tnb, *kP;
+    float *wvl, *wvr, *wvc, *kavg;
+    float *kq, *prob, *probg;
+    float *wq;
+} rc_tables;
+
+/* Per-ray record of rc_trace (layout shared with the CPU oracle). */
+typedef struct rc_ray {
+    int32_t nb, g;              /* band (sorted rank order) and g-point       */
+    int32_t ic, jc, kc, level;  /* cell and grid level where the ray ended     */
+    int32_t steps, status;      /* march steps; end status (RC_RAY_*)          */
+    uint32_t path_hash;         /* FNV-1a over every visited (ic, jc, kc)      */
+    float sx, sy, sz;           /* emission direction                          */
+    float tra;                  /* transmissivity at the end                   */
+    float de;                   /* FP32 tally of the ray                       */
+    double de64;                /* unused by the GPU (0)                       */
+    double sumabs;              /* unused by the GPU (0)                       */
+} rc_ray;
+
+#define RC_RAY_ABSORBED 0
+#define RC_RAY_WALL     1
+#define RC_RAY_LOST     2
+#define RC_RAY_CAPPED   3
+#define RC_RAY_STOPPED  4
+
+/* Solver variants */
+#define RC_VARIANT_PLAIN   0   /* thread per emitting cell, FP32 per-cell sums  */
+#define RC_VARIANT_BATCHED 1   /* persistent wavefront work queue, FP64 tallies */
+
+typedef struct rc_stats {
+    double   kernel_ms;         /* device time of the last run (HIP events)   */
+    double   march_ms;          /* time of the march kernel(s) alone          */
+    int64_t  rays;              /* rays marched in the last run               */
+    int64_t  steps;             /* march steps executed in the last run       */
+    int64_t  capped;            /* rays that hit the last level's step cap    */
+    int32_t  launches;          /* kernel launches in the last run            */
+    int32_t  variant;
+} rc_stats;
+
+/* ---- 1. reference entry points --------------------------------------- */
+
+/* src/montecarlo/mc_gpu_noint.cu:660  extern "C" void mc_gpu_(myfloatF *Tfort,
+ * int *zstart, int *rank).  Tfort: (imax+2)(jmax+2)(kmax+2) temperatures in
+ * Kelvin, i fastest (idx_T, param.h:132).  Copies the field, launches the
+ * solve asynchronously and returns. */
+void mc_gpu_(float *Tfort, int *zstart, int *rank);
+
+/* src/montecarlo/mc_gpu_noint.cu:876  extern "C" void get_results_(myfloatF
+ * resfort[], myfloatF varfort[], int *rank).  Waits for the solve started by
+ * mc_gpu_ and scatters div(q_r) and sqrt(variance) to the Fortran layout
+ * idx_F (param.h:138, k local to the slab), size (imax+2)(jmax+2)(kmax/p_col+2).
+ * Planes i=0 and i=imax+1 receive the wall fluxes when calcwalls==1 (else 0). */
+void get_results_(float *resfort, float *varfort, int *rank);
+
+/* ---- 2. extensions ----------------------------------------------------- */
+
+void        rc_default_params(rc_params *p);
+int         rc_set_params(const rc_params *p);       /* used by mc_gpu_        */
+int         rc_get_params(rc_params *p);
+int         rc_set_table_dir(const char *dir);       /* default "properties"   */
+int         rc_set_variant(int variant);             /* used by mc_gpu_        */
+const char *rc_last_error(void);
+const char *rc_version(void);
+
+/* tables: reader and writer of the reference text formats, grey synthesis */
+int rc_read_tables(const char *dir, double chanHeight, rc_tables *t);
+int rc_write_tables(const char *dir, const rc_tables *t, double chanHeight);
+int rc_grey_tables(const rc_params *p, float Tmax, rc_tables *t);
+/* sort_idx (mc_gpu_noint.cu:1278): idx[rank] = band, ascending kavg, ties by index */
+int rc_sort_idx(const float *kavg, int nB, int32_t *idx);
+/* Gridn::mk_grid (definitions.h:68-136) of one level */
+int rc_grid(const rc_params *p, int level, float *x, float *xu, float *y,
+            float *yu, float *z, float *zu);
+
+/* session: device-resident solver state */
+typedef struct rc_ctx rc_ctx;
+rc_ctx *rc_create(const rc_params *p);
+void    rc_destroy(rc_ctx *c);
+/* tables for a narrow-band run (ignored for grey runs, which synthesise
+ * their tables from each field's Tmax) */
+int rc_load_tables(rc_ctx *c, const rc_tables *t);
+/* nsnap temperature snapshots of the same shape, host arrays of
+ * (imax+2)(jmax+2)(kmax+2) floats each; copied to the device */
+int rc_set_fields(rc_ctx *c, int nsnap, const float *const *T, int zstart);
+/* launch the solve of every loaded snapshot (asynchronous) */
+int rc_run(rc_ctx *c, int variant);
+/* wait for the last run; fills st (may be NULL) */
+int rc_sync(rc_ctx *c, rc_stats *st);
+/* per emitting cell of snapshot s (order: i fastest, then j, then local k):
+ * mean over nVar and the reference's sqrt-variance (mc_gpu_noint.cu:234-239) */
+int rc_results(rc_ctx *c, int s, float *mean, float *sdev);
+/* per (emitting cell, run) FP64 tallies of snapshot s, [cell][nVar] */
+int rc_tallies(rc_ctx *c, int s, double *de);
+/* wall results of snapshot s: [bound][k][j] means and sdev, and tallies
+ * [bound][k][j][nVar]; bound 0 = west (i=0), 1 = east (i=imax+1) */
+int rc_surface_results(rc_ctx *c, int s, float *mean, float *sdev, double *de);
+/* trace single rays of snapshot 0: fluid rays given as (i,j,k,v,h) with k
+ * global, surface rays as (bound,j,k,v,h); one record per ray */
+int rc_trace(rc_ctx *c, int surface, const int32_t *rays, int n, rc_ray *out);
+/* scalar set-up values of snapshot s: Tmax, kappamax, Ibmax, Ibmax_w,
+ * Ibmax_e; ints tm, tstar */
+int rc_info(rc_ctx *c, int s, float *f5, int32_t *i2);
+int rc_device_synchronize(void);
+
+#ifdef __cplusplus
+}
+#endif
+#endif

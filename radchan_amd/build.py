@@ -1,0 +1,85 @@
+"""Build the native pieces in-tree.
+
+``build_library()`` compiles the HIP library ``radchan_amd/libradchan.so``
+for gfx950 with hipcc.  ``build_oracle()`` compiles the CPU checker
+(``oracle/liboracle.so``) and, when the RadChan sources are present, the
+reference drivers under ``oracle/_ref``.
+
+Numerics flags: ``-ffp-contract=off`` (no implicit fusion: the FP32 path is
+reproduced bit for bit by the CPU checker), packed FP32 instructions disabled,
+hardware FP64 atomics.
+"""
+from __future__ import annotations
+
+import os
+import shutil
+import subprocess
+import sys
+
+HERE = os.path.dirname(os.path.abspath(__file__))
+ROOT = os.path.dirname(HERE)
+CSRC = os.path.join(HERE, "csrc")
+LIB = os.path.join(HERE, "libradchan.so")
+ORACLE_DIR = os.path.join(ROOT, "oracle")
+
+SOURCES = ["rc_tables.cpp", "rc_capi.cpp", "rc_solver.hip", "rc_batched.hip"]
+HIP_FLAGS = [
+    "--offload-arch=gfx950",
+    "-O3",
+    "-std=c++17",
+    "-fPIC",
+    "-ffp-contract=off",
+    "-munsafe-fp-atomics",
+    "-Xclang", "-target-feature", "-Xclang", "-packed-fp32-ops",
+    "-Wall", "-Wno-unused-function", "-Wno-unused-variable",
+]
+
+
+def _hipcc() -> str:
+    for cand in (os.environ.get("HIPCC"), "/opt/rocm/bin/hipcc", shutil.which("hipcc")):
+        if cand and os.path.exists(cand):
+            return cand
+    raise RuntimeError("hipcc not found: the HIP library cannot be built")
+
+
+def _run(cmd, cwd=None):
+    res = subprocess.run(cmd, cwd=cwd, stdout=subprocess.PIPE, stderr=subprocess.STDOUT, text=True)
+    if res.returncode != 0:
+        sys.stderr.write(res.stdout)
+        raise RuntimeError("build step failed: " + " ".join(cmd))
+    return res.stdout
+
+
+def build_library(verbose: bool = False) -> str:
+    """Compile every source for gfx950 and link ``libradchan.so``."""
+    hipcc = _hipcc()
+    objdir = os.path.join(HERE, "_build")
+    os.makedirs(objdir, exist_ok=True)
+    objs = []
+    for src in SOURCES:
+        obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+        cmd = [hipcc, *HIP_FLAGS, "-c", os.path.join(CSRC, src), "-o", obj]
+        if src.endswith(".hip"):
+            cmd[1:1] = ["-x", "hip"]
+        out = _run(cmd)
+        if verbose and out.strip():
+            print(out)
+        objs.append(obj)
+    tmp = LIB + ".tmp"
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    os.replace(tmp, LIB)
+    return LIB
+
+
+def build_oracle(with_reference: bool = True) -> None:
+    """Compile the CPU checker and, if the RadChan tree is present, its
+    reference drivers (outputs only under oracle/_ref)."""
+    _run(["make", "-C", ORACLE_DIR, "-s", "liboracle.so"])
+    ref = os.environ.get("RADCHAN_REF", "/root/reference")
+    if with_reference and os.path.isdir(os.path.join(ref, "src", "montecarlo")):
+        _run(["make", "-C", ORACLE_DIR, "-s", "ref", f"REF={ref}"])
+
+
+if __name__ == "__main__":
+    build_oracle()
+    print(build_library(verbose=True))

@@ -1,0 +1,730 @@
+/*
+ * rc_capi.cpp -- session state and the C ABI of the radiative solver.
+ *
+ * mc_gpu_ / get_results_ follow src/montecarlo/mc_gpu_noint.cu:660-874 and
+ * :876-1021: copy the temperature field, launch asynchronously, later
+ * synchronise and scatter mean div(q_r) and sqrt(variance) to the Fortran
+ * layout.  The rc_* extensions expose the same machinery as a device-
+ * resident session (tables and grids uploaded once, several snapshots).
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <map>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rc_internal.h"
+#include "rc_kernels.h"
+
+namespace rc {
+void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int nsnap, int surface,
+                    double *de, unsigned long long *stats, int *queue, int wg_per_cu, int rays_per_lane);
+}
+
+using namespace rc;
+
+namespace {
+
+#define RC_HIP(call)                                                                         \
+    do {                                                                                     \
+        hipError_t e_ = (call);                                                              \
+        if (e_ != hipSuccess) {                                                              \
+            set_error(std::string("HIP error: ") + hipGetErrorString(e_) + " at " #call);   \
+            return -10;                                                                      \
+        }                                                                                    \
+    } while (0)
+
+struct DevBuf {
+    void *p = nullptr;
+    size_t bytes = 0;
+    DevBuf() = default;
+    DevBuf(const DevBuf &) = delete;
+    DevBuf &operator=(const DevBuf &) = delete;
+    ~DevBuf() { release(); }
+    void release() {
+        if (p) (void)hipFree(p);
+        p = nullptr; bytes = 0;
+    }
+    int ensure(size_t n) {
+        if (n <= bytes && p) return 0;
+        release();
+        if (n == 0) n = 16;
+        if (hipMalloc(&p, n) != hipSuccess) {
+            p = nullptr;
+            set_error("device allocation of " + std::to_string(n) + " bytes failed");
+            return -11;
+        }
+        bytes = n;
+        return 0;
+    }
+    template <class T> T *as() const { return static_cast<T *>(p); }
+};
+
+template <class T> int upload(DevBuf &b, const std::vector<T> &v, hipStream_t st) {
+    if (int e = b.ensure(v.size() * sizeof(T))) return e;
+    RC_HIP(hipMemcpyAsync(b.p, v.data(), v.size() * sizeof(T), hipMemcpyHostToDevice, st));
+    return 0;
+}
+
+/* device tables in sorted band order for one temperature table */
+struct TableSet {
+    float Tnb0 = 0.f, invdT = 0.f;
+    std::vector<float> Tnb;
+    DevBuf kb, bfix, cdf_gs, rank_of;
+};
+
+struct SnapState {
+    float Tmax = 0.f, kappamax = 0.f, Ibmax = 0.f, Ibmax_w = 0.f, Ibmax_e = 0.f;
+    int tm = 0, tstar = 0;
+    DevBuf T0, cdf_b, cdf_g, cdf_bs;
+    DevBuf lvT[RC_MAXLEV], lvTw[RC_MAXLEV], lvTz[RC_MAXLEV];
+    std::unique_ptr<TableSet> grey;     /* grey runs: tables follow Tmax */
+    Tables greyhost;
+};
+
+}  // namespace
+
+struct rc_ctx {
+    rc_params p{};
+    int kloc = 0, ncell = 0, nsurf = 0, zstart = 1;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    Tables tab;
+    bool have_tables = false;
+    std::vector<int32_t> idx, rank_of;
+    TableSet nbset;
+    std::vector<Grid> grids;
+    DevBuf xu[RC_MAXLEV], x0, y0, z0;
+    std::vector<std::unique_ptr<SnapState>> snaps;
+    DevBuf dsnap;
+    DevBuf de, ds, mean, sdev, smean, ssdev;
+    DevBuf keys, stats, queue;
+    DevBuf tr_rays, tr_unit, tr_ulist, tr_out;
+    rc_stats last{};
+    bool launched = false;
+    int wg_per_cu = 0, rays_per_lane = 0;   /* batched schedule; 0 = default */
+};
+
+namespace {
+
+int build_tableset(rc_ctx *c, const Tables &t, TableSet &ts, std::vector<int32_t> &idx, std::vector<int32_t> &rank_of) {
+    const rc_params &p = c->p;
+    const int nB = t.nB, nT = t.nT, nQ = t.nQ;
+    idx.resize(nB); rank_of.resize(nB);
+    if (p.srt == 1 && !p.grey) sort_idx(t.kavg.data(), nB, idx.data());
+    else for (int b = 0; b < nB; ++b) idx[b] = b;
+    for (int r = 0; r < nB; ++r) rank_of[idx[r]] = r;
+    ts.Tnb = t.Tnb;
+    const float dT = t.Tnb[1] - t.Tnb[0];
+    ts.Tnb0 = t.Tnb[0];
+    ts.invdT = 1.0f / dT;
+    const bool gt = p.grey && p.grey_true;
+    std::vector<float2> kb((size_t)nB * nQ * (nT + 1));
+    std::vector<float> brow(nT + 1);
+    for (int r = 0; r < nB; ++r) {
+        const int b = idx[r];
+        for (int q = 0; q < nT; ++q)
+            brow[q] = gt ? (float)(kStefan * std::pow((double)t.Tnb[q], 4) / kPi) : planck(t.Tnb[q], t.wvc[b]);
+        brow[nT] = brow[nT - 1];
+        for (int g = 0; g < nQ; ++g) {
+            float2 *row = &kb[((size_t)r * nQ + g) * (nT + 1)];
+            for (int q = 0; q < nT; ++q) row[q] = make_float2(t.kq[((size_t)b * nT + q) * nQ + g], brow[q]);
+            row[nT] = row[nT - 1];
+        }
+    }
+    std::vector<float> bfix((size_t)6 * nB);
+    for (int f = 0; f < 6; ++f)
+        for (int r = 0; r < nB; ++r) {
+            const double Tw = p.Tw[f];
+            bfix[(size_t)f * nB + r] = gt ? (float)(kStefan * Tw * Tw * Tw * Tw / kPi) : planck(p.Tw[f], t.wvc[idx[r]]);
+        }
+    std::vector<float> cgs(nQ);
+    {
+        std::vector<float> wq = t.wq;
+        if ((int)wq.size() != nQ) wq = reference_wq(nQ);
+        if ((int)wq.size() != nQ) { set_error("no quadrature weights for nQ"); return -1; }
+        float acc = 0.f;
+        for (int g = 0; g < nQ; ++g) { acc = (g == 0) ? wq[0] : acc + wq[g]; cgs[g] = acc; }
+        for (int g = 0; g < nQ; ++g) cgs[g] = cgs[g] / acc;
+    }
+    if (int e = upload(ts.kb, kb, c->stream)) return e;
+    if (int e = upload(ts.bfix, bfix, c->stream)) return e;
+    if (int e = upload(ts.cdf_gs, cgs, c->stream)) return e;
+    if (int e = upload(ts.rank_of, rank_of, c->stream)) return e;
+    RC_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+/* Tmax, kappamax, Ibmax and the band-pdf row of one field (mc_gpu_noint.cu
+ * :696-710, read_tables.cpp:35-38, mc_gpu_noint.cu:74, :1110-1131) */
+int snapshot_scalars(const rc_params &p, const Tables &t, SnapState &s) {
+    const float dT = t.Tnb[1] - t.Tnb[0];
+    s.tm = (int)((s.Tmax - t.Tnb[0]) / dT);
+    if (!(s.Tmax >= t.Tnb[0]) || s.tm < 0 || s.tm + 1 >= t.nT) {
+        set_error("Tmax " + std::to_string(s.Tmax) + " K lies outside the temperature table [" +
+                  std::to_string(t.Tnb[0]) + ", " + std::to_string(t.Tnb[t.nT - 1]) + "]");
+        return -5;
+    }
+    const int tm = s.tm;
+    if (p.grey) s.kappamax = p.abscoeff;
+    else s.kappamax = (t.kP[tm + 1] - t.kP[tm]) / (t.Tnb[tm + 1] - t.Tnb[tm]) * (s.Tmax - t.Tnb[tm]) + t.kP[tm];
+    const double T4 = (double)s.Tmax * s.Tmax * s.Tmax * s.Tmax;
+    s.Ibmax = (float)(4.0 * (double)s.kappamax * T4 * kStefan / p.photmax);
+    s.Ibmax_w = (float)((double)p.eps[FW] * kStefan * T4 / p.photmax);
+    s.Ibmax_e = (float)((double)p.eps[FE] * kStefan * T4 / p.photmax);
+    s.tstar = ((s.Tmax - t.Tnb[tm]) < (t.Tnb[tm + 1] - s.Tmax)) ? tm : tm + 1;
+    return 0;
+}
+
+int snapshot_rows(rc_ctx *c, const Tables &t, SnapState &s) {
+    const rc_params &p = c->p;
+    const int nB = t.nB, nQ = t.nQ, ts = s.tstar;
+    std::vector<float> cb(t.prob.begin() + (size_t)ts * nB, t.prob.begin() + (size_t)(ts + 1) * nB);
+    std::vector<float> cg(t.probg.begin() + (size_t)ts * nB * nQ, t.probg.begin() + (size_t)(ts + 1) * nB * nQ);
+    /* wall band CDF: readS (read_tables.cpp:188-220) on row t* */
+    std::vector<float> cs(nB);
+    const bool gt = p.grey && p.grey_true;
+    const float Tr = t.Tnb[ts];
+    float acc = 0.f;
+    for (int b = 0; b < nB; ++b) {
+        const float ib = gt ? 1.0f : planck(Tr, t.wvc[b]);
+        acc = (b == 0) ? ib * (t.wvr[0] - t.wvl[0]) * 100.f : acc + ib * (t.wvr[b] - t.wvl[b]) * 100.f;
+        cs[b] = acc;
+    }
+    for (int b = 0; b < nB; ++b) cs[b] = cs[b] / acc;
+    if (int e = upload(s.cdf_b, cb, c->stream)) return e;
+    if (int e = upload(s.cdf_g, cg, c->stream)) return e;
+    if (int e = upload(s.cdf_bs, cs, c->stream)) return e;
+    return 0;
+}
+
+DevCommon make_common(const rc_ctx *c) {
+    const rc_params &p = c->p;
+    DevCommon cm{};
+    cm.imax = p.imax; cm.jmax = p.jmax; cm.kmax = p.kmax; cm.kloc = c->kloc; cm.zstart = c->zstart;
+    cm.nVar = p.nVar; cm.photmax = p.photmax; cm.srt = (p.srt == 1 && !p.grey) ? 1 : 0;
+    cm.nB = p.nB; cm.nQ = p.nQ; cm.nT = p.nT; cm.grid_num = p.grid_num;
+    cm.ncell = c->ncell; cm.nsurf = c->nsurf;
+    cm.Lx = p.Lx; cm.Ly = p.Ly; cm.Lz = p.Lz; cm.toll = p.toll; cm.tollS = p.tollS;
+    for (int f = 0; f < 6; ++f) cm.bd[f] = p.bd[f];
+    cm.adiab[0] = p.adiabw == 1; cm.adiab[1] = p.adiabe == 1;
+    cm.k0 = (uint32_t)p.seed; cm.k1 = (uint32_t)(p.seed >> 32);
+    return cm;
+}
+
+/* push the DevSnap descriptors */
+int push_snaps(rc_ctx *c) {
+    const rc_params &p = c->p;
+    std::vector<DevSnap> v(c->snaps.size());
+    for (size_t s = 0; s < c->snaps.size(); ++s) {
+        SnapState &S = *c->snaps[s];
+        const TableSet &ts = S.grey ? *S.grey : c->nbset;
+        DevSnap d{};
+        d.Tmax = S.Tmax; d.Ibmax = S.Ibmax; d.Ibmax_w = S.Ibmax_w; d.Ibmax_e = S.Ibmax_e;
+        d.Tnb0 = ts.Tnb0; d.invdT = ts.invdT; d.tstar = S.tstar; d.snap = (int)s;
+        d.kb = ts.kb.as<float2>(); d.bfix = ts.bfix.as<float>();
+        d.cdf_b = S.cdf_b.as<float>(); d.cdf_g = S.cdf_g.as<float>(); d.cdf_bs = S.cdf_bs.as<float>();
+        d.cdf_gs = ts.cdf_gs.as<float>(); d.rank_of = ts.rank_of.as<int>();
+        d.T0 = S.T0.as<float>();
+        d.x0 = c->x0.as<float>(); d.y0 = c->y0.as<float>(); d.z0 = c->z0.as<float>();
+        for (int g = 0; g < p.grid_num; ++g) {
+            DevLevel &L = d.lev[g];
+            L.im = p.maxi[g]; L.jm = p.maxj[g]; L.km = p.maxk[g]; L.maxs = p.maxs[g];
+            L.dy = p.Ly * 1.0f / (float)L.jm;
+            L.dz = p.Lz * 1.0f / (float)L.km;
+            L.xu = c->xu[g].as<float>();
+            L.T = S.lvT[g].as<float>();
+            L.Tw = S.lvTw[g].p ? S.lvTw[g].as<float>() : nullptr;
+            L.Tz = S.lvTz[g].p ? S.lvTz[g].as<float>() : nullptr;
+        }
+        v[s] = d;
+    }
+    return upload(c->dsnap, v, c->stream);
+}
+
+}  // namespace
+
+/* ========================================================================= */
+extern "C" {
+
+void rc_default_params(rc_params *p) { default_params(*p); }
+const char *rc_last_error(void) { return last_error(); }
+const char *rc_version(void) { return "radchan-mi355x 0.1 (gfx950)"; }
+
+int rc_read_tables(const char *dir, double chanHeight, rc_tables *t) {
+    Tables tt;
+    tt.resize(t->nB, t->nT, t->nQ);
+    if (int e = read_tables(dir, chanHeight, tt)) return e;
+    std::memcpy(t->Tnb, tt.Tnb.data(), tt.Tnb.size() * 4);
+    std::memcpy(t->kP, tt.kP.data(), tt.kP.size() * 4);
+    std::memcpy(t->wvl, tt.wvl.data(), tt.wvl.size() * 4);
+    std::memcpy(t->wvr, tt.wvr.data(), tt.wvr.size() * 4);
+    std::memcpy(t->wvc, tt.wvc.data(), tt.wvc.size() * 4);
+    std::memcpy(t->kavg, tt.kavg.data(), tt.kavg.size() * 4);
+    std::memcpy(t->kq, tt.kq.data(), tt.kq.size() * 4);
+    std::memcpy(t->prob, tt.prob.data(), tt.prob.size() * 4);
+    std::memcpy(t->probg, tt.probg.data(), tt.probg.size() * 4);
+    return 0;
+}
+
+static void copy_in(Tables &tt, const rc_tables *t) {
+    tt.resize(t->nB, t->nT, t->nQ);
+    std::memcpy(tt.Tnb.data(), t->Tnb, tt.Tnb.size() * 4);
+    std::memcpy(tt.kP.data(), t->kP, tt.kP.size() * 4);
+    std::memcpy(tt.wvl.data(), t->wvl, tt.wvl.size() * 4);
+    std::memcpy(tt.wvr.data(), t->wvr, tt.wvr.size() * 4);
+    std::memcpy(tt.wvc.data(), t->wvc, tt.wvc.size() * 4);
+    std::memcpy(tt.kavg.data(), t->kavg, tt.kavg.size() * 4);
+    std::memcpy(tt.kq.data(), t->kq, tt.kq.size() * 4);
+    std::memcpy(tt.prob.data(), t->prob, tt.prob.size() * 4);
+    std::memcpy(tt.probg.data(), t->probg, tt.probg.size() * 4);
+    if (t->wq) std::memcpy(tt.wq.data(), t->wq, tt.wq.size() * 4);
+    else tt.wq = reference_wq(t->nQ);
+}
+
+int rc_write_tables(const char *dir, const rc_tables *t, double chanHeight) {
+    Tables tt;
+    copy_in(tt, t);
+    return write_tables(dir, tt, chanHeight);
+}
+
+int rc_grey_tables(const rc_params *p, float Tmax, rc_tables *t) {
+    Tables tt;
+    tt.resize(t->nB, t->nT, t->nQ);
+    if (int e = grey_tables(*p, Tmax, tt)) return e;
+    std::memcpy(t->Tnb, tt.Tnb.data(), tt.Tnb.size() * 4);
+    std::memcpy(t->kP, tt.kP.data(), tt.kP.size() * 4);
+    std::memcpy(t->wvl, tt.wvl.data(), tt.wvl.size() * 4);
+    std::memcpy(t->wvr, tt.wvr.data(), tt.wvr.size() * 4);
+    std::memcpy(t->wvc, tt.wvc.data(), tt.wvc.size() * 4);
+    std::memcpy(t->kavg, tt.kavg.data(), tt.kavg.size() * 4);
+    std::memcpy(t->kq, tt.kq.data(), tt.kq.size() * 4);
+    std::memcpy(t->prob, tt.prob.data(), tt.prob.size() * 4);
+    std::memcpy(t->probg, tt.probg.data(), tt.probg.size() * 4);
+    if (t->wq) t->wq[0] = 1.0f;
+    return 0;
+}
+
+int rc_sort_idx(const float *kavg, int nB, int32_t *idx) {
+    if (nB < 1) { set_error("nB must be positive"); return -1; }
+    sort_idx(kavg, nB, idx);
+    return 0;
+}
+
+int rc_grid(const rc_params *p, int level, float *x, float *xu, float *y, float *yu, float *z, float *zu) {
+    if (level < 0 || level >= p->grid_num) { set_error("level out of range"); return -1; }
+    Grid g;
+    build_grid(*p, level, g);
+    std::memcpy(x, g.x.data(), g.x.size() * 4);
+    std::memcpy(xu, g.xu.data(), g.xu.size() * 4);
+    std::memcpy(y, g.y.data(), g.y.size() * 4);
+    std::memcpy(yu, g.yu.data(), g.yu.size() * 4);
+    std::memcpy(z, g.z.data(), g.z.size() * 4);
+    std::memcpy(zu, g.zu.data(), g.zu.size() * 4);
+    return 0;
+}
+
+int rc_device_synchronize(void) {
+    RC_HIP(hipDeviceSynchronize());
+    return 0;
+}
+
+rc_ctx *rc_create(const rc_params *pp) {
+    rc_params p = *pp;
+    if (p.grey) p.srt = 0;                              /* NarrowBand.h:220-221 */
+    if (check_params(p)) return nullptr;
+    int ndev = 0;
+    if (hipGetDeviceCount(&ndev) != hipSuccess || ndev < 1) {
+        set_error("no HIP device available");
+        return nullptr;
+    }
+    auto c = std::make_unique<rc_ctx>();
+    c->p = p;
+    c->kloc = p.kmax / p.p_col;
+    c->ncell = p.imax * p.jmax * c->kloc;
+    c->nsurf = p.jmax * c->kloc;
+    if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        set_error("cannot create HIP stream/events");
+        return nullptr;
+    }
+    c->grids.resize(p.grid_num);
+    for (int g = 0; g < p.grid_num; ++g) {
+        build_grid(p, g, c->grids[g]);
+        if (upload(c->xu[g], c->grids[g].xu, c->stream)) return nullptr;
+    }
+    if (upload(c->x0, c->grids[0].x, c->stream) || upload(c->y0, c->grids[0].y, c->stream) ||
+        upload(c->z0, c->grids[0].z, c->stream))
+        return nullptr;
+    if (c->stats.ensure(8 * sizeof(unsigned long long)) || c->queue.ensure(64 * sizeof(int))) return nullptr;
+    if (hipStreamSynchronize(c->stream) != hipSuccess) { set_error("stream synchronize failed"); return nullptr; }
+    return c.release();
+}
+
+void rc_destroy(rc_ctx *c) {
+    if (!c) return;
+    if (c->stream) (void)hipStreamSynchronize(c->stream);
+    if (c->ev0) (void)hipEventDestroy(c->ev0);
+    if (c->ev1) (void)hipEventDestroy(c->ev1);
+    hipStream_t st = c->stream;
+    delete c;
+    if (st) (void)hipStreamDestroy(st);
+}
+
+int rc_load_tables(rc_ctx *c, const rc_tables *t) {
+    if (!c || !t) { set_error("null argument"); return -1; }
+    if (c->p.grey) return 0;
+    if (t->nB != c->p.nB || t->nT != c->p.nT || t->nQ != c->p.nQ) {
+        set_error("table sizes differ from the parameters (nB, nT, nQ)");
+        return -2;
+    }
+    copy_in(c->tab, t);
+    for (int q = 1; q < c->tab.nT; ++q)
+        if (!(c->tab.Tnb[q] > c->tab.Tnb[q - 1])) { set_error("temperature table is not increasing"); return -3; }
+    if (int e = build_tableset(c, c->tab, c->nbset, c->idx, c->rank_of)) return e;
+    c->have_tables = true;
+    return 0;
+}
+
+int rc_set_fields(rc_ctx *c, int nsnap, const float *const *T, int zstart) {
+    if (!c || nsnap < 1 || !T) { set_error("bad arguments to rc_set_fields"); return -1; }
+    const rc_params &p = c->p;
+    if (zstart < 1 || zstart + c->kloc - 1 > p.kmax) { set_error("zstart outside 1..kmax-kmax/p_col+1"); return -2; }
+    if (!p.grey && !c->have_tables) { set_error("narrow-band run without tables (rc_load_tables)"); return -3; }
+    if (c->launched) { (void)hipStreamSynchronize(c->stream); c->launched = false; }
+    c->zstart = zstart;
+    const size_t nT0 = (size_t)(p.imax + 2) * (p.jmax + 2) * (p.kmax + 2);
+    c->snaps.resize(nsnap);
+    for (int s = 0; s < nsnap; ++s) {
+        if (!c->snaps[s]) c->snaps[s] = std::make_unique<SnapState>();
+        SnapState &S = *c->snaps[s];
+        for (size_t q = 0; q < nT0; ++q)
+            if (!std::isfinite(T[s][q]) || !(T[s][q] > 0.f)) {
+                set_error("temperature field " + std::to_string(s) + " holds a non-positive or non-finite value");
+                return -4;
+            }
+        S.Tmax = field_tmax(p, T[s]);
+        const Tables *tb = &c->tab;
+        if (p.grey) {
+            S.greyhost.resize(p.nB, p.nT, 1);
+            if (int e = grey_tables(p, S.Tmax, S.greyhost)) return e;
+            if (!S.grey) S.grey = std::make_unique<TableSet>();
+            std::vector<int32_t> idx, ro;
+            if (int e = build_tableset(c, S.greyhost, *S.grey, idx, ro)) return e;
+            tb = &S.greyhost;
+        }
+        if (int e = snapshot_scalars(p, *tb, S)) return e;
+        if (int e = snapshot_rows(c, *tb, S)) return e;
+        if (int e = S.T0.ensure(nT0 * 4)) return e;
+        RC_HIP(hipMemcpyAsync(S.T0.p, T[s], nT0 * 4, hipMemcpyHostToDevice, c->stream));
+        for (int g = 0; g < p.grid_num; ++g) {
+            const size_t nl = (size_t)p.maxi[g] * p.maxj[g] * p.maxk[g];
+            if (int e = S.lvT[g].ensure(nl * 4)) return e;
+            const bool wx = (p.bd[FW] == 2 && p.adiabw == 1) || (p.bd[FE] == 2 && p.adiabe == 1);
+            const bool wz = p.bd[FB] == 4 || p.bd[FT] == 4;
+            if (wx) { if (int e = S.lvTw[g].ensure((size_t)2 * p.maxj[g] * p.maxk[g] * 4)) return e; }
+            if (wz) { if (int e = S.lvTz[g].ensure((size_t)2 * p.maxi[g] * p.maxj[g] * 4)) return e; }
+            launch_level_T(c->stream, S.T0.as<float>(), p, g, S.lvT[g].as<float>(),
+                           wx ? S.lvTw[g].as<float>() : nullptr, wz ? S.lvTz[g].as<float>() : nullptr);
+        }
+        RC_HIP(hipGetLastError());
+    }
+    if (int e = push_snaps(c)) return e;
+    const size_t nde = (size_t)nsnap * c->ncell * p.nVar;
+    const size_t nds = (size_t)nsnap * 2 * c->nsurf * p.nVar;
+    if (int e = c->de.ensure(nde * 8)) return e;
+    if (int e = c->ds.ensure(nds * 8)) return e;
+    if (int e = c->mean.ensure((size_t)nsnap * c->ncell * 4)) return e;
+    if (int e = c->sdev.ensure((size_t)nsnap * c->ncell * 4)) return e;
+    if (int e = c->smean.ensure((size_t)nsnap * 2 * c->nsurf * 4)) return e;
+    if (int e = c->ssdev.ensure((size_t)nsnap * 2 * c->nsurf * 4)) return e;
+    RC_HIP(hipMemsetAsync(c->ds.p, 0, nds * 8, c->stream));
+    RC_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+int rc_run(rc_ctx *c, int variant) {
+    if (!c || c->snaps.empty()) { set_error("rc_run before rc_set_fields"); return -1; }
+    if (variant != RC_VARIANT_PLAIN && variant != RC_VARIANT_BATCHED) { set_error("unknown variant"); return -2; }
+    const rc_params &p = c->p;
+    const DevCommon cm = make_common(c);
+    const int nsnap = (int)c->snaps.size();
+    const DevSnap *ds = c->dsnap.as<DevSnap>();
+    const bool walls = p.calcwalls == 1 && (p.adiabw == 1 || p.adiabe == 1);
+    int launches = 0;
+    RC_HIP(hipMemsetAsync(c->stats.p, 0, 8 * sizeof(unsigned long long), c->stream));
+    RC_HIP(hipEventRecord(c->ev0, c->stream));
+    if (variant == RC_VARIANT_PLAIN) {
+        if (cm.srt) {
+            const size_t nk = (size_t)std::max(c->ncell, 2 * c->nsurf) * p.photmax;
+            if (int e = c->keys.ensure(nk * 2)) return e;
+        }
+        for (int s = 0; s < nsnap; ++s) {
+            double *de = c->de.as<double>() + (size_t)s * c->ncell * p.nVar;
+            for (int v = 0; v < p.nVar; ++v) {
+                if (cm.srt) { launch_sort(c->stream, cm, ds, s, v, 0, c->ncell, nullptr, c->keys.as<uint16_t>()); ++launches; }
+                launch_fluid_plain(c->stream, cm, ds, s, v, cm.srt ? c->keys.as<uint16_t>() : nullptr, de,
+                                   c->stats.as<unsigned long long>());
+                ++launches;
+            }
+            if (walls) {
+                double *dw = c->ds.as<double>() + (size_t)s * 2 * c->nsurf * p.nVar;
+                for (int v = 0; v < p.nVar; ++v) {
+                    if (cm.srt) { launch_sort(c->stream, cm, ds, s, v, 1, 2 * c->nsurf, nullptr, c->keys.as<uint16_t>()); ++launches; }
+                    launch_surface_plain(c->stream, cm, ds, s, v, cm.srt ? c->keys.as<uint16_t>() : nullptr, dw,
+                                         c->stats.as<unsigned long long>());
+                    ++launches;
+                }
+            }
+        }
+    } else {
+        launch_batched(c->stream, cm, ds, nsnap, 0, c->de.as<double>(), c->stats.as<unsigned long long>(),
+                       c->queue.as<int>(), c->wg_per_cu, c->rays_per_lane);
+        ++launches;
+        if (walls) {
+            launch_batched(c->stream, cm, ds, nsnap, 1, c->ds.as<double>(), c->stats.as<unsigned long long>(),
+                           c->queue.as<int>() + 16, c->wg_per_cu, c->rays_per_lane);
+            ++launches;
+        }
+    }
+    for (int s = 0; s < nsnap; ++s) {
+        launch_finalize(c->stream, c->de.as<double>() + (size_t)s * c->ncell * p.nVar, c->ncell, p.nVar,
+                        c->mean.as<float>() + (size_t)s * c->ncell, c->sdev.as<float>() + (size_t)s * c->ncell);
+        ++launches;
+        if (walls) {
+            launch_finalize(c->stream, c->ds.as<double>() + (size_t)s * 2 * c->nsurf * p.nVar, 2 * c->nsurf, p.nVar,
+                            c->smean.as<float>() + (size_t)s * 2 * c->nsurf,
+                            c->ssdev.as<float>() + (size_t)s * 2 * c->nsurf);
+            ++launches;
+        }
+    }
+    RC_HIP(hipGetLastError());
+    RC_HIP(hipEventRecord(c->ev1, c->stream));
+    c->last = rc_stats{};
+    c->last.launches = launches;
+    c->last.variant = variant;
+    c->launched = true;
+    return 0;
+}
+
+int rc_sync(rc_ctx *c, rc_stats *st) {
+    if (!c) { set_error("null context"); return -1; }
+    if (c->launched) {
+        RC_HIP(hipStreamSynchronize(c->stream));
+        float ms = 0.f;
+        RC_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        unsigned long long h[8];
+        RC_HIP(hipMemcpy(h, c->stats.p, sizeof h, hipMemcpyDeviceToHost));
+        c->last.kernel_ms = ms;
+        c->last.march_ms = ms;
+        c->last.rays = (int64_t)h[0];
+        c->last.steps = (int64_t)h[1];
+        c->last.capped = (int64_t)h[2];
+        c->launched = false;
+    }
+    if (st) *st = c->last;
+    return 0;
+}
+
+int rc_results(rc_ctx *c, int s, float *mean, float *sdev) {
+    if (!c || s < 0 || s >= (int)c->snaps.size()) { set_error("bad snapshot index"); return -1; }
+    if (int e = rc_sync(c, nullptr)) return e;
+    if (mean) RC_HIP(hipMemcpy(mean, c->mean.as<float>() + (size_t)s * c->ncell, (size_t)c->ncell * 4, hipMemcpyDeviceToHost));
+    if (sdev) RC_HIP(hipMemcpy(sdev, c->sdev.as<float>() + (size_t)s * c->ncell, (size_t)c->ncell * 4, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int rc_tallies(rc_ctx *c, int s, double *de) {
+    if (!c || s < 0 || s >= (int)c->snaps.size()) { set_error("bad snapshot index"); return -1; }
+    if (int e = rc_sync(c, nullptr)) return e;
+    const size_t n = (size_t)c->ncell * c->p.nVar;
+    RC_HIP(hipMemcpy(de, c->de.as<double>() + (size_t)s * n, n * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int rc_surface_results(rc_ctx *c, int s, float *mean, float *sdev, double *de) {
+    if (!c || s < 0 || s >= (int)c->snaps.size()) { set_error("bad snapshot index"); return -1; }
+    if (int e = rc_sync(c, nullptr)) return e;
+    const size_t n = (size_t)2 * c->nsurf;
+    if (mean) RC_HIP(hipMemcpy(mean, c->smean.as<float>() + (size_t)s * n, n * 4, hipMemcpyDeviceToHost));
+    if (sdev) RC_HIP(hipMemcpy(sdev, c->ssdev.as<float>() + (size_t)s * n, n * 4, hipMemcpyDeviceToHost));
+    if (de) RC_HIP(hipMemcpy(de, c->ds.as<double>() + (size_t)s * n * c->p.nVar, n * c->p.nVar * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int rc_info(rc_ctx *c, int s, float *f5, int32_t *i2) {
+    if (!c || s < 0 || s >= (int)c->snaps.size()) { set_error("bad snapshot index"); return -1; }
+    const SnapState &S = *c->snaps[s];
+    if (f5) { f5[0] = S.Tmax; f5[1] = S.kappamax; f5[2] = S.Ibmax; f5[3] = S.Ibmax_w; f5[4] = S.Ibmax_e; }
+    if (i2) { i2[0] = S.tm; i2[1] = S.tstar; }
+    return 0;
+}
+
+int rc_trace(rc_ctx *c, int surface, const int32_t *rays, int n, rc_ray *out) {
+    if (!c || c->snaps.empty() || n < 1) { set_error("rc_trace needs a loaded field and rays"); return -1; }
+    if (int e = rc_sync(c, nullptr)) return e;
+    const rc_params &p = c->p;
+    const DevCommon cm = make_common(c);
+    for (int q = 0; q < n; ++q) {
+        const int a = rays[5 * q], j = rays[5 * q + 1], k = rays[5 * q + 2], v = rays[5 * q + 3], h = rays[5 * q + 4];
+        const bool ok = (surface ? (a == 0 || a == 1) : (a >= 1 && a <= p.imax)) && j >= 1 && j <= p.jmax &&
+                        k >= 1 && k <= p.kmax && v >= 0 && v < p.nVar && h >= 0 && h < p.photmax;
+        if (!ok) { set_error("traced ray out of range"); return -2; }
+    }
+    /* units (cell id, run, tag) whose sorted band lists are needed */
+    std::vector<int32_t> ulist, unit(n);
+    std::map<std::tuple<int, int, int>, int> seen;
+    for (int q = 0; q < n; ++q) {
+        const int a = rays[5 * q], j = rays[5 * q + 1], k = rays[5 * q + 2], v = rays[5 * q + 3];
+        const int cid = surface ? (j - 1) + p.jmax * (k - 1) : (a - 1) + p.imax * ((j - 1) + p.jmax * (k - 1));
+        const int tag = surface ? (a ? TAG_EAST : TAG_WEST) : TAG_FLUID;
+        auto key = std::make_tuple(cid, v, tag);
+        auto it = seen.find(key);
+        if (it == seen.end()) {
+            const int u = (int)seen.size();
+            seen[key] = u;
+            ulist.push_back(cid); ulist.push_back(v); ulist.push_back(tag);
+            unit[q] = u;
+        } else unit[q] = it->second;
+    }
+    std::vector<int32_t> rv(rays, rays + 5 * (size_t)n);
+    if (int e = upload(c->tr_rays, rv, c->stream)) return e;
+    if (int e = upload(c->tr_unit, unit, c->stream)) return e;
+    if (int e = upload(c->tr_ulist, ulist, c->stream)) return e;
+    const int nunit = (int)seen.size();
+    uint16_t *keys = nullptr;
+    if (cm.srt) {
+        if (int e = c->keys.ensure((size_t)nunit * p.photmax * 2)) return e;
+        keys = c->keys.as<uint16_t>();
+        launch_sort(c->stream, cm, c->dsnap.as<DevSnap>(), 0, 0, surface, nunit, c->tr_ulist.as<int32_t>(), keys);
+    }
+    if (int e = c->tr_out.ensure((size_t)n * sizeof(rc_ray))) return e;
+    launch_trace(c->stream, cm, c->dsnap.as<DevSnap>(), surface, c->tr_rays.as<int32_t>(), c->tr_unit.as<int32_t>(), n,
+                 keys, c->tr_out.as<rc_ray>());
+    RC_HIP(hipGetLastError());
+    RC_HIP(hipMemcpyAsync(out, c->tr_out.p, (size_t)n * sizeof(rc_ray), hipMemcpyDeviceToHost, c->stream));
+    RC_HIP(hipStreamSynchronize(c->stream));
+    return 0;
+}
+
+/* ------------------------------------------------------------------------- */
+/* reference entry points                                                     */
+
+static std::mutex g_mu;
+static bool g_params_set = false;
+static rc_params g_params;
+static std::string g_dir = "properties";
+static int g_variant = RC_VARIANT_BATCHED;
+static rc_ctx *g_ctx = nullptr;
+static bool g_pending = false;
+static std::string g_tab_dir;
+
+static rc_params &gparams() {
+    if (!g_params_set) { default_params(g_params); g_params_set = true; }
+    return g_params;
+}
+
+int rc_set_params(const rc_params *p) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    rc_params q = *p;
+    if (q.grey) q.srt = 0;
+    if (check_params(q)) return -1;
+    gparams() = q;
+    g_params_set = true;
+    if (g_ctx && !g_pending) { rc_destroy(g_ctx); g_ctx = nullptr; g_tab_dir.clear(); }
+    return 0;
+}
+
+int rc_get_params(rc_params *p) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    *p = gparams();
+    return 0;
+}
+
+int rc_set_table_dir(const char *dir) {
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_dir = dir ? dir : "properties";
+    g_tab_dir.clear();
+    return 0;
+}
+
+int rc_set_variant(int variant) {
+    if (variant != RC_VARIANT_PLAIN && variant != RC_VARIANT_BATCHED) { set_error("unknown variant"); return -1; }
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_variant = variant;
+    return 0;
+}
+
+static void report(const char *where) { std::fprintf(stderr, "radchan %s: %s\n", where, last_error()); }
+
+void mc_gpu_(float *Tfort, int *zstart, int *rank) {
+    (void)rank;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!Tfort || !zstart) { set_error("null argument"); report("mc_gpu_"); return; }
+    if (g_pending && g_ctx) {
+        /* a launch was never harvested: finish it and drop its results */
+        (void)rc_sync(g_ctx, nullptr);
+        g_pending = false;
+        std::fprintf(stderr, "radchan mc_gpu_: previous solve was not harvested by get_results_; discarded\n");
+    }
+    const rc_params &p = gparams();
+    if (g_ctx && std::memcmp(&g_ctx->p, &p, sizeof p) != 0) { rc_destroy(g_ctx); g_ctx = nullptr; g_tab_dir.clear(); }
+    if (!g_ctx) {
+        g_ctx = rc_create(&p);
+        if (!g_ctx) { report("mc_gpu_"); return; }
+    }
+    if (!p.grey && g_tab_dir != g_dir) {
+        Tables t;
+        t.resize(p.nB, p.nT, p.nQ);
+        if (read_tables(g_dir, p.chanHeight, t)) { report("mc_gpu_"); return; }
+        t.wq = reference_wq(p.nQ);
+        rc_tables v = t.view();
+        if (rc_load_tables(g_ctx, &v)) { report("mc_gpu_"); return; }
+        g_tab_dir = g_dir;
+    }
+    const float *T = Tfort;
+    if (rc_set_fields(g_ctx, 1, &T, *zstart)) { report("mc_gpu_"); return; }
+    if (rc_run(g_ctx, g_variant)) { report("mc_gpu_"); return; }
+    g_pending = true;
+}
+
+void get_results_(float *resfort, float *varfort, int *rank) {
+    (void)rank;
+    std::lock_guard<std::mutex> lk(g_mu);
+    if (!g_pending || !g_ctx) { set_error("get_results_ called without a preceding mc_gpu_"); report("get_results_"); return; }
+    if (!resfort || !varfort) { set_error("null argument"); report("get_results_"); return; }
+    rc_ctx *c = g_ctx;
+    const rc_params &p = c->p;
+    std::vector<float> m((size_t)c->ncell), sd((size_t)c->ncell);
+    if (rc_results(c, 0, m.data(), sd.data())) { report("get_results_"); return; }
+    g_pending = false;
+    const size_t nx = (size_t)p.imax + 2, ny = (size_t)p.jmax + 2;
+    auto F = [&](int i, int j, int kl) { return (size_t)i + nx * ((size_t)j + ny * (size_t)kl); };
+    for (int kl = 1; kl <= c->kloc; ++kl)
+        for (int j = 1; j <= p.jmax; ++j)
+            for (int i = 1; i <= p.imax; ++i) {
+                const size_t n = (size_t)(i - 1) + (size_t)p.imax * ((size_t)(j - 1) + (size_t)p.jmax * (kl - 1));
+                resfort[F(i, j, kl)] = m[n];
+                varfort[F(i, j, kl)] = sd[n];
+            }
+    if (p.adiabw == 1 || p.adiabe == 1) {
+        std::vector<float> wm((size_t)2 * c->nsurf, 0.f), ws((size_t)2 * c->nsurf, 0.f);
+        if (p.calcwalls == 1 && rc_surface_results(c, 0, wm.data(), ws.data(), nullptr)) { report("get_results_"); return; }
+        for (int kl = 1; kl <= c->kloc; ++kl)
+            for (int j = 1; j <= p.jmax; ++j) {
+                const size_t n = (size_t)(j - 1) + (size_t)p.jmax * (kl - 1);
+                resfort[F(0, j, kl)] = wm[n];
+                varfort[F(0, j, kl)] = ws[n];
+                resfort[F(p.imax + 1, j, kl)] = wm[c->nsurf + n];
+                varfort[F(p.imax + 1, j, kl)] = ws[c->nsurf + n];
+            }
+    }
+}
+
+}  // extern "C"

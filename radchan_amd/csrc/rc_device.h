@@ -1,0 +1,148 @@
+/*
+ * rc_device.h -- device arithmetic of the radiative solver (gfx950).
+ *
+ * The operation sequence of every function here is part of the library's
+ * numerics contract: FP32 with no implicit contraction (built with
+ * -ffp-contract=off), explicit fmaf, IEEE division and sqrt, and no
+ * hardware transcendental instructions, so that a CPU restatement with the
+ * same sequence reproduces every ray bit for bit.
+ */
+#ifndef RC_DEVICE_H
+#define RC_DEVICE_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+namespace rcd {
+
+/* ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers:
+ * as easy as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies with
+ * a Weyl-sequence key schedule. ---------------------------------------- */
+__device__ __forceinline__ void philox10(uint32_t &c0, uint32_t &c1, uint32_t &c2, uint32_t &c3,
+                                         uint32_t k0, uint32_t k1) {
+#pragma unroll
+    for (int r = 0; r < 10; ++r) {
+        const uint32_t lo0 = 0xD2511F53u * c0, hi0 = __umulhi(0xD2511F53u, c0);
+        const uint32_t lo1 = 0xCD9E8D57u * c2, hi1 = __umulhi(0xCD9E8D57u, c2);
+        const uint32_t n0 = hi1 ^ c1 ^ k0;
+        const uint32_t n2 = hi0 ^ c3 ^ k1;
+        c0 = n0; c1 = lo1; c2 = n2; c3 = lo0;
+        k0 += 0x9E3779B9u;
+        k1 += 0xBB67AE85u;
+    }
+}
+
+/* uniform on (0,1], multiple of 2^-24 */
+__device__ __forceinline__ float u01(uint32_t x) { return (float)((x >> 8) + 1u) * 0x1p-24f; }
+
+struct U4 { float a, b, c, d; };
+
+/* the four uniforms of ray h of run v of cell `cell` in stream `tag` */
+__device__ __forceinline__ U4 draw4(uint32_t k0, uint32_t k1, uint32_t h, uint32_t v, uint32_t cell, uint32_t tag) {
+    uint32_t c0 = h, c1 = v, c2 = cell, c3 = tag;
+    philox10(c0, c1, c2, c3, k0, k1);
+    U4 u;
+    u.a = u01(c0); u.b = u01(c1); u.c = u01(c2); u.d = u01(c3);
+    return u;
+}
+
+/* sin and cos of 2 pi u, u in (0,1]: exact quadrant split of 4u, octant
+ * fold, Taylor polynomials (degree 9 / 10) on [0, pi/4]. */
+__device__ __forceinline__ void sincos2pi(float u, float &s, float &c) {
+    const float x = u * 4.0f;
+    const int q = (int)x;
+    float f = x - (float)q;
+    const bool fold = f > 0.5f;
+    if (fold) f = 1.0f - f;
+    const float y = f * 1.57079637f;
+    const float y2 = y * y;
+    float ps = fmaf(y2, (float)(1.0 / 362880.0), (float)(-1.0 / 5040.0));
+    ps = fmaf(y2, ps, (float)(1.0 / 120.0));
+    ps = fmaf(y2, ps, (float)(-1.0 / 6.0));
+    const float sp = fmaf(y * y2, ps, y);
+    float pc = fmaf(y2, (float)(-1.0 / 3628800.0), (float)(1.0 / 40320.0));
+    pc = fmaf(y2, pc, (float)(-1.0 / 720.0));
+    pc = fmaf(y2, pc, (float)(1.0 / 24.0));
+    pc = fmaf(y2, pc, -0.5f);
+    const float cp = fmaf(y2, pc, 1.0f);
+    const float a = fold ? cp : sp;
+    const float b = fold ? sp : cp;
+    switch (q & 3) {
+        case 0: s = a;  c = b;  break;
+        case 1: s = b;  c = -a; break;
+        case 2: s = -a; c = -b; break;
+        default: s = -b; c = a; break;
+    }
+}
+
+/* expm1(x) and exp(x): x = n ln2 + r with |r| <= ln2/2 (Cody-Waite, fma),
+ * expm1(r) from its Taylor series through r^9/9!, then rescaled by 2^n. */
+__device__ __forceinline__ void expm1_exp(float x, float &em1, float &e) {
+    if (x < -87.0f) { em1 = -1.0f; e = 0.0f; return; }
+    if (x > 88.0f) x = 88.0f;
+    const float n = rintf(x * 1.44269502f);
+    float r = fmaf(-n, 0.693145752f, x);
+    r = fmaf(-n, 1.42860677e-6f, r);
+    float p = fmaf(r, (float)(1.0 / 362880.0), (float)(1.0 / 40320.0));
+    p = fmaf(r, p, (float)(1.0 / 5040.0));
+    p = fmaf(r, p, (float)(1.0 / 720.0));
+    p = fmaf(r, p, (float)(1.0 / 120.0));
+    p = fmaf(r, p, (float)(1.0 / 24.0));
+    p = fmaf(r, p, (float)(1.0 / 6.0));
+    p = fmaf(r, p, 0.5f);
+    p = fmaf(r * r, p, r);
+    const int ni = (int)n;
+    const float sc = __int_as_float((ni + 127) << 23);
+    e = fmaf(sc, p, sc);
+    em1 = (ni == 0) ? p : fmaf(sc, p, sc - 1.0f);
+}
+
+/* first index with cdf[i] > R; R == 1 and past-the-end map to n-1 */
+__device__ __forceinline__ int upper_bound(const float *__restrict__ cdf, int n, float R) {
+    int lo = 0, hi = n;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (R < cdf[mid]) hi = mid; else lo = mid + 1;
+    }
+    if (R == 1.0f) lo = n - 1;
+    if (lo > n - 1) lo = n - 1;
+    return lo;
+}
+
+__device__ __forceinline__ float lerp(float a, float b, float w) { return fmaf(w, b - a, a); }
+
+/* row t and weight of temperature T in a table starting at Tnb0 with
+ * spacing 1/invdT, clamped to the first and last rows */
+__device__ __forceinline__ void tpos(float T, float Tnb0, float invdT, int nT, int &t, float &a) {
+    const float pos = (T - Tnb0) * invdT;
+    if (!(pos > 0.0f)) { t = 0; a = 0.0f; return; }
+    if (pos >= (float)(nT - 1)) { t = nT - 1; a = 0.0f; return; }
+    t = (int)pos;
+    a = pos - (float)t;
+}
+
+/* isotropic direction from (u0, u1): phi = 2 pi u0, cos theta = 1 - 2 u1 */
+__device__ __forceinline__ void iso_dir(float u0, float u1, float &sx, float &sy, float &sz) {
+    const float ct = 1.0f - 2.0f * u1;
+    const float st = 2.0f * sqrtf(u1 * (1.0f - u1));
+    float sp, cp;
+    sincos2pi(u0, sp, cp);
+    sx = ct; sy = st * cp; sz = st * sp;
+}
+
+/* Lambertian direction into the domain from the west (bound 0) or east
+ * (bound 1) wall: sin^2 theta = u1 */
+__device__ __forceinline__ void lamb_dir(float u0, float u1, int bound, float &sx, float &sy, float &sz) {
+    const float a = sqrtf(1.0f - u1);
+    const float b = sqrtf(u1);
+    float sp, cp;
+    sincos2pi(u0, sp, cp);
+    if (bound == 0) { sx = a;  sy = b * cp; sz = b * sp; }
+    else            { sx = -b; sy = a * cp; sz = a * sp; }
+}
+
+__device__ __forceinline__ uint32_t fnv(uint32_t h, uint32_t w) { return (h ^ w) * 16777619u; }
+
+}  // namespace rcd
+
+#endif

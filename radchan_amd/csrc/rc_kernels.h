@@ -1,0 +1,70 @@
+/*
+ * rc_kernels.h -- device-side descriptors shared by the kernels and the
+ * session code, and the kernel launchers.
+ */
+#ifndef RC_KERNELS_H
+#define RC_KERNELS_H
+
+#include <hip/hip_runtime.h>
+#include <stdint.h>
+
+#include "../../include/radchan.h"
+
+namespace rc {
+
+/* Philox stream tags (4th counter word); the snapshot index is added as
+ * 16 * snapshot so every snapshot draws its own streams */
+enum { TAG_FLUID = 0, TAG_WEST = 1, TAG_EAST = 2, TAG_NOISE = 3 };
+
+/* solve-wide constants, passed by value */
+struct DevCommon {
+    int imax, jmax, kmax, kloc, zstart;
+    int nVar, photmax, srt, nB, nQ, nT, grid_num;
+    int ncell, nsurf;              /* emitting fluid cells, wall cells per wall */
+    float Lx, Ly, Lz, toll, tollS;
+    int bd[6], adiab[2];
+    uint32_t k0, k1;               /* Philox key */
+};
+
+/* one grid level of one snapshot */
+struct DevLevel {
+    int im, jm, km, maxs;
+    float dy, dz;                  /* uniform y/z spacing: face j at j*dy */
+    int pad0, pad1;
+    const float *xu;               /* im+1 x faces                        */
+    const float *T;                /* [km][jm][im] cell temperatures      */
+    const float *Tw;               /* [2][km][jm] adiabatic x-wall T      */
+    const float *Tz;               /* [2][jm][im] bd=4 z-wall T           */
+};
+
+/* one temperature snapshot (device memory) */
+struct DevSnap {
+    float Tmax, Ibmax, Ibmax_w, Ibmax_e;
+    float Tnb0, invdT;
+    int tstar, snap;
+    const float2 *kb;              /* [(rank*nQ+g)*(nT+1)+t] = {k, Ib}    */
+    const float *bfix;             /* [6][nB] fixed-temperature wall Ib   */
+    const float *cdf_b;            /* [nB] fluid band CDF, row t*         */
+    const float *cdf_g;            /* [nB][nQ] g CDFs, row t*             */
+    const float *cdf_bs;           /* [nB] wall band CDF, row t*          */
+    const float *cdf_gs;           /* [nQ] cumulative quadrature weights  */
+    const int *rank_of;            /* [nB] sorted rank of each band       */
+    const float *T0;               /* ghost-padded input field            */
+    const float *x0, *y0, *z0;     /* level-0 cell centres                */
+    DevLevel lev[RC_MAXLEV];
+};
+
+void launch_level_T(hipStream_t st, const float *T0, const rc_params &p, int g, float *Tl, float *Tw, float *Tz);
+void launch_sort(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v, int surface, int nunit,
+                 const int32_t *ulist, uint16_t *keys);
+void launch_fluid_plain(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v, const uint16_t *keys,
+                        double *de, unsigned long long *stats);
+void launch_surface_plain(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v,
+                          const uint16_t *keys, double *de, unsigned long long *stats);
+void launch_trace(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int surface, const int32_t *rays,
+                  const int32_t *unit, int n, const uint16_t *keys, rc_ray *out);
+void launch_finalize(hipStream_t st, const double *de, int n, int nVar, float *mean, float *sdev);
+
+}  // namespace rc
+
+#endif

@@ -1,0 +1,297 @@
+/*
+ * rc_march.h -- the per-ray device code shared by the plain and batched
+ * solvers: band draws, ray set-up and the multigrid DDA march.
+ *
+ * Reference: src/montecarlo/mc_gpu_noint.cu kernel_fluid :115-224,
+ * march_ray :245-658, find_band :1103-1169, kernel_surface :1381-1462.
+ */
+#ifndef RC_MARCH_H
+#define RC_MARCH_H
+
+#include <hip/hip_runtime.h>
+
+#include "rc_device.h"
+#include "rc_internal.h"
+#include "rc_kernels.h"
+
+namespace rc {
+
+using namespace rcd;
+
+/* ------------------------------------------------------------------------ */
+/* band draws and the srt enumeration                                        */
+
+/* (rank, g) key of one band draw: band from the Planck-weighted band CDF of
+ * the row nearest Tmax, g from that band's g CDF (find_band :1103-1169) */
+__device__ __forceinline__ uint32_t band_key(const DevCommon &cm, const DevSnap &sn, bool surface, float u2, float u3) {
+    const int nb = upper_bound(surface ? sn.cdf_bs : sn.cdf_b, cm.nB, u2);
+    const float *gc = surface ? sn.cdf_gs : sn.cdf_g + (size_t)nb * cm.nQ;
+    const int g = upper_bound(gc, cm.nQ, u3);
+    return (uint32_t)(sn.rank_of[nb] * cm.nQ + g);
+}
+
+/* ------------------------------------------------------------------------ */
+/* the march                                                                 */
+
+struct Ray {
+    int ic, jc, kc, f0, f1, f2;
+    float xp, yp, zp, sx, sy, sz, isx, isy, isz, tra;
+};
+
+struct En {
+    const float2 *__restrict__ kb;   /* (rank, g) row: nT+1 pairs {k, Ib} */
+    float W, Bi, invBi;
+    int rank;
+};
+
+__device__ __forceinline__ void kb_at(const DevSnap &sn, const float2 *__restrict__ kb, int nT, float T, float &K, float &B) {
+    int t; float a;
+    tpos(T, sn.Tnb0, sn.invdT, nT, t, a);
+    const float2 q0 = kb[t], q1 = kb[t + 1];
+    K = lerp(q0.x, q1.x, a);
+    B = lerp(q0.y, q1.y, a);
+}
+
+__device__ __forceinline__ void set_dir(Ray &r, float sx, float sy, float sz) {
+    r.sx = sx; r.sy = sy; r.sz = sz;
+    r.f0 = sx < 0.f; r.f1 = sy < 0.f; r.f2 = sz < 0.f;
+    r.isx = 1.0f / sx; r.isy = 1.0f / sy; r.isz = 1.0f / sz;
+}
+
+/* wall emission tally W tra (Bw - Bi)/Bi, then the ray ends */
+__device__ __forceinline__ void wall_hit(const DevSnap &sn, const En &e, int nT, Ray &r, float Bw, float &de) {
+    de -= (e.W * r.tra) * ((Bw - e.Bi) * e.invBi);
+    r.tra = 0.f;
+}
+
+/* one grid level of the march (march_ray, mc_gpu_noint.cu:245-658);
+ * returns the steps taken on this level */
+template <bool TR>
+__device__ __forceinline__ int march_level(const DevCommon &cm, const DevSnap &sn, const DevLevel &L, Ray &r,
+                                           const En &e, float tol, float &de, int &steps, uint32_t &hash,
+                                           int &status) {
+    int counter = 0;
+    const int nT = cm.nT;
+    while (r.tra > tol && counter < L.maxs) {
+        const float fx = L.xu[r.ic - r.f0];
+        const float fy = (float)(r.jc - r.f1) * L.dy;
+        const float fz = (float)(r.kc - r.f2) * L.dz;
+        const float dsx = (fx - r.xp) * r.isx;
+        const float dsy = (fy - r.yp) * r.isy;
+        const float dsz = (fz - r.zp) * r.isz;
+        const float Tc = L.T[(size_t)(r.ic - 1) + (size_t)L.im * ((size_t)(r.jc - 1) + (size_t)L.jm * (size_t)(r.kc - 1))];
+        float ds = dsx;
+        if (ds > dsy) ds = dsy;
+        if (ds > dsz) ds = dsz;
+        r.xp = (ds == dsx) ? fx : fmaf(ds, r.sx, r.xp);
+        r.yp = (ds == dsy) ? fy : fmaf(ds, r.sy, r.yp);
+        r.zp = (ds == dsz) ? fz : fmaf(ds, r.sz, r.zp);
+        float K, B;
+        kb_at(sn, e.kb, nT, Tc, K, B);
+        float em1, ex;
+        expm1_exp(-(K * ds), em1, ex);
+        const float alpha = -em1;
+        const float fac = (B - e.Bi) * e.invBi;
+        de -= ((e.W * r.tra) * alpha) * fac;
+        r.tra = r.tra * ex;
+        if (r.tra < tol) {
+            de -= (e.W * r.tra) * fac;
+            r.tra = 0.f;
+            status = RC_RAY_ABSORBED;
+        }
+        if (TR) hash = fnv(fnv(fnv(hash, (uint32_t)r.ic), (uint32_t)r.jc), (uint32_t)r.kc);
+        steps += 1;
+        /* one index moves along the crossed axis; boundary conditions */
+        int face = -1;
+        if (dsx < dsy && dsx < dsz) {
+            if (r.f0) { if (--r.ic == 0) face = FW; }
+            else { if (++r.ic == L.im + 1) face = FE; }
+        } else if (!(dsx < dsy) && dsy < dsz) {
+            if (r.f1) { if (--r.jc == 0) face = FS; }
+            else { if (++r.jc == L.jm + 1) face = FN; }
+        } else {
+            if (r.f2) { if (--r.kc == 0) face = FB; }
+            else { if (++r.kc == L.km + 1) face = FT; }
+        }
+        if (face >= 0) {
+            const int bd = cm.bd[face];
+            if (bd == 1) {
+                switch (face) {
+                    case FW: r.ic = L.im; r.xp = cm.Lx; break;
+                    case FE: r.ic = 1; r.xp = 0.f; break;
+                    case FS: r.jc = L.jm; r.yp = cm.Ly; break;
+                    case FN: r.jc = 1; r.yp = 0.f; break;
+                    case FB: r.kc = L.km; r.zp = cm.Lz; break;
+                    default: r.kc = 1; r.zp = 0.f; break;
+                }
+            } else if (bd == 2) {
+                float Bw;
+                if ((face == FW && cm.adiab[0]) || (face == FE && cm.adiab[1])) {
+                    const float Tw = L.Tw[(size_t)(face == FE) * L.jm * L.km + (size_t)(r.jc - 1) + (size_t)L.jm * (r.kc - 1)];
+                    float Kw;
+                    kb_at(sn, e.kb, nT, Tw, Kw, Bw);
+                } else {
+                    Bw = sn.bfix[(size_t)face * cm.nB + e.rank];
+                }
+                wall_hit(sn, e, nT, r, Bw, de);
+                status = RC_RAY_WALL;
+            } else if (bd == 3) {
+                switch (face) {
+                    case FW: r.xp = 0.f; r.sx = -r.sx; r.isx = -r.isx; r.ic = 1; r.f0 = r.sx < 0.f; break;
+                    case FE: r.xp = cm.Lx; r.sx = -r.sx; r.isx = -r.isx; r.ic = L.im; r.f0 = r.sx < 0.f; break;
+                    case FS: r.yp = 0.f; r.sy = -r.sy; r.isy = -r.isy; r.jc = 1; r.f1 = r.sy < 0.f; break;
+                    case FN: r.yp = cm.Ly; r.sy = -r.sy; r.isy = -r.isy; r.jc = L.jm; r.f1 = r.sy < 0.f; break;
+                    case FB: r.zp = 0.f; r.sz = -r.sz; r.isz = -r.isz; r.kc = 1; r.f2 = r.sz < 0.f; break;
+                    default: r.zp = cm.Lz; r.sz = -r.sz; r.isz = -r.isz; r.kc = L.km; r.f2 = r.sz < 0.f; break;
+                }
+            } else if (bd == 4) {
+                const float Tw = L.Tz[(size_t)(face == FT) * L.im * L.jm + (size_t)(r.ic - 1) + (size_t)L.im * (r.jc - 1)];
+                float Kw, Bw;
+                kb_at(sn, e.kb, nT, Tw, Kw, Bw);
+                wall_hit(sn, e, nT, r, Bw, de);
+                status = RC_RAY_WALL;
+            } else {
+                r.tra = 0.f;
+                status = RC_RAY_LOST;
+            }
+        }
+        counter += 1;
+    }
+    return counter;
+}
+
+/* all grid levels with index coarsening (mc_gpu_noint.cu:210-224) */
+template <bool TR>
+__device__ __forceinline__ void march_all(const DevCommon &cm, const DevSnap &sn, Ray &r, const En &e, float tol,
+                                          float &de, int &steps, uint32_t &hash, int &status, int &level) {
+    status = RC_RAY_STOPPED;
+    level = 0;
+    for (int g = 0; g < cm.grid_num; ++g) {
+        const DevLevel &L = sn.lev[g];
+        level = g;
+        const int n = march_level<TR>(cm, sn, L, r, e, tol, de, steps, hash, status);
+        if (!(r.tra > tol)) return;
+        if (g == cm.grid_num - 1) {
+            if (n >= L.maxs) status = RC_RAY_CAPPED;
+            return;
+        }
+        const DevLevel &N = sn.lev[g + 1];
+        const int ri = L.im / N.im, rj = L.jm / N.jm, rk = L.km / N.km;
+        r.ic = (r.ic + ri - 1) / ri;
+        r.jc = (r.jc + rj - 1) / rj;
+        r.kc = (r.kc + rk - 1) / rk;
+    }
+}
+
+/* per-ray energy constants: W = Ibmax B(Ti)/B(Tmax) K(Ti)/K(Tmax) for the
+ * fluid (:181-186), W = Ibmax_s B(Tw)/B(Tmax) for walls (:1419-1424) */
+__device__ __forceinline__ void energy(const DevCommon &cm, const DevSnap &sn, uint32_t key, float Ti, bool surface,
+                                       int bound, En &e) {
+    const int rank = (int)(key / (uint32_t)cm.nQ);
+    e.rank = rank;
+    e.kb = sn.kb + (size_t)key * (cm.nT + 1);
+    float Ki, Bi, Km, Bm;
+    kb_at(sn, e.kb, cm.nT, Ti, Ki, Bi);
+    kb_at(sn, e.kb, cm.nT, sn.Tmax, Km, Bm);
+    e.Bi = Bi;
+    e.invBi = 1.0f / Bi;
+    if (!surface) e.W = sn.Ibmax * (((Bi / Bm) * Ki) / Km);
+    else e.W = (bound ? sn.Ibmax_e : sn.Ibmax_w) * (Bi / Bm);
+}
+
+/* emitting fluid cell n -> (i, j, k global) */
+__device__ __forceinline__ void cell_ijk(const DevCommon &cm, int n, int &i, int &j, int &k) {
+    i = n % cm.imax + 1;
+    j = (n / cm.imax) % cm.jmax + 1;
+    k = n / (cm.imax * cm.jmax) + cm.zstart;
+}
+
+/* one fluid ray from the centre of cell (i,j,k) (kernel_fluid :115-224) */
+template <bool TR>
+__device__ __forceinline__ float fluid_ray(const DevCommon &cm, const DevSnap &sn, int i, int j, int k, uint32_t cid,
+                                           int v, int h, uint32_t key, bool have_key, float Ti, int &steps,
+                                           rc_ray *rec) {
+    const U4 u = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)v, cid, TAG_FLUID + 16u * (uint32_t)sn.snap);
+    if (!have_key) key = band_key(cm, sn, false, u.c, u.d);
+    float sx, sy, sz;
+    iso_dir(u.a, u.b, sx, sy, sz);
+    if (sx == 0.f) sx = 1e-10f;
+    if (sy == 0.f) sy = 1e-10f;
+    if (sz == 0.f) sz = 1e-10f;
+    Ray r;
+    set_dir(r, sx, sy, sz);
+    r.ic = i; r.jc = j; r.kc = k;
+    r.xp = sn.x0[i]; r.yp = sn.y0[j]; r.zp = sn.z0[k];
+    r.tra = 1.0f;
+    En e;
+    energy(cm, sn, key, Ti, false, 0, e);
+    float de = 0.f;
+    int st = 0, status, level;
+    uint32_t hash = 2166136261u;
+    march_all<TR>(cm, sn, r, e, cm.toll, de, st, hash, status, level);
+    steps += st;
+    if (TR) {
+        rec->nb = e.rank; rec->g = (int)(key % (uint32_t)cm.nQ);
+        rec->ic = r.ic; rec->jc = r.jc; rec->kc = r.kc; rec->level = level;
+        rec->steps = st; rec->status = status; rec->path_hash = hash;
+        rec->sx = sx; rec->sy = sy; rec->sz = sz; rec->tra = r.tra; rec->de = de;
+        rec->de64 = 0.0; rec->sumabs = 0.0;
+    }
+    return de;
+}
+
+/* one ray from the centre of wall cell (j,k) (kernel_surface :1381-1462) */
+template <bool TR>
+__device__ __forceinline__ float surface_ray(const DevCommon &cm, const DevSnap &sn, int bound, int j, int k,
+                                             uint32_t cid, int v, int h, uint32_t key, bool have_key, float Tw,
+                                             int &steps, rc_ray *rec) {
+    const U4 u = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)v, cid,
+                       (bound ? TAG_EAST : TAG_WEST) + 16u * (uint32_t)sn.snap);
+    if (!have_key) key = band_key(cm, sn, true, u.c, u.d);
+    float sx, sy, sz;
+    lamb_dir(u.a, u.b, bound, sx, sy, sz);
+    if (fabsf(sx) <= 1e-10) sx = (float)(1 - 2 * bound) * 1e-10f;
+    if (fabsf(sy) <= 1e-10) sy = (float)((sy > 0.f) - (sy <= 0.f)) * 1e-10f;
+    if (fabsf(sz) <= 1e-10) sz = (float)((sz > 0.f) - (sz <= 0.f)) * 1e-10f;
+    Ray r;
+    set_dir(r, sx, sy, sz);
+    r.ic = bound ? cm.imax : 1; r.jc = j; r.kc = k;
+    r.xp = bound ? cm.Lx : 0.f; r.yp = sn.y0[j]; r.zp = sn.z0[k];
+    r.tra = 1.0f;
+    En e;
+    energy(cm, sn, key, Tw, true, bound, e);
+    /* on-face index adjustment (:1429-1441) */
+    const DevLevel &L0 = sn.lev[0];
+    if (r.f0 && r.xp == L0.xu[r.ic - 1]) r.ic -= 1;
+    if (r.f1 && r.yp == (float)(r.jc - 1) * L0.dy) r.jc -= 1;
+    if (r.f2 && r.zp == (float)(r.kc - 1) * L0.dz) r.kc -= 1;
+    if (!r.f0 && r.xp == L0.xu[r.ic]) r.ic += 1;
+    if (!r.f1 && r.yp == (float)r.jc * L0.dy) r.jc += 1;
+    if (!r.f2 && r.zp == (float)r.kc * L0.dz) r.kc += 1;
+    float de = 0.f;
+    int st = 0, status, level;
+    uint32_t hash = 2166136261u;
+    march_all<TR>(cm, sn, r, e, cm.tollS, de, st, hash, status, level);
+    steps += st;
+    if (TR) {
+        rec->nb = e.rank; rec->g = (int)(key % (uint32_t)cm.nQ);
+        rec->ic = r.ic; rec->jc = r.jc; rec->kc = r.kc; rec->level = level;
+        rec->steps = st; rec->status = status; rec->path_hash = hash;
+        rec->sx = sx; rec->sy = sy; rec->sz = sz; rec->tra = r.tra; rec->de = de;
+        rec->de64 = 0.0; rec->sumabs = 0.0;
+    }
+    return de;
+}
+
+__device__ __forceinline__ float cell_T(const DevCommon &cm, const DevSnap &sn, int i, int j, int k) {
+    return sn.T0[(size_t)i + (size_t)(cm.imax + 2) * ((size_t)j + (size_t)(cm.jmax + 2) * (size_t)k)];
+}
+__device__ __forceinline__ float wall_T(const DevCommon &cm, const DevSnap &sn, int bound, int j, int k) {
+    const size_t row = (size_t)(cm.imax + 2) * ((size_t)j + (size_t)(cm.jmax + 2) * (size_t)k);
+    /* texel coordinate 1.0 (imax+1.0): halfway between ghost and first cell */
+    return bound ? lerp(sn.T0[row + cm.imax], sn.T0[row + cm.imax + 1], 0.5f) : lerp(sn.T0[row], sn.T0[row + 1], 0.5f);
+}
+
+}  // namespace rc
+
+#endif

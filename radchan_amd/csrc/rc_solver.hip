@@ -1,0 +1,319 @@
+/*
+ * rc_solver.hip -- MI355X (gfx950) narrow-band correlated-k OERM Monte Carlo
+ * solver for the radiative source term div(q_r) of a channel flow, a drop-in
+ * for RadChan's src/montecarlo path on one GPU.
+ *
+ * Reference path (RadChan): src/montecarlo/mc_gpu_noint.cu
+ *   kernel_fluid :55-243, march_ray :245-658, mc_gpu_ :660-874,
+ *   get_results_ :876-1021, find_band :1103-1169, wave_find :1234-1267,
+ *   kernel_surface :1316-1482; tables src/montecarlo/memory_copy_noint.cu.
+ *
+ * Device data layout (all structure-of-arrays in HBM):
+ *   per snapshot and grid level: interior temperatures T_g[km][jm][im]
+ *   (level 0 is the interior of the input; coarse levels hold the value the
+ *   reference samples with a texture at the mapped cell centre), the
+ *   adiabatic x-wall temperatures [2][km][jm] and the bd=4 z-wall
+ *   temperatures [2][jm][im]; x faces per level;
+ *   tables in sorted band ("rank") order: kb[(rank*nQ+g)*(nT+1)+t] = {k, Ib}
+ *   pairs, one 16-byte load per march step, the last row repeated so that the
+ *   clamped top row needs no branch.
+ *
+ * Variants:
+ *   plain   -- the reference's launch shape: one thread per emitting cell,
+ *              photmax rays in sequence, FP32 per-cell sums, a sort pre-pass
+ *              per run (nVar launches of each kernel);
+ *   batched -- see rc_batched.hip.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdio>
+#include <cstring>
+#include <memory>
+#include <mutex>
+#include <string>
+#include <vector>
+
+#include "rc_device.h"
+#include "rc_internal.h"
+#include "rc_kernels.h"
+#include "rc_march.h"
+
+using namespace rcd;
+
+namespace rc {
+
+/* ------------------------------------------------------------------------ */
+/* set-up kernels                                                            */
+
+/* linear texture fetch of the ghost-padded temperature volume at
+ * unnormalised texel coordinates with clamp-to-edge addressing (the
+ * reference's cudaFilterModeLinear + cudaAddressModeClamp texture,
+ * memory_copy_noint.cu:131-168): texel i is centred at i + 0.5 */
+__device__ float tex3_T(const float *__restrict__ T, int nx, int ny, int nz, float tx, float ty, float tz) {
+    const int n[3] = {nx, ny, nz};
+    const float crd[3] = {tx, ty, tz};
+    int i0[3], i1[3];
+    float w[3];
+#pragma unroll
+    for (int d = 0; d < 3; ++d) {
+        const float b = crd[d] - 0.5f;
+        const float fl = floorf(b);
+        w[d] = b - fl;
+        int lo = (int)fl, hi = lo + 1;
+        lo = min(max(lo, 0), n[d] - 1);
+        hi = min(max(hi, 0), n[d] - 1);
+        i0[d] = lo; i1[d] = hi;
+    }
+    auto at = [&](int i, int j, int k) { return T[(size_t)i + (size_t)nx * ((size_t)j + (size_t)ny * (size_t)k)]; };
+    const float c00 = lerp(at(i0[0], i0[1], i0[2]), at(i1[0], i0[1], i0[2]), w[0]);
+    const float c10 = lerp(at(i0[0], i1[1], i0[2]), at(i1[0], i1[1], i0[2]), w[0]);
+    const float c01 = lerp(at(i0[0], i0[1], i1[2]), at(i1[0], i0[1], i1[2]), w[0]);
+    const float c11 = lerp(at(i0[0], i1[1], i1[2]), at(i1[0], i1[1], i1[2]), w[0]);
+    const float c0 = lerp(c00, c10, w[1]);
+    const float c1 = lerp(c01, c11, w[1]);
+    return lerp(c0, c1, w[2]);
+}
+
+/* mapped texel coordinate of coarse index ic at ratio r (:334-336) */
+__device__ __forceinline__ float tmap(int ic, int r) { return ((float)ic - 0.5f) * (float)r + 1.0f; }
+
+/* per-level temperatures: cell values, adiabatic x walls, bd=4 z walls */
+__global__ void __launch_bounds__(256) k_level_T(const float *__restrict__ T0, int imax, int jmax, int kmax,
+                                                 int im, int jm, int km, float *__restrict__ Tl,
+                                                 float *__restrict__ Tw, float *__restrict__ Tz) {
+    const int rx = imax / im, ry = jmax / jm, rz = kmax / km;
+    const int nx = imax + 2, ny = jmax + 2, nz = kmax + 2;
+    const size_t ncell = (size_t)im * jm * km;
+    const size_t nw = (size_t)jm * km, nzf = (size_t)im * jm;
+    const size_t total = ncell + (Tw ? 2 * nw : 0) + (Tz ? 2 * nzf : 0);
+    for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total; id += (size_t)gridDim.x * blockDim.x) {
+        if (id < ncell) {
+            const int i = (int)(id % im) + 1, j = (int)((id / im) % jm) + 1, k = (int)(id / ((size_t)im * jm)) + 1;
+            Tl[id] = tex3_T(T0, nx, ny, nz, tmap(i, rx), tmap(j, ry), tmap(k, rz));
+            continue;
+        }
+        size_t q = id - ncell;
+        if (Tw && q < 2 * nw) {
+            const int b = (int)(q / nw);
+            const size_t r = q % nw;
+            const int j = (int)(r % jm) + 1, k = (int)(r / jm) + 1;
+            const float tx = b ? (float)imax + 1.0f : 1.0f;
+            Tw[q] = tex3_T(T0, nx, ny, nz, tx, tmap(j, ry), tmap(k, rz));
+            continue;
+        }
+        if (Tw) q -= 2 * nw;
+        const int b = (int)(q / nzf);
+        const size_t r = q % nzf;
+        const int i = (int)(r % im) + 1, j = (int)(r / im) + 1;
+        const float tz = b ? (float)kmax + 0.5f : 1.5f;
+        Tz[q] = tex3_T(T0, nx, ny, nz, tmap(i, rx), tmap(j, ry), tz);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* the srt enumeration                                                       */
+
+/* One workgroup (one wavefront) per (cell, run): draws photmax keys and
+ * bitonic-sorts them in LDS.  The sorted sequence is the one the reference's
+ * histogram enumeration produces (kernel_find :1023-1049 + wave_find
+ * :1234-1267): ascending sorted-band rank, then g; equal keys are identical
+ * draws, so any correct sort gives the same sequence. */
+__global__ void __launch_bounds__(64) k_sort_bands(DevCommon cm, const DevSnap *__restrict__ snaps, int s, int v,
+                                                   int surface, int nunit, const int32_t *__restrict__ ulist,
+                                                   uint16_t *__restrict__ keys) {
+    extern __shared__ uint32_t lds[];
+    const DevSnap &sn = snaps[s];
+    const int P = cm.photmax;
+    int n2 = 1;
+    while (n2 < P) n2 <<= 1;
+    for (int u = blockIdx.x; u < nunit; u += gridDim.x) {
+        uint32_t cid, tag;
+        int vv = v;
+        if (ulist) {   /* explicit (cell id, run, tag) triples */
+            cid = (uint32_t)ulist[3 * u]; vv = ulist[3 * u + 1]; tag = (uint32_t)ulist[3 * u + 2];
+        } else if (!surface) {
+            cid = (uint32_t)u + (uint32_t)cm.imax * cm.jmax * (cm.zstart - 1); tag = TAG_FLUID;
+        } else {
+            const int b = u / cm.nsurf;
+            cid = (uint32_t)(u % cm.nsurf) + (uint32_t)cm.jmax * (cm.zstart - 1); tag = b ? TAG_EAST : TAG_WEST;
+        }
+        const uint32_t tagx = tag + 16u * (uint32_t)sn.snap;
+        for (int h = threadIdx.x; h < n2; h += blockDim.x) {
+            uint32_t key = 0xFFFFFFFFu;
+            if (h < P) {
+                const U4 r = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)vv, cid, tagx);
+                key = band_key(cm, sn, tag != TAG_FLUID, r.c, r.d);
+            }
+            lds[h] = key;
+        }
+        __syncthreads();
+        for (int k = 2; k <= n2; k <<= 1)
+            for (int j = k >> 1; j > 0; j >>= 1) {
+                for (int i = threadIdx.x; i < n2; i += blockDim.x) {
+                    const int ixj = i ^ j;
+                    if (ixj > i) {
+                        const uint32_t a = lds[i], b = lds[ixj];
+                        const bool up = (i & k) == 0;
+                        if ((a > b) == up) { lds[i] = b; lds[ixj] = a; }
+                    }
+                }
+                __syncthreads();
+            }
+        for (int h = threadIdx.x; h < P; h += blockDim.x) keys[(size_t)u * P + h] = (uint16_t)lds[h];
+        __syncthreads();
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* plain variant: the reference's launch shape                               */
+
+__global__ void __launch_bounds__(128) k_fluid_plain(DevCommon cm, const DevSnap *__restrict__ snaps, int s, int v,
+                                                     const uint16_t *__restrict__ keys, double *__restrict__ de,
+                                                     unsigned long long *__restrict__ stats) {
+    const DevSnap &sn = snaps[s];
+    int64_t steps = 0;
+    int64_t rays = 0;
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < cm.ncell; n += gridDim.x * blockDim.x) {
+        int i, j, k;
+        cell_ijk(cm, n, i, j, k);
+        const uint32_t cid = (uint32_t)n + (uint32_t)cm.imax * cm.jmax * (cm.zstart - 1);
+        const float Ti = cell_T(cm, sn, i, j, k);
+        float acc = 0.f;
+        int st = 0;
+        for (int h = 0; h < cm.photmax; ++h) {
+            const uint32_t key = keys ? (uint32_t)keys[(size_t)n * cm.photmax + h] : 0u;
+            acc += fluid_ray<false>(cm, sn, i, j, k, cid, v, h, key, keys != nullptr, Ti, st, nullptr);
+        }
+        steps += st;
+        rays += cm.photmax;
+        de[(size_t)n * cm.nVar + v] = (double)acc;
+    }
+    /* wavefront-reduced counters */
+    for (int o = 32; o > 0; o >>= 1) {
+        steps += __shfl_xor(steps, o);
+        rays += __shfl_xor(rays, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], (unsigned long long)rays);
+        atomicAdd(&stats[1], (unsigned long long)steps);
+    }
+}
+
+__global__ void __launch_bounds__(128) k_surface_plain(DevCommon cm, const DevSnap *__restrict__ snaps, int s, int v,
+                                                       const uint16_t *__restrict__ keys, double *__restrict__ de,
+                                                       unsigned long long *__restrict__ stats) {
+    const DevSnap &sn = snaps[s];
+    int64_t steps = 0, rays = 0;
+    const int nsurf = cm.nsurf;
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < 2 * nsurf; n += gridDim.x * blockDim.x) {
+        const int bound = n / nsurf, m = n % nsurf;
+        if ((bound == 0 && !cm.adiab[0]) || (bound == 1 && !cm.adiab[1])) continue;
+        const int j = m % cm.jmax + 1, k = m / cm.jmax + cm.zstart;
+        const uint32_t cid = (uint32_t)m + (uint32_t)cm.jmax * (cm.zstart - 1);
+        const float Tw = wall_T(cm, sn, bound, j, k);
+        float acc = 0.f;
+        int st = 0;
+        for (int h = 0; h < cm.photmax; ++h) {
+            const uint32_t key = keys ? (uint32_t)keys[(size_t)n * cm.photmax + h] : 0u;
+            acc += surface_ray<false>(cm, sn, bound, j, k, cid, v, h, key, keys != nullptr, Tw, st, nullptr);
+        }
+        steps += st;
+        rays += cm.photmax;
+        de[(size_t)n * cm.nVar + v] = (double)acc;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        steps += __shfl_xor(steps, o);
+        rays += __shfl_xor(rays, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], (unsigned long long)rays);
+        atomicAdd(&stats[1], (unsigned long long)steps);
+    }
+}
+
+/* traced rays: rays[5*q] = (i or bound, j, k, v, h); unit[q] indexes keys */
+__global__ void __launch_bounds__(64) k_trace(DevCommon cm, const DevSnap *__restrict__ snaps, int surface,
+                                              const int32_t *__restrict__ rays, const int32_t *__restrict__ unit,
+                                              int n, const uint16_t *__restrict__ keys, rc_ray *__restrict__ out) {
+    const DevSnap &sn = snaps[0];
+    const int q = blockIdx.x * blockDim.x + threadIdx.x;
+    if (q >= n) return;
+    const int a = rays[5 * q], j = rays[5 * q + 1], k = rays[5 * q + 2], v = rays[5 * q + 3], h = rays[5 * q + 4];
+    const bool hk = keys != nullptr;
+    const uint32_t key = hk ? (uint32_t)keys[(size_t)unit[q] * cm.photmax + h] : 0u;
+    int st = 0;
+    if (!surface) {
+        const uint32_t cid = (uint32_t)((a - 1) + cm.imax * ((j - 1) + cm.jmax * (k - 1)));
+        fluid_ray<true>(cm, sn, a, j, k, cid, v, h, key, hk, cell_T(cm, sn, a, j, k), st, &out[q]);
+    } else {
+        const uint32_t cid = (uint32_t)((j - 1) + cm.jmax * (k - 1));
+        surface_ray<true>(cm, sn, a, j, k, cid, v, h, key, hk, wall_T(cm, sn, a, j, k), st, &out[q]);
+    }
+}
+
+/* mean over runs and the reference's variance 1/((n-1/2)(n-1)) sum (x-m)^2
+ * (mc_gpu_noint.cu:234-239), reported as its square root (:984); 0 for a
+ * single run */
+__global__ void k_finalize(const double *__restrict__ de, int n, int nVar, float *__restrict__ mean,
+                           float *__restrict__ sdev) {
+    for (int c = blockIdx.x * blockDim.x + threadIdx.x; c < n; c += gridDim.x * blockDim.x) {
+        double m = 0.0;
+        for (int v = 0; v < nVar; ++v) m += de[(size_t)c * nVar + v];
+        m /= nVar;
+        double var = 0.0;
+        if (nVar > 1) {
+            for (int v = 0; v < nVar; ++v) {
+                const double d = de[(size_t)c * nVar + v] - m;
+                var += d * d;
+            }
+            var = var / (nVar - 0.5) / (nVar - 1);
+        }
+        mean[c] = (float)m;
+        sdev[c] = (float)sqrt(var);
+    }
+}
+
+/* ------------------------------------------------------------------------ */
+/* launchers used by the session code                                        */
+
+void launch_level_T(hipStream_t st, const float *T0, const rc_params &p, int g, float *Tl, float *Tw, float *Tz) {
+    const size_t n = (size_t)p.maxi[g] * p.maxj[g] * p.maxk[g];
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_level_T, dim3(blocks), dim3(256), 0, st, T0, p.imax, p.jmax, p.kmax, p.maxi[g], p.maxj[g],
+                       p.maxk[g], Tl, Tw, Tz);
+}
+
+void launch_sort(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v, int surface, int nunit,
+                 const int32_t *ulist, uint16_t *keys) {
+    int n2 = 1;
+    while (n2 < cm.photmax) n2 <<= 1;
+    const int blocks = std::min(nunit, 1 << 20);
+    hipLaunchKernelGGL(k_sort_bands, dim3(blocks), dim3(64), (size_t)n2 * 4, st, cm, snaps, s, v, surface, nunit,
+                       ulist, keys);
+}
+
+void launch_fluid_plain(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v, const uint16_t *keys,
+                        double *de, unsigned long long *stats) {
+    const int blocks = std::max(1, std::min((cm.ncell + 127) / 128, 1 << 20));
+    hipLaunchKernelGGL(k_fluid_plain, dim3(blocks), dim3(128), 0, st, cm, snaps, s, v, keys, de, stats);
+}
+
+void launch_surface_plain(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v,
+                          const uint16_t *keys, double *de, unsigned long long *stats) {
+    const int blocks = std::max(1, std::min((2 * cm.nsurf + 127) / 128, 1 << 20));
+    hipLaunchKernelGGL(k_surface_plain, dim3(blocks), dim3(128), 0, st, cm, snaps, s, v, keys, de, stats);
+}
+
+void launch_trace(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int surface, const int32_t *rays,
+                  const int32_t *unit, int n, const uint16_t *keys, rc_ray *out) {
+    hipLaunchKernelGGL(k_trace, dim3((n + 63) / 64), dim3(64), 0, st, cm, snaps, surface, rays, unit, n, keys, out);
+}
+
+void launch_finalize(hipStream_t st, const double *de, int n, int nVar, float *mean, float *sdev) {
+    const int blocks = std::max(1, std::min((n + 255) / 256, 65536));
+    hipLaunchKernelGGL(k_finalize, dim3(blocks), dim3(256), 0, st, de, n, nVar, mean, sdev);
+}
+
+}  // namespace rc

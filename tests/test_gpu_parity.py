@@ -1,0 +1,123 @@
+"""GPU parity: the HIP library against the CPU oracle on identical Philox
+streams, called through the C ABI.
+
+Bar: integer traversal (band, g, cell sequence hash, end cell, level,
+steps, status) bit-exact; per-ray FP32 tallies bit-exact (same operation
+sequence); per-ray FP32 vs the oracle's FP64 energy path within
+1e-5 * sum|terms|; plain-variant per-(cell, run) tallies equal to the
+oracle's FP32 sums bit for bit; batched-variant FP64 tallies equal to the
+FP64 sum of the oracle's per-ray FP32 tallies within 1e-12 relative.
+"""
+import numpy as np
+import pytest
+
+from radchan_amd import VARIANT_BATCHED, VARIANT_PLAIN, Solver
+from oracle import oracle as O
+from tests import cases
+
+pytestmark = pytest.mark.gpu
+
+INT_FIELDS = ("nb", "g", "ic", "jc", "kc", "level", "steps", "status", "path_hash")
+
+
+def _rays(p, n, seed, surface=False):
+    rng = np.random.default_rng(seed)
+    a = rng.integers(0, 2, n) if surface else rng.integers(1, p.imax + 1, n)
+    j = rng.integers(1, p.jmax + 1, n)
+    k = rng.integers(1, p.kmax + 1, n)
+    v = rng.integers(0, p.nVar, n)
+    h = rng.integers(0, p.photmax, n)
+    return np.stack([a, j, k, v, h], axis=1).astype(np.int32)
+
+
+def _check_trace(p, tab, T, surface=False, n=512, seed=0):
+    print(f"trace parity: {p.imax}x{p.jmax}x{p.kmax} grey={p.grey} surface={surface} n={n}", flush=True)
+    s = Solver(p, tab)
+    s.set_fields([T])
+    o = O.Oracle(p, tab, T)
+    rays = _rays(p, n, seed, surface)
+    g = s.trace(rays, surface)
+    c = o.trace(rays, surface)
+    for f in INT_FIELDS:
+        bad = np.nonzero(g[f] != c[f])[0]
+        assert bad.size == 0, f"{f} differs for rays {rays[bad[:5]]}: gpu {g[f][bad[:5]]} cpu {c[f][bad[:5]]}"
+    for f in ("sx", "sy", "sz", "tra", "de"):
+        assert np.array_equal(g[f].view(np.uint32), c[f].view(np.uint32)), f"{f} not bit-exact"
+    tol = 1e-5 * np.maximum(c["sumabs"], 1e-300)
+    assert np.all(np.abs(g["de"].astype(np.float64) - c["de64"]) <= tol + 1e-30)
+    s.close()
+    return g
+
+
+def test_trace_grey(nb_tables):
+    p, tab, T = cases.grey_case()
+    _check_trace(p, tab, T)
+
+
+def test_trace_narrow_band(nb_tables):
+    p, tab, T = cases.nb_case(nb_tables)
+    g = _check_trace(p, tab, T, n=1024)
+    assert len(np.unique(g["nb"])) > 20
+
+
+def test_trace_narrow_band_unsorted(nb_tables):
+    p, tab, T = cases.nb_case(nb_tables, srt=0)
+    _check_trace(p, tab, T)
+
+
+def test_trace_multigrid_walls(nb_tables):
+    p, tab, T = cases.mg_case(nb_tables)
+    g = _check_trace(p, tab, T, n=1024)
+    assert set(np.unique(g["level"])) >= {0, 1, 2}
+    _check_trace(p, tab, T, surface=True, n=512, seed=1)
+
+
+def test_trace_boundary_conditions(nb_tables):
+    p, tab, T = cases.bc_case(nb_tables)
+    g = _check_trace(p, tab, T, n=1024)
+    assert (g["status"] == 1).any() and (g["status"] == 2).any()
+
+
+def _field_parity(p, tab, T, variant, surface=False):
+    print(f"field parity: variant={variant} {p.imax}x{p.jmax}x{p.kmax} grid_num={p.grid_num}", flush=True)
+    s = Solver(p, tab)
+    s.set_fields([T])
+    st = s.solve(variant)
+    o = O.Oracle(p, tab, T)
+    if not surface:
+        got = s.tallies()
+        de32, de32d, de64, steps = o.solve(O.all_cells(p))
+    else:
+        got = s.surface()[2].reshape(-1, p.nVar)
+        de32, de32d, de64, steps = o.solve(O.all_wall_cells(p), surface=True)
+    if variant == VARIANT_PLAIN:
+        assert np.array_equal(got, de32.astype(np.float64)), "plain tallies differ from the oracle FP32 sums"
+    else:
+        scale = np.abs(de32d).max()
+        assert np.abs(got - de32d).max() <= 1e-12 * scale
+    if not surface and not p.calcwalls:
+        assert st["steps"] == steps.sum()
+        assert st["rays"] == p.imax * p.jmax * p.kmax * p.photmax * p.nVar
+    s.close()
+
+
+@pytest.mark.parametrize("variant", [VARIANT_PLAIN, VARIANT_BATCHED])
+def test_field_grey(variant):
+    _field_parity(*cases.grey_case(), variant)
+
+
+@pytest.mark.parametrize("variant", [VARIANT_PLAIN, VARIANT_BATCHED])
+def test_field_narrow_band(nb_tables, variant):
+    _field_parity(*cases.nb_case(nb_tables), variant)
+
+
+@pytest.mark.parametrize("variant", [VARIANT_PLAIN, VARIANT_BATCHED])
+def test_field_multigrid_and_walls(nb_tables, variant):
+    p, tab, T = cases.mg_case(nb_tables)
+    _field_parity(p, tab, T, variant)
+    _field_parity(p, tab, T, variant, surface=True)
+
+
+@pytest.mark.parametrize("variant", [VARIANT_PLAIN, VARIANT_BATCHED])
+def test_field_boundary_conditions(nb_tables, variant):
+    _field_parity(*cases.bc_case(nb_tables), variant)
