@@ -345,13 +345,13 @@ template <class G> inline void wall_tally(const orc_ctx &c, const Energy &e, Ray
         if (fixed) Bw64 = (double)Bfix; else { double K64; kb_d(c, e, (double)Tw_, &K64, &Bw64); }
         const double term64 = e.W64 * r.tra64 * ((Bw64 - e.Bi64) / e.Bi64);
         t.de64 -= term64;
-        t.sumabs += std::fabs(term64);
+        t.sumabs += std::fabs(e.W64 * r.tra64) * (std::fabs(Bw64) / e.Bi64 + 1.0);
     } else {
         double Bw;
         if (fixed) Bw = (double)Bfix; else { double K; kb_d(c, e, (double)Tw_, &K, &Bw); }
         const double term = e.W64 * (double)r.tra * ((Bw - e.Bi64) / e.Bi64);
         t.de64 -= term;
-        t.sumabs += std::fabs(term);
+        t.sumabs += std::fabs(e.W64 * (double)r.tra) * (std::fabs(Bw) / e.Bi64 + 1.0);
     }
 }
 
@@ -400,16 +400,17 @@ int march_level(const orc_ctx &c, int lev, Ray<G> &r, const Energy &e, float tol
             kb_d(c, e, (double)Tc, &K64, &B64);
             const double x64 = K64 * (double)ds;
             const double fac64 = (B64 - e.Bi64) / e.Bi64;
-            const double term64 = e.W64 * r.tra64 * (-std::expm1(-x64)) * fac64;
+            const double wta64 = e.W64 * r.tra64 * (-std::expm1(-x64));
+            const double term64 = wta64 * fac64;
             t.de64 -= term64;
-            t.sumabs += std::fabs(term64);
+            t.sumabs += std::fabs(wta64) * (std::fabs(B64) / e.Bi64 + 1.0);
             r.tra = (G)((float)r.tra * ex);
             r.tra64 *= std::exp(-x64);
             if ((float)r.tra < tol_dump) {
                 t.de -= (e.W * (float)r.tra) * fac;
                 const double d64 = e.W64 * r.tra64 * fac64;
                 t.de64 -= d64;
-                t.sumabs += std::fabs(d64);
+                t.sumabs += std::fabs(e.W64 * r.tra64) * (std::fabs(B64) / e.Bi64 + 1.0);
                 r.tra = 0; r.tra64 = 0.0;
                 *status = ORC_ABSORBED;
             }
@@ -418,14 +419,15 @@ int march_level(const orc_ctx &c, int lev, Ray<G> &r, const Energy &e, float tol
             kb_d(c, e, (double)Tc, &K, &B);
             const double x = K * (double)ds;
             const double fac = (B - e.Bi64) / e.Bi64;
-            const double term = e.W64 * (double)r.tra * (-std::expm1(-x)) * fac;
+            const double wta = e.W64 * (double)r.tra * (-std::expm1(-x));
+            const double term = wta * fac;
             t.de64 -= term;
-            t.sumabs += std::fabs(term);
+            t.sumabs += std::fabs(wta) * (std::fabs(B) / e.Bi64 + 1.0);
             r.tra = (G)((double)r.tra * std::exp(-x));
             if ((double)r.tra < (double)tol_dump) {
                 const double d = e.W64 * (double)r.tra * fac;
                 t.de64 -= d;
-                t.sumabs += std::fabs(d);
+                t.sumabs += std::fabs(e.W64 * (double)r.tra) * (std::fabs(B) / e.Bi64 + 1.0);
                 r.tra = 0;
                 *status = ORC_ABSORBED;
             }

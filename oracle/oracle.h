@@ -75,7 +75,9 @@ typedef struct orc_ray {
     float tra;                 /* transmissivity when the ray ended           */
     float de;                  /* FP32 tally of this ray                      */
     double de64;               /* FP64-energy tally on the same FP32 geometry */
-    double sumabs;             /* sum of |terms| (FP64), tolerance scale      */
+    double sumabs;             /* sum over terms of |absorbed| + |emitted|    */
+                               /* parts (FP64): the scale of the tally before */
+                               /* the reciprocity difference cancels          */
 } orc_ray;
 
 /* ray end status */

@@ -4,7 +4,8 @@ streams, called through the C ABI.
 Bar: integer traversal (band, g, cell sequence hash, end cell, level,
 steps, status) bit-exact; per-ray FP32 tallies bit-exact (same operation
 sequence); per-ray FP32 vs the oracle's FP64 energy path within
-1e-5 * sum|terms|; plain-variant per-(cell, run) tallies equal to the
+1e-5 * sum over terms of (|absorbed part| + |emitted part|), the scale of
+the tally before the reciprocity difference cancels; plain-variant per-(cell, run) tallies equal to the
 oracle's FP32 sums bit for bit; batched-variant FP64 tallies equal to the
 FP64 sum of the oracle's per-ray FP32 tallies within 1e-12 relative.
 """
