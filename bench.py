@@ -1,0 +1,202 @@
+#!/usr/bin/env python3
+"""Benchmark of the radiative solver on one MI355X.
+
+A step is one solve of the narrow-band correlated-k OERM estimator over the
+whole field: every emitting cell x photmax rays x nVar independent runs,
+inputs (field, tables, grids) resident in HBM before the timed region.
+
+Workload (default): 64x64x64 cells, x stretched (fact_mesh 1.5), synthetic
+H2O-like tables of 360 bands x 16 g-points x 20 temperatures, photmax 200,
+sorted bands (srt=1), nVar 8, FP32 march with FP64 tallies.
+
+Printed (one JSON line on stdout): rays/s of the batched solver, the plain
+solver's rays/s beside it, the measured std-error eps of div(q_r) and the
+projected time to eps = 1% for both, the roofline of the dominant kernel and
+a CPU baseline (the C++ oracle in FP32 on the host cores, a bounded sample).
+"""
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+
+import numpy as np
+
+ROOT = os.path.dirname(os.path.abspath(__file__))
+sys.path.insert(0, ROOT)
+
+HBM_PEAK_GBS = 8000.0           # MI355X HBM3E
+B_STEP = 32                     # 3 face coords + T + 2 k + 2 Ib texels (FP32)
+B_RAY = 32                      # 2 x (2 k + 2 Ib texels) at emission and Tmax
+B_CELL = 20                     # T in, FP64 tally pair out
+
+
+def log(*a):
+    print(*a, file=sys.stderr, flush=True)
+
+
+def std_error(de: np.ndarray):
+    """eps = sqrt(sum_c se_c^2) / sqrt(sum_c mean_c^2), se_c the standard
+    error of the mean over the nVar runs; also the p95 of se_c/|mean_c| over
+    cells with |mean_c| > 1% of the largest."""
+    m = de.mean(axis=1)
+    se = de.std(axis=1, ddof=1) / np.sqrt(de.shape[1])
+    eps = float(np.sqrt((se ** 2).sum()) / np.sqrt((m ** 2).sum()))
+    big = np.abs(m) > 0.01 * np.abs(m).max()
+    p95 = float(np.percentile(se[big] / np.abs(m[big]), 95))
+    return eps, p95
+
+
+def cpu_baseline(p, tab, T, seconds: float):
+    """The C++ oracle (FP32, reference semantics) on a sample of cells of the
+    same workload, on the host cores."""
+    from oracle import oracle as O
+    try:
+        ncores = len(os.sched_getaffinity(0))
+    except AttributeError:
+        ncores = os.cpu_count() or 1
+    ncores = max(1, min(ncores, 16))
+    o = O.Oracle(p, tab, T)
+    cells = O.all_cells(p)
+    rng = np.random.default_rng(123)
+    order = rng.permutation(len(cells))
+    n = 16 * ncores
+    t0 = time.perf_counter()
+    o.solve(cells[order[:n]], nthreads=ncores)
+    dt = time.perf_counter() - t0
+    n = int(min(len(cells), max(n, n * seconds / max(dt, 1e-3))))
+    sample = cells[order[:n]]
+    t0 = time.perf_counter()
+    _, _, _, steps = o.solve(sample, nthreads=ncores)
+    dt = time.perf_counter() - t0
+    rays = n * p.photmax * p.nVar
+    return {"value": rays / dt, "unit": "rays/s", "cores": ncores, "kind": "port",
+            "sample": f"{n} of {len(cells)} cells (random), all {p.photmax} rays x {p.nVar} runs each: "
+                      f"{rays} rays, {int(steps.sum())} steps in {dt:.1f} s",
+            "steps_per_s": float(steps.sum() / dt)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=10)
+    ap.add_argument("--warmup", type=int, default=2)
+    ap.add_argument("--workload", default="nb64", choices=["nb64", "nb128", "big256", "grey32", "mg192"])
+    ap.add_argument("--nvar", type=int, default=8)
+    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--cpu-seconds", type=float, default=15.0)
+    ap.add_argument("--no-cpu", action="store_true")
+    ap.add_argument("--no-plain", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
+        print("bench.py: this build runs on one GPU; the multi-GPU path (cell slabs per GPU) is not "
+              "part of it, so --gpus must be 1", file=sys.stderr)
+        return 2
+
+    from radchan_amd import VARIANT_BATCHED, VARIANT_PLAIN, Solver, config
+    from radchan_amd import fields as F
+    from radchan_amd import lib as L
+    from radchan_amd.tables import synthetic_nb
+
+    p = config(args.workload)
+    p.nVar = args.nvar
+    tab = None
+    if p.grey:
+        T = F.channel_field(p, 500.0, 1000.0, noise=0.02, seed=0)
+    else:
+        tab = synthetic_nb()
+        T = F.channel_field(p, 300.0, 1500.0, noise=0.02, seed=0)
+    log(f"bench: workload {args.workload} {p.imax}x{p.jmax}x{p.kmax} photmax {p.photmax} nVar {p.nVar}")
+    s = Solver(p, tab)
+    s.set_fields([T])
+    lib = L.load()
+
+    for _ in range(args.warmup):
+        s.solve(VARIANT_BATCHED)
+    lib.rc_device_synchronize()
+    march = []
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        st = s.solve(VARIANT_BATCHED)
+        march.append(st["march_ms"])
+    lib.rc_device_synchronize()
+    wall = time.perf_counter() - t0
+    ms_per_step = 1e3 * wall / args.steps
+    rays = st["rays"]
+    steps = st["steps"]
+    value = rays / (wall / args.steps)
+    de = s.tallies()
+    eps, p95 = std_error(de)
+    log(f"batched: {ms_per_step:.2f} ms/step, {value / 1e9:.3f} Grays/s, {steps / rays:.2f} steps/ray, eps {eps:.4g}")
+
+    plain = None
+    if not args.no_plain and args.plain_steps > 0:
+        s.solve(VARIANT_PLAIN)
+        lib.rc_device_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(args.plain_steps):
+            sp = s.solve(VARIANT_PLAIN)
+        lib.rc_device_synchronize()
+        pwall = (time.perf_counter() - t0) / args.plain_steps
+        eps_p, _ = std_error(s.tallies())
+        plain = {"rays_per_s": sp["rays"] / pwall, "ms_per_step": 1e3 * pwall, "eps": eps_p,
+                 "time_to_1pct_s": pwall * (eps_p / 0.01) ** 2}
+        log(f"plain:   {1e3 * pwall:.2f} ms/step, {sp['rays'] / pwall / 1e9:.3f} Grays/s, eps {eps_p:.4g}")
+
+    ncell = s.ncell
+    alg_bytes = steps * B_STEP + rays * B_RAY + ncell * B_CELL
+    march_ms = float(np.mean(march))
+    achieved = alg_bytes / (march_ms * 1e-3) / 1e9
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "k_batched<false> (march)", "kernel_ms": round(march_ms, 4),
+            "algorithmic_bytes": int(alg_bytes)}
+
+    cpu = None
+    if not args.no_cpu:
+        try:
+            cpu = cpu_baseline(p, tab, T, args.cpu_seconds)
+            log(f"cpu:     {cpu['value'] / 1e6:.2f} Mrays/s on {cpu['cores']} cores")
+        except Exception as e:  # the baseline is reported, never required
+            log(f"cpu baseline unavailable: {e}")
+
+    out = {
+        "metric": "rays/s (batched solver) and time to 1% std-error of div q_r; batched vs plain, 1 MI355X",
+        "value": value,
+        "unit": "rays/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": ms_per_step,
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "fp32 march, fp64 tallies",
+        "data": "synthetic (H2O-like tables, parabolic channel T field with 2% smoothed noise)",
+        "config": {"workload": f"narrow-band correlated-k OERM {p.imax}x{p.jmax}x{p.kmax}, {p.nB} bands x "
+                               f"{p.nQ} g x {p.nT} T, photmax {p.photmax}, nVar {p.nVar}, srt {p.srt}, "
+                               f"grid_num {p.grid_num}" if not p.grey else
+                               f"grey OERM {p.imax}x{p.jmax}x{p.kmax}, photmax {p.photmax}, nVar {p.nVar}",
+                   "cells": ncell, "rays_per_step": int(rays), "steps_per_ray": steps / rays},
+        "eps": eps,
+        "eps_p95_cell": p95,
+        "time_to_1pct_s": (wall / args.steps) * (eps / 0.01) ** 2,
+        "plain": plain,
+        "speedup_vs_plain": (value / plain["rays_per_s"]) if plain else None,
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(out), flush=True)
+    s.close()
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

@@ -95,7 +95,8 @@ typedef struct rc_ray {
 
 typedef struct rc_stats {
     double   kernel_ms;         /* device time of the last run (HIP events)   */
-    double   march_ms;          /* time of the march kernel(s) alone          */
+    double   march_ms;          /* device time of the march kernels alone     */
+                                /* (band sort and march, not finalize)        */
     int64_t  rays;              /* rays marched in the last run               */
     int64_t  steps;             /* march steps executed in the last run       */
     int64_t  capped;            /* rays that hit the last level's step cap    */
@@ -121,6 +122,17 @@ void get_results_(float *resfort, float *varfort, int *rank);
 /* ---- 2. extensions ----------------------------------------------------- */
 
 void        rc_default_params(rc_params *p);
+/* fill p from the values of the reference's compile-time macros, passed by
+ * position (param.h/NarrowBand.h macro names equal the field names, so a
+ * file that includes those headers cannot name the fields); used by
+ * radchan_param_shim.h */
+int         rc_params_compiled(rc_params *p, int imax_, int jmax_, int kmax_, int p_col_, float Lx_, float Ly_,
+                               float Lz_, double fact_mesh_, int nVar_, int photmax_, double chanHeight_,
+                               int grid_num_, const int *maxi_, const int *maxj_, const int *maxk_,
+                               const int *maxs_, int srt_, int grey_, int calcwalls_, const float *eps6,
+                               const float *Tw6, int adiabw_, int adiabe_, float toll_, float tollS_,
+                               const int *bd6, int nT_, int nQ_, int nB_, float abscoeff_, float Tmins_,
+                               float wvmax_);
 int         rc_set_params(const rc_params *p);       /* used by mc_gpu_        */
 int         rc_get_params(rc_params *p);
 int         rc_set_table_dir(const char *dir);       /* default "properties"   */
@@ -166,6 +178,11 @@ int rc_trace(rc_ctx *c, int surface, const int32_t *rays, int n, rc_ray *out);
 /* scalar set-up values of snapshot s: Tmax, kappamax, Ibmax, Ibmax_w,
  * Ibmax_e; ints tm, tstar */
 int rc_info(rc_ctx *c, int s, float *f5, int32_t *i2);
+/* schedule of the batched solver: persistent workgroups (one wavefront
+ * each) per compute unit, and the number of idle lanes that triggers a
+ * refill; 0 selects the default.  Chosen by the autotuner. */
+int rc_set_schedule(rc_ctx *c, int wg_per_cu, int threshold);
+int rc_get_schedule(rc_ctx *c, int32_t *wg_per_cu, int32_t *threshold);
 int rc_device_synchronize(void);
 
 #ifdef __cplusplus

@@ -24,12 +24,14 @@
 
 namespace rc {
 void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int nsnap, int surface,
-                    double *de, unsigned long long *stats, int *queue, int wg_per_cu, int rays_per_lane);
+                    double *de, unsigned long long *stats, int *queue, int wg_per_cu, int threshold);
 }
 
 using namespace rc;
 
 namespace {
+
+constexpr int kGuideM = 2048;   /* guide-table resolution for the band CDFs */
 
 #define RC_HIP(call)                                                                         \
     do {                                                                                     \
@@ -82,7 +84,7 @@ struct TableSet {
 struct SnapState {
     float Tmax = 0.f, kappamax = 0.f, Ibmax = 0.f, Ibmax_w = 0.f, Ibmax_e = 0.f;
     int tm = 0, tstar = 0;
-    DevBuf T0, cdf_b, cdf_g, cdf_bs;
+    DevBuf T0, cdf_b, cdf_g, cdf_bs, guide_b, guide_bs;
     DevBuf lvT[RC_MAXLEV], lvTw[RC_MAXLEV], lvTz[RC_MAXLEV];
     std::unique_ptr<TableSet> grey;     /* grey runs: tables follow Tmax */
     Tables greyhost;
@@ -94,7 +96,7 @@ struct rc_ctx {
     rc_params p{};
     int kloc = 0, ncell = 0, nsurf = 0, zstart = 1;
     hipStream_t stream = nullptr;
-    hipEvent_t ev0 = nullptr, ev1 = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evm0 = nullptr, evm1 = nullptr;
     Tables tab;
     bool have_tables = false;
     std::vector<int32_t> idx, rank_of;
@@ -108,7 +110,7 @@ struct rc_ctx {
     DevBuf tr_rays, tr_unit, tr_ulist, tr_out;
     rc_stats last{};
     bool launched = false;
-    int wg_per_cu = 0, rays_per_lane = 0;   /* batched schedule; 0 = default */
+    int wg_per_cu = 0, threshold = 0;       /* batched schedule; 0 = default */
 };
 
 namespace {
@@ -198,6 +200,18 @@ int snapshot_rows(rc_ctx *c, const Tables &t, SnapState &s) {
         cs[b] = acc;
     }
     for (int b = 0; b < nB; ++b) cs[b] = cs[b] / acc;
+    /* guide tables: g[m] = first index with cdf > m/M, m = 0..M */
+    auto guide = [&](const std::vector<float> &cdf) {
+        const int M = kGuideM;
+        std::vector<int32_t> g(M + 1);
+        for (int q = 0; q <= M; ++q) {
+            const float thr = (float)q / (float)M;
+            g[q] = (int32_t)(std::upper_bound(cdf.begin(), cdf.end(), thr) - cdf.begin());
+        }
+        return g;
+    };
+    if (int e = upload(s.guide_b, guide(cb), c->stream)) return e;
+    if (int e = upload(s.guide_bs, guide(cs), c->stream)) return e;
     if (int e = upload(s.cdf_b, cb, c->stream)) return e;
     if (int e = upload(s.cdf_g, cg, c->stream)) return e;
     if (int e = upload(s.cdf_bs, cs, c->stream)) return e;
@@ -231,6 +245,7 @@ int push_snaps(rc_ctx *c) {
         d.kb = ts.kb.as<float2>(); d.bfix = ts.bfix.as<float>();
         d.cdf_b = S.cdf_b.as<float>(); d.cdf_g = S.cdf_g.as<float>(); d.cdf_bs = S.cdf_bs.as<float>();
         d.cdf_gs = ts.cdf_gs.as<float>(); d.rank_of = ts.rank_of.as<int>();
+        d.guide_b = S.guide_b.as<int>(); d.guide_bs = S.guide_bs.as<int>(); d.gM = kGuideM;
         d.T0 = S.T0.as<float>();
         d.x0 = c->x0.as<float>(); d.y0 = c->y0.as<float>(); d.z0 = c->z0.as<float>();
         for (int g = 0; g < p.grid_num; ++g) {
@@ -350,7 +365,8 @@ rc_ctx *rc_create(const rc_params *pp) {
     c->ncell = p.imax * p.jmax * c->kloc;
     c->nsurf = p.jmax * c->kloc;
     if (hipStreamCreateWithFlags(&c->stream, hipStreamNonBlocking) != hipSuccess ||
-        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess) {
+        hipEventCreate(&c->ev0) != hipSuccess || hipEventCreate(&c->ev1) != hipSuccess ||
+        hipEventCreate(&c->evm0) != hipSuccess || hipEventCreate(&c->evm1) != hipSuccess) {
         set_error("cannot create HIP stream/events");
         return nullptr;
     }
@@ -372,6 +388,8 @@ void rc_destroy(rc_ctx *c) {
     if (c->stream) (void)hipStreamSynchronize(c->stream);
     if (c->ev0) (void)hipEventDestroy(c->ev0);
     if (c->ev1) (void)hipEventDestroy(c->ev1);
+    if (c->evm0) (void)hipEventDestroy(c->evm0);
+    if (c->evm1) (void)hipEventDestroy(c->evm1);
     hipStream_t st = c->stream;
     delete c;
     if (st) (void)hipStreamDestroy(st);
@@ -460,6 +478,7 @@ int rc_run(rc_ctx *c, int variant) {
     int launches = 0;
     RC_HIP(hipMemsetAsync(c->stats.p, 0, 8 * sizeof(unsigned long long), c->stream));
     RC_HIP(hipEventRecord(c->ev0, c->stream));
+    RC_HIP(hipEventRecord(c->evm0, c->stream));
     if (variant == RC_VARIANT_PLAIN) {
         if (cm.srt) {
             const size_t nk = (size_t)std::max(c->ncell, 2 * c->nsurf) * p.photmax;
@@ -485,14 +504,15 @@ int rc_run(rc_ctx *c, int variant) {
         }
     } else {
         launch_batched(c->stream, cm, ds, nsnap, 0, c->de.as<double>(), c->stats.as<unsigned long long>(),
-                       c->queue.as<int>(), c->wg_per_cu, c->rays_per_lane);
+                       c->queue.as<int>(), c->wg_per_cu, c->threshold);
         ++launches;
         if (walls) {
             launch_batched(c->stream, cm, ds, nsnap, 1, c->ds.as<double>(), c->stats.as<unsigned long long>(),
-                           c->queue.as<int>() + 16, c->wg_per_cu, c->rays_per_lane);
+                           c->queue.as<int>() + 16, c->wg_per_cu, c->threshold);
             ++launches;
         }
     }
+    RC_HIP(hipEventRecord(c->evm1, c->stream));
     for (int s = 0; s < nsnap; ++s) {
         launch_finalize(c->stream, c->de.as<double>() + (size_t)s * c->ncell * p.nVar, c->ncell, p.nVar,
                         c->mean.as<float>() + (size_t)s * c->ncell, c->sdev.as<float>() + (size_t)s * c->ncell);
@@ -517,12 +537,13 @@ int rc_sync(rc_ctx *c, rc_stats *st) {
     if (!c) { set_error("null context"); return -1; }
     if (c->launched) {
         RC_HIP(hipStreamSynchronize(c->stream));
-        float ms = 0.f;
+        float ms = 0.f, mms = 0.f;
         RC_HIP(hipEventElapsedTime(&ms, c->ev0, c->ev1));
+        RC_HIP(hipEventElapsedTime(&mms, c->evm0, c->evm1));
         unsigned long long h[8];
         RC_HIP(hipMemcpy(h, c->stats.p, sizeof h, hipMemcpyDeviceToHost));
         c->last.kernel_ms = ms;
-        c->last.march_ms = ms;
+        c->last.march_ms = mms;
         c->last.rays = (int64_t)h[0];
         c->last.steps = (int64_t)h[1];
         c->last.capped = (int64_t)h[2];
@@ -555,6 +576,23 @@ int rc_surface_results(rc_ctx *c, int s, float *mean, float *sdev, double *de) {
     if (mean) RC_HIP(hipMemcpy(mean, c->smean.as<float>() + (size_t)s * n, n * 4, hipMemcpyDeviceToHost));
     if (sdev) RC_HIP(hipMemcpy(sdev, c->ssdev.as<float>() + (size_t)s * n, n * 4, hipMemcpyDeviceToHost));
     if (de) RC_HIP(hipMemcpy(de, c->ds.as<double>() + (size_t)s * n * c->p.nVar, n * c->p.nVar * 8, hipMemcpyDeviceToHost));
+    return 0;
+}
+
+int rc_set_schedule(rc_ctx *c, int wg_per_cu, int threshold) {
+    if (!c || wg_per_cu < 0 || wg_per_cu > 64 || threshold < 0 || threshold > 64) {
+        set_error("schedule: wg_per_cu in 0..64, threshold in 0..64 (0 = default)");
+        return -1;
+    }
+    c->wg_per_cu = wg_per_cu;
+    c->threshold = threshold;
+    return 0;
+}
+
+int rc_get_schedule(rc_ctx *c, int32_t *wg_per_cu, int32_t *threshold) {
+    if (!c) { set_error("null context"); return -1; }
+    *wg_per_cu = c->wg_per_cu;
+    *threshold = c->threshold;
     return 0;
 }
 
@@ -638,6 +676,32 @@ int rc_set_params(const rc_params *p) {
     gparams() = q;
     g_params_set = true;
     if (g_ctx && !g_pending) { rc_destroy(g_ctx); g_ctx = nullptr; g_tab_dir.clear(); }
+    return 0;
+}
+
+int rc_params_compiled(rc_params *p, int imax_, int jmax_, int kmax_, int p_col_, float Lx_, float Ly_, float Lz_,
+                       double fact_mesh_, int nVar_, int photmax_, double chanHeight_, int grid_num_,
+                       const int *maxi_, const int *maxj_, const int *maxk_, const int *maxs_, int srt_, int grey_,
+                       int calcwalls_, const float *eps6, const float *Tw6, int adiabw_, int adiabe_, float toll_,
+                       float tollS_, const int *bd6, int nT_, int nQ_, int nB_, float abscoeff_, float Tmins_,
+                       float wvmax_) {
+    if (!p || grid_num_ < 1 || grid_num_ > RC_MAXLEV) { set_error("rc_params_compiled: bad grid_num"); return -1; }
+    default_params(*p);
+    p->imax = imax_; p->jmax = jmax_; p->kmax = kmax_; p->p_col = p_col_;
+    p->Lx = Lx_; p->Ly = Ly_; p->Lz = Lz_;
+    p->fact_mesh = fact_mesh_; p->nVar = nVar_; p->photmax = photmax_; p->chanHeight = chanHeight_;
+    p->grid_num = grid_num_;
+    for (int g = 0; g < RC_MAXLEV; ++g) {
+        const bool on = g < grid_num_;
+        p->maxi[g] = on ? maxi_[g] : 1; p->maxj[g] = on ? maxj_[g] : 1;
+        p->maxk[g] = on ? maxk_[g] : 1; p->maxs[g] = on ? maxs_[g] : 1;
+    }
+    p->srt = srt_; p->grey = grey_; p->calcwalls = calcwalls_; p->grey_true = 0;
+    for (int f = 0; f < 6; ++f) { p->eps[f] = eps6[f]; p->Tw[f] = Tw6[f]; p->bd[f] = bd6[f]; }
+    p->adiabw = adiabw_; p->adiabe = adiabe_;
+    p->toll = toll_; p->tollS = tollS_;
+    p->nT = nT_; p->nQ = nQ_; p->nB = nB_;
+    p->abscoeff = abscoeff_; p->Tmins = Tmins_; p->wvmax = wvmax_;
     return 0;
 }
 

@@ -15,6 +15,17 @@
 
 namespace rcd {
 
+/* load through a pointer known to address global memory (pointers read from
+ * descriptors in memory are otherwise generic and compile to FLAT loads) */
+template <class T> __device__ __forceinline__ T gld(const T *p, size_t i) {
+    return ((const __attribute__((address_space(1))) T *)p)[i];
+}
+typedef float rc_v2f __attribute__((ext_vector_type(2)));
+__device__ __forceinline__ float2 gld(const float2 *p, size_t i) {
+    const rc_v2f v = ((const __attribute__((address_space(1))) rc_v2f *)p)[i];
+    return make_float2(v.x, v.y);
+}
+
 /* ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers:
  * as easy as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies with
  * a Weyl-sequence key schedule. ---------------------------------------- */
@@ -78,10 +89,11 @@ __device__ __forceinline__ void sincos2pi(float u, float &s, float &c) {
 /* expm1(x) and exp(x): x = n ln2 + r with |r| <= ln2/2 (Cody-Waite, fma),
  * expm1(r) from its Taylor series through r^9/9!, then rescaled by 2^n. */
 __device__ __forceinline__ void expm1_exp(float x, float &em1, float &e) {
-    if (x < -87.0f) { em1 = -1.0f; e = 0.0f; return; }
-    if (x > 88.0f) x = 88.0f;
-    const float n = rintf(x * 1.44269502f);
-    float r = fmaf(-n, 0.693145752f, x);
+    const bool under = x < -87.0f;
+    float xc = under ? -87.0f : x;
+    xc = (xc > 88.0f) ? 88.0f : xc;
+    const float n = rintf(xc * 1.44269502f);
+    float r = fmaf(-n, 0.693145752f, xc);
     r = fmaf(-n, 1.42860677e-6f, r);
     float p = fmaf(r, (float)(1.0 / 362880.0), (float)(1.0 / 40320.0));
     p = fmaf(r, p, (float)(1.0 / 5040.0));
@@ -93,8 +105,10 @@ __device__ __forceinline__ void expm1_exp(float x, float &em1, float &e) {
     p = fmaf(r * r, p, r);
     const int ni = (int)n;
     const float sc = __int_as_float((ni + 127) << 23);
-    e = fmaf(sc, p, sc);
-    em1 = (ni == 0) ? p : fmaf(sc, p, sc - 1.0f);
+    const float ev = fmaf(sc, p, sc);
+    const float mv = (ni == 0) ? p : fmaf(sc, p, sc - 1.0f);
+    e = under ? 0.0f : ev;
+    em1 = under ? -1.0f : mv;
 }
 
 /* first index with cdf[i] > R; R == 1 and past-the-end map to n-1 */
@@ -102,11 +116,38 @@ __device__ __forceinline__ int upper_bound(const float *__restrict__ cdf, int n,
     int lo = 0, hi = n;
     while (lo < hi) {
         const int mid = (lo + hi) >> 1;
-        if (R < cdf[mid]) hi = mid; else lo = mid + 1;
+        if (R < gld(cdf, mid)) hi = mid; else lo = mid + 1;
     }
     if (R == 1.0f) lo = n - 1;
     if (lo > n - 1) lo = n - 1;
     return lo;
+}
+
+/* upper_bound with a guide table g[m] = first index with cdf > m/M
+ * (Chen and Asau, 1974): the answer lies in [g[m-1], g[m+2]] for
+ * m = floor(R M), one slot of slack either side against rounding of R M.
+ * Returns exactly what upper_bound returns. */
+__device__ __forceinline__ int upper_bound_guided(const float *__restrict__ cdf, int n, const int *__restrict__ g,
+                                                  int M, float R) {
+    int m = (int)(R * (float)M);
+    int lo = gld(g, max(m - 1, 0)), hi = gld(g, min(m + 2, M));
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if (R < gld(cdf, mid)) hi = mid; else lo = mid + 1;
+    }
+    if (R == 1.0f) lo = n - 1;
+    if (lo > n - 1) lo = n - 1;
+    return lo;
+}
+
+/* upper_bound over a short non-decreasing CDF: count of entries <= R, all
+ * loads issued together */
+__device__ __forceinline__ int upper_bound_short(const float *__restrict__ cdf, int n, float R) {
+    int c = 0;
+    for (int q = 0; q < n; ++q) c += (gld(cdf, q) <= R) ? 1 : 0;
+    if (R == 1.0f) c = n - 1;
+    if (c > n - 1) c = n - 1;
+    return c;
 }
 
 __device__ __forceinline__ float lerp(float a, float b, float w) { return fmaf(w, b - a, a); }
@@ -115,10 +156,11 @@ __device__ __forceinline__ float lerp(float a, float b, float w) { return fmaf(w
  * spacing 1/invdT, clamped to the first and last rows */
 __device__ __forceinline__ void tpos(float T, float Tnb0, float invdT, int nT, int &t, float &a) {
     const float pos = (T - Tnb0) * invdT;
-    if (!(pos > 0.0f)) { t = 0; a = 0.0f; return; }
-    if (pos >= (float)(nT - 1)) { t = nT - 1; a = 0.0f; return; }
-    t = (int)pos;
-    a = pos - (float)t;
+    const bool lo = !(pos > 0.0f);
+    const bool hi = pos >= (float)(nT - 1);
+    const int ti = (int)pos;
+    t = lo ? 0 : (hi ? nT - 1 : ti);
+    a = (lo || hi) ? 0.0f : pos - (float)ti;
 }
 
 /* isotropic direction from (u0, u1): phi = 2 pi u0, cos theta = 1 - 2 u1 */

@@ -49,6 +49,9 @@ struct DevSnap {
     const float *cdf_bs;           /* [nB] wall band CDF, row t*          */
     const float *cdf_gs;           /* [nQ] cumulative quadrature weights  */
     const int *rank_of;            /* [nB] sorted rank of each band       */
+    const int *guide_b;            /* [gM+1] first index with cdf_b > m/gM */
+    const int *guide_bs;           /* [gM+1] same for cdf_bs               */
+    int gM, pad2;
     const float *T0;               /* ghost-padded input field            */
     const float *x0, *y0, *z0;     /* level-0 cell centres                */
     DevLevel lev[RC_MAXLEV];
@@ -64,6 +67,7 @@ void launch_surface_plain(hipStream_t st, const DevCommon &cm, const DevSnap *sn
 void launch_trace(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int surface, const int32_t *rays,
                   const int32_t *unit, int n, const uint16_t *keys, rc_ray *out);
 void launch_finalize(hipStream_t st, const double *de, int n, int nVar, float *mean, float *sdev);
+void launch_zero(hipStream_t st, double *p, size_t n);
 
 }  // namespace rc
 

@@ -24,10 +24,11 @@ using namespace rcd;
 /* (rank, g) key of one band draw: band from the Planck-weighted band CDF of
  * the row nearest Tmax, g from that band's g CDF (find_band :1103-1169) */
 __device__ __forceinline__ uint32_t band_key(const DevCommon &cm, const DevSnap &sn, bool surface, float u2, float u3) {
-    const int nb = upper_bound(surface ? sn.cdf_bs : sn.cdf_b, cm.nB, u2);
+    const int nb = surface ? upper_bound_guided(sn.cdf_bs, cm.nB, sn.guide_bs, sn.gM, u2)
+                           : upper_bound_guided(sn.cdf_b, cm.nB, sn.guide_b, sn.gM, u2);
     const float *gc = surface ? sn.cdf_gs : sn.cdf_g + (size_t)nb * cm.nQ;
-    const int g = upper_bound(gc, cm.nQ, u3);
-    return (uint32_t)(sn.rank_of[nb] * cm.nQ + g);
+    const int g = (cm.nQ <= 32) ? upper_bound_short(gc, cm.nQ, u3) : upper_bound(gc, cm.nQ, u3);
+    return (uint32_t)(gld(sn.rank_of, nb) * cm.nQ + g);
 }
 
 /* ------------------------------------------------------------------------ */
@@ -47,7 +48,7 @@ struct En {
 __device__ __forceinline__ void kb_at(const DevSnap &sn, const float2 *__restrict__ kb, int nT, float T, float &K, float &B) {
     int t; float a;
     tpos(T, sn.Tnb0, sn.invdT, nT, t, a);
-    const float2 q0 = kb[t], q1 = kb[t + 1];
+    const float2 q0 = gld(kb, t), q1 = gld(kb, t + 1);
     K = lerp(q0.x, q1.x, a);
     B = lerp(q0.y, q1.y, a);
 }
@@ -64,6 +65,109 @@ __device__ __forceinline__ void wall_hit(const DevSnap &sn, const En &e, int nT,
     r.tra = 0.f;
 }
 
+/* boundary condition of the face the ray just crossed (the one index that
+ * left 1..n): periodic, black (reference wall, adiabatic midpoint or fixed
+ * temperature), specular, black at the adjacent-cell temperature (z), lost
+ * (z) -- mc_gpu_noint.cu:375-654 */
+__device__ __forceinline__ void boundary(const DevCommon &cm, const DevSnap &sn, const DevLevel &L, Ray &r, const En &e,
+                                      float &de, int &status) {
+    const int nT = cm.nT;
+    int face;
+    if (r.ic == 0) face = FW;
+    else if (r.ic == L.im + 1) face = FE;
+    else if (r.jc == 0) face = FS;
+    else if (r.jc == L.jm + 1) face = FN;
+    else if (r.kc == 0) face = FB;
+    else face = FT;
+    const int bd = cm.bd[face];
+    if (bd == 1) {
+        switch (face) {
+            case FW: r.ic = L.im; r.xp = cm.Lx; break;
+            case FE: r.ic = 1; r.xp = 0.f; break;
+            case FS: r.jc = L.jm; r.yp = cm.Ly; break;
+            case FN: r.jc = 1; r.yp = 0.f; break;
+            case FB: r.kc = L.km; r.zp = cm.Lz; break;
+            default: r.kc = 1; r.zp = 0.f; break;
+        }
+    } else if (bd == 2) {
+        float Bw;
+        if ((face == FW && cm.adiab[0]) || (face == FE && cm.adiab[1])) {
+            const float Tw = gld(L.Tw, (size_t)(face == FE) * L.jm * L.km + (size_t)(r.jc - 1) + (size_t)L.jm * (r.kc - 1));
+            float Kw;
+            kb_at(sn, e.kb, nT, Tw, Kw, Bw);
+        } else {
+            Bw = gld(sn.bfix, (size_t)face * cm.nB + e.rank);
+        }
+        wall_hit(sn, e, nT, r, Bw, de);
+        status = RC_RAY_WALL;
+    } else if (bd == 3) {
+        switch (face) {
+            case FW: r.xp = 0.f; r.sx = -r.sx; r.isx = -r.isx; r.ic = 1; r.f0 = r.sx < 0.f; break;
+            case FE: r.xp = cm.Lx; r.sx = -r.sx; r.isx = -r.isx; r.ic = L.im; r.f0 = r.sx < 0.f; break;
+            case FS: r.yp = 0.f; r.sy = -r.sy; r.isy = -r.isy; r.jc = 1; r.f1 = r.sy < 0.f; break;
+            case FN: r.yp = cm.Ly; r.sy = -r.sy; r.isy = -r.isy; r.jc = L.jm; r.f1 = r.sy < 0.f; break;
+            case FB: r.zp = 0.f; r.sz = -r.sz; r.isz = -r.isz; r.kc = 1; r.f2 = r.sz < 0.f; break;
+            default: r.zp = cm.Lz; r.sz = -r.sz; r.isz = -r.isz; r.kc = L.km; r.f2 = r.sz < 0.f; break;
+        }
+    } else if (bd == 4) {
+        const float Tw = gld(L.Tz, (size_t)(face == FT) * L.im * L.jm + (size_t)(r.ic - 1) + (size_t)L.im * (r.jc - 1));
+        float Kw, Bw;
+        kb_at(sn, e.kb, nT, Tw, Kw, Bw);
+        wall_hit(sn, e, nT, r, Bw, de);
+        status = RC_RAY_WALL;
+    } else {
+        r.tra = 0.f;
+        status = RC_RAY_LOST;
+    }
+}
+
+/* one march step (one iteration of march_ray's loop, mc_gpu_noint.cu:252-654):
+ * distances to the three candidate faces, position update, attenuation and
+ * reciprocity tally in the current cell, transmissivity cut-off, then one
+ * index move; the boundary condition only when an index leaves the grid.
+ * Written with selects so that the common path has no divergent branch. */
+__device__ __forceinline__ void march_step(const DevCommon &cm, const DevSnap &sn, const DevLevel &L, Ray &r,
+                                           const En &e, float tol, float &de, int &status) {
+    const int nT = cm.nT;
+    const float fx = gld(L.xu, r.ic - r.f0);
+    const float fy = (float)(r.jc - r.f1) * L.dy;
+    const float fz = (float)(r.kc - r.f2) * L.dz;
+    const float dsx = (fx - r.xp) * r.isx;
+    const float dsy = (fy - r.yp) * r.isy;
+    const float dsz = (fz - r.zp) * r.isz;
+    const uint32_t cell = (uint32_t)(r.ic - 1) + (uint32_t)L.im * ((uint32_t)(r.jc - 1) + (uint32_t)L.jm * (uint32_t)(r.kc - 1));
+    const float Tc = gld(L.T, cell);
+    float ds = dsx;
+    ds = (ds > dsy) ? dsy : ds;
+    ds = (ds > dsz) ? dsz : ds;
+    r.xp = (ds == dsx) ? fx : fmaf(ds, r.sx, r.xp);
+    r.yp = (ds == dsy) ? fy : fmaf(ds, r.sy, r.yp);
+    r.zp = (ds == dsz) ? fz : fmaf(ds, r.sz, r.zp);
+    float K, B;
+    kb_at(sn, e.kb, nT, Tc, K, B);
+    float em1, ex;
+    expm1_exp(-(K * ds), em1, ex);
+    const float alpha = -em1;
+    const float fac = (B - e.Bi) * e.invBi;
+    de -= ((e.W * r.tra) * alpha) * fac;
+    const float tra = r.tra * ex;
+    const bool dump = tra < tol;
+    const float dd = (e.W * tra) * fac;
+    de = dump ? de - dd : de;
+    r.tra = dump ? 0.f : tra;
+    status = dump ? RC_RAY_ABSORBED : status;
+    /* one index moves along the crossed axis */
+    const bool xa = dsx < dsy && dsx < dsz;
+    const bool ya = !(dsx < dsy) && dsy < dsz;
+    const bool za = !xa && !ya;
+    r.ic += xa ? (r.f0 ? -1 : 1) : 0;
+    r.jc += ya ? (r.f1 ? -1 : 1) : 0;
+    r.kc += za ? (r.f2 ? -1 : 1) : 0;
+    const bool out = (uint32_t)(r.ic - 1) >= (uint32_t)L.im || (uint32_t)(r.jc - 1) >= (uint32_t)L.jm ||
+                     (uint32_t)(r.kc - 1) >= (uint32_t)L.km;
+    if (out) boundary(cm, sn, L, r, e, de, status);
+}
+
 /* one grid level of the march (march_ray, mc_gpu_noint.cu:245-658);
  * returns the steps taken on this level */
 template <bool TR>
@@ -71,90 +175,10 @@ __device__ __forceinline__ int march_level(const DevCommon &cm, const DevSnap &s
                                            const En &e, float tol, float &de, int &steps, uint32_t &hash,
                                            int &status) {
     int counter = 0;
-    const int nT = cm.nT;
     while (r.tra > tol && counter < L.maxs) {
-        const float fx = L.xu[r.ic - r.f0];
-        const float fy = (float)(r.jc - r.f1) * L.dy;
-        const float fz = (float)(r.kc - r.f2) * L.dz;
-        const float dsx = (fx - r.xp) * r.isx;
-        const float dsy = (fy - r.yp) * r.isy;
-        const float dsz = (fz - r.zp) * r.isz;
-        const float Tc = L.T[(size_t)(r.ic - 1) + (size_t)L.im * ((size_t)(r.jc - 1) + (size_t)L.jm * (size_t)(r.kc - 1))];
-        float ds = dsx;
-        if (ds > dsy) ds = dsy;
-        if (ds > dsz) ds = dsz;
-        r.xp = (ds == dsx) ? fx : fmaf(ds, r.sx, r.xp);
-        r.yp = (ds == dsy) ? fy : fmaf(ds, r.sy, r.yp);
-        r.zp = (ds == dsz) ? fz : fmaf(ds, r.sz, r.zp);
-        float K, B;
-        kb_at(sn, e.kb, nT, Tc, K, B);
-        float em1, ex;
-        expm1_exp(-(K * ds), em1, ex);
-        const float alpha = -em1;
-        const float fac = (B - e.Bi) * e.invBi;
-        de -= ((e.W * r.tra) * alpha) * fac;
-        r.tra = r.tra * ex;
-        if (r.tra < tol) {
-            de -= (e.W * r.tra) * fac;
-            r.tra = 0.f;
-            status = RC_RAY_ABSORBED;
-        }
         if (TR) hash = fnv(fnv(fnv(hash, (uint32_t)r.ic), (uint32_t)r.jc), (uint32_t)r.kc);
+        march_step(cm, sn, L, r, e, tol, de, status);
         steps += 1;
-        /* one index moves along the crossed axis; boundary conditions */
-        int face = -1;
-        if (dsx < dsy && dsx < dsz) {
-            if (r.f0) { if (--r.ic == 0) face = FW; }
-            else { if (++r.ic == L.im + 1) face = FE; }
-        } else if (!(dsx < dsy) && dsy < dsz) {
-            if (r.f1) { if (--r.jc == 0) face = FS; }
-            else { if (++r.jc == L.jm + 1) face = FN; }
-        } else {
-            if (r.f2) { if (--r.kc == 0) face = FB; }
-            else { if (++r.kc == L.km + 1) face = FT; }
-        }
-        if (face >= 0) {
-            const int bd = cm.bd[face];
-            if (bd == 1) {
-                switch (face) {
-                    case FW: r.ic = L.im; r.xp = cm.Lx; break;
-                    case FE: r.ic = 1; r.xp = 0.f; break;
-                    case FS: r.jc = L.jm; r.yp = cm.Ly; break;
-                    case FN: r.jc = 1; r.yp = 0.f; break;
-                    case FB: r.kc = L.km; r.zp = cm.Lz; break;
-                    default: r.kc = 1; r.zp = 0.f; break;
-                }
-            } else if (bd == 2) {
-                float Bw;
-                if ((face == FW && cm.adiab[0]) || (face == FE && cm.adiab[1])) {
-                    const float Tw = L.Tw[(size_t)(face == FE) * L.jm * L.km + (size_t)(r.jc - 1) + (size_t)L.jm * (r.kc - 1)];
-                    float Kw;
-                    kb_at(sn, e.kb, nT, Tw, Kw, Bw);
-                } else {
-                    Bw = sn.bfix[(size_t)face * cm.nB + e.rank];
-                }
-                wall_hit(sn, e, nT, r, Bw, de);
-                status = RC_RAY_WALL;
-            } else if (bd == 3) {
-                switch (face) {
-                    case FW: r.xp = 0.f; r.sx = -r.sx; r.isx = -r.isx; r.ic = 1; r.f0 = r.sx < 0.f; break;
-                    case FE: r.xp = cm.Lx; r.sx = -r.sx; r.isx = -r.isx; r.ic = L.im; r.f0 = r.sx < 0.f; break;
-                    case FS: r.yp = 0.f; r.sy = -r.sy; r.isy = -r.isy; r.jc = 1; r.f1 = r.sy < 0.f; break;
-                    case FN: r.yp = cm.Ly; r.sy = -r.sy; r.isy = -r.isy; r.jc = L.jm; r.f1 = r.sy < 0.f; break;
-                    case FB: r.zp = 0.f; r.sz = -r.sz; r.isz = -r.isz; r.kc = 1; r.f2 = r.sz < 0.f; break;
-                    default: r.zp = cm.Lz; r.sz = -r.sz; r.isz = -r.isz; r.kc = L.km; r.f2 = r.sz < 0.f; break;
-                }
-            } else if (bd == 4) {
-                const float Tw = L.Tz[(size_t)(face == FT) * L.im * L.jm + (size_t)(r.ic - 1) + (size_t)L.im * (r.jc - 1)];
-                float Kw, Bw;
-                kb_at(sn, e.kb, nT, Tw, Kw, Bw);
-                wall_hit(sn, e, nT, r, Bw, de);
-                status = RC_RAY_WALL;
-            } else {
-                r.tra = 0.f;
-                status = RC_RAY_LOST;
-            }
-        }
         counter += 1;
     }
     return counter;
@@ -206,11 +230,11 @@ __device__ __forceinline__ void cell_ijk(const DevCommon &cm, int n, int &i, int
     k = n / (cm.imax * cm.jmax) + cm.zstart;
 }
 
-/* one fluid ray from the centre of cell (i,j,k) (kernel_fluid :115-224) */
-template <bool TR>
-__device__ __forceinline__ float fluid_ray(const DevCommon &cm, const DevSnap &sn, int i, int j, int k, uint32_t cid,
-                                           int v, int h, uint32_t key, bool have_key, float Ti, int &steps,
-                                           rc_ray *rec) {
+/* set-up of one fluid ray from the centre of cell (i,j,k) (kernel_fluid
+ * :115-186): direction, band, per-ray energy constants */
+__device__ __forceinline__ uint32_t fluid_setup(const DevCommon &cm, const DevSnap &sn, int i, int j, int k,
+                                                uint32_t cid, int v, int h, uint32_t key, bool have_key, float Ti,
+                                                Ray &r, En &e) {
     const U4 u = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)v, cid, TAG_FLUID + 16u * (uint32_t)sn.snap);
     if (!have_key) key = band_key(cm, sn, false, u.c, u.d);
     float sx, sy, sz;
@@ -218,33 +242,19 @@ __device__ __forceinline__ float fluid_ray(const DevCommon &cm, const DevSnap &s
     if (sx == 0.f) sx = 1e-10f;
     if (sy == 0.f) sy = 1e-10f;
     if (sz == 0.f) sz = 1e-10f;
-    Ray r;
     set_dir(r, sx, sy, sz);
     r.ic = i; r.jc = j; r.kc = k;
-    r.xp = sn.x0[i]; r.yp = sn.y0[j]; r.zp = sn.z0[k];
+    r.xp = gld(sn.x0, i); r.yp = gld(sn.y0, j); r.zp = gld(sn.z0, k);
     r.tra = 1.0f;
-    En e;
     energy(cm, sn, key, Ti, false, 0, e);
-    float de = 0.f;
-    int st = 0, status, level;
-    uint32_t hash = 2166136261u;
-    march_all<TR>(cm, sn, r, e, cm.toll, de, st, hash, status, level);
-    steps += st;
-    if (TR) {
-        rec->nb = e.rank; rec->g = (int)(key % (uint32_t)cm.nQ);
-        rec->ic = r.ic; rec->jc = r.jc; rec->kc = r.kc; rec->level = level;
-        rec->steps = st; rec->status = status; rec->path_hash = hash;
-        rec->sx = sx; rec->sy = sy; rec->sz = sz; rec->tra = r.tra; rec->de = de;
-        rec->de64 = 0.0; rec->sumabs = 0.0;
-    }
-    return de;
+    return key;
 }
 
-/* one ray from the centre of wall cell (j,k) (kernel_surface :1381-1462) */
-template <bool TR>
-__device__ __forceinline__ float surface_ray(const DevCommon &cm, const DevSnap &sn, int bound, int j, int k,
-                                             uint32_t cid, int v, int h, uint32_t key, bool have_key, float Tw,
-                                             int &steps, rc_ray *rec) {
+/* set-up of one ray from the centre of wall cell (j,k) (kernel_surface
+ * :1381-1441) */
+__device__ __forceinline__ uint32_t surface_setup(const DevCommon &cm, const DevSnap &sn, int bound, int j, int k,
+                                                  uint32_t cid, int v, int h, uint32_t key, bool have_key, float Tw,
+                                                  Ray &r, En &e) {
     const U4 u = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)v, cid,
                        (bound ? TAG_EAST : TAG_WEST) + 16u * (uint32_t)sn.snap);
     if (!have_key) key = band_key(cm, sn, true, u.c, u.d);
@@ -253,43 +263,78 @@ __device__ __forceinline__ float surface_ray(const DevCommon &cm, const DevSnap 
     if (fabsf(sx) <= 1e-10) sx = (float)(1 - 2 * bound) * 1e-10f;
     if (fabsf(sy) <= 1e-10) sy = (float)((sy > 0.f) - (sy <= 0.f)) * 1e-10f;
     if (fabsf(sz) <= 1e-10) sz = (float)((sz > 0.f) - (sz <= 0.f)) * 1e-10f;
-    Ray r;
     set_dir(r, sx, sy, sz);
     r.ic = bound ? cm.imax : 1; r.jc = j; r.kc = k;
-    r.xp = bound ? cm.Lx : 0.f; r.yp = sn.y0[j]; r.zp = sn.z0[k];
+    r.xp = bound ? cm.Lx : 0.f; r.yp = gld(sn.y0, j); r.zp = gld(sn.z0, k);
     r.tra = 1.0f;
-    En e;
     energy(cm, sn, key, Tw, true, bound, e);
     /* on-face index adjustment (:1429-1441) */
     const DevLevel &L0 = sn.lev[0];
-    if (r.f0 && r.xp == L0.xu[r.ic - 1]) r.ic -= 1;
+    if (r.f0 && r.xp == gld(L0.xu, r.ic - 1)) r.ic -= 1;
     if (r.f1 && r.yp == (float)(r.jc - 1) * L0.dy) r.jc -= 1;
     if (r.f2 && r.zp == (float)(r.kc - 1) * L0.dz) r.kc -= 1;
-    if (!r.f0 && r.xp == L0.xu[r.ic]) r.ic += 1;
+    if (!r.f0 && r.xp == gld(L0.xu, r.ic)) r.ic += 1;
     if (!r.f1 && r.yp == (float)r.jc * L0.dy) r.jc += 1;
     if (!r.f2 && r.zp == (float)r.kc * L0.dz) r.kc += 1;
+    return key;
+}
+
+template <bool TR>
+__device__ __forceinline__ void fill_record(const DevCommon &cm, const Ray &r, const En &e, uint32_t key, int st,
+                                            int status, int level, uint32_t hash, float de, rc_ray *rec) {
+    if (TR) {
+        rec->nb = e.rank; rec->g = (int)(key % (uint32_t)cm.nQ);
+        rec->ic = r.ic; rec->jc = r.jc; rec->kc = r.kc; rec->level = level;
+        rec->steps = st; rec->status = status; rec->path_hash = hash;
+        rec->tra = r.tra; rec->de = de;
+        rec->de64 = 0.0; rec->sumabs = 0.0;
+    }
+}
+
+/* one fluid ray from the centre of cell (i,j,k) (kernel_fluid :115-224) */
+template <bool TR>
+__device__ __forceinline__ float fluid_ray(const DevCommon &cm, const DevSnap &sn, int i, int j, int k, uint32_t cid,
+                                           int v, int h, uint32_t key, bool have_key, float Ti, int &steps,
+                                           rc_ray *rec) {
+    Ray r;
+    En e;
+    key = fluid_setup(cm, sn, i, j, k, cid, v, h, key, have_key, Ti, r, e);
+    if (TR) { rec->sx = r.sx; rec->sy = r.sy; rec->sz = r.sz; }
+    float de = 0.f;
+    int st = 0, status, level;
+    uint32_t hash = 2166136261u;
+    march_all<TR>(cm, sn, r, e, cm.toll, de, st, hash, status, level);
+    steps += st;
+    fill_record<TR>(cm, r, e, key, st, status, level, hash, de, rec);
+    return de;
+}
+
+/* one ray from the centre of wall cell (j,k) (kernel_surface :1381-1462) */
+template <bool TR>
+__device__ __forceinline__ float surface_ray(const DevCommon &cm, const DevSnap &sn, int bound, int j, int k,
+                                             uint32_t cid, int v, int h, uint32_t key, bool have_key, float Tw,
+                                             int &steps, rc_ray *rec) {
+    Ray r;
+    En e;
+    key = surface_setup(cm, sn, bound, j, k, cid, v, h, key, have_key, Tw, r, e);
+    if (TR) { rec->sx = r.sx; rec->sy = r.sy; rec->sz = r.sz; }
     float de = 0.f;
     int st = 0, status, level;
     uint32_t hash = 2166136261u;
     march_all<TR>(cm, sn, r, e, cm.tollS, de, st, hash, status, level);
     steps += st;
-    if (TR) {
-        rec->nb = e.rank; rec->g = (int)(key % (uint32_t)cm.nQ);
-        rec->ic = r.ic; rec->jc = r.jc; rec->kc = r.kc; rec->level = level;
-        rec->steps = st; rec->status = status; rec->path_hash = hash;
-        rec->sx = sx; rec->sy = sy; rec->sz = sz; rec->tra = r.tra; rec->de = de;
-        rec->de64 = 0.0; rec->sumabs = 0.0;
-    }
+    fill_record<TR>(cm, r, e, key, st, status, level, hash, de, rec);
     return de;
 }
 
 __device__ __forceinline__ float cell_T(const DevCommon &cm, const DevSnap &sn, int i, int j, int k) {
-    return sn.T0[(size_t)i + (size_t)(cm.imax + 2) * ((size_t)j + (size_t)(cm.jmax + 2) * (size_t)k)];
+    return gld(sn.T0, (size_t)i + (size_t)(cm.imax + 2) * ((size_t)j + (size_t)(cm.jmax + 2) * (size_t)k));
 }
 __device__ __forceinline__ float wall_T(const DevCommon &cm, const DevSnap &sn, int bound, int j, int k) {
     const size_t row = (size_t)(cm.imax + 2) * ((size_t)j + (size_t)(cm.jmax + 2) * (size_t)k);
     /* texel coordinate 1.0 (imax+1.0): halfway between ghost and first cell */
-    return bound ? lerp(sn.T0[row + cm.imax], sn.T0[row + cm.imax + 1], 0.5f) : lerp(sn.T0[row], sn.T0[row + 1], 0.5f);
+    return bound ? lerp(gld(sn.T0, row + cm.imax), gld(sn.T0, row + cm.imax + 1), 0.5f)
+                 : lerp(gld(sn.T0, row), gld(sn.T0, row + 1), 0.5f);
 }
 
 }  // namespace rc

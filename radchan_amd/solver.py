@@ -99,6 +99,9 @@ class Solver:
                                   out.ctypes.data_as(C.POINTER(L.Ray))), "rc_trace")
         return out
 
+    def set_schedule(self, wg_per_cu: int = 0, threshold: int = 0) -> None:
+        L.check(self.lib.rc_set_schedule(self.ctx, wg_per_cu, threshold), "rc_set_schedule")
+
     def info(self, s: int = 0) -> dict:
         f = np.zeros(5, np.float32)
         i = np.zeros(2, np.int32)
