@@ -183,6 +183,13 @@ int rc_info(rc_ctx *c, int s, float *f5, int32_t *i2);
  * refill; 0 selects the default.  Chosen by the autotuner. */
 int rc_set_schedule(rc_ctx *c, int wg_per_cu, int threshold);
 int rc_get_schedule(rc_ctx *c, int32_t *wg_per_cu, int32_t *threshold);
+/* record the batched solve (queue reset, tally reset, persistent kernel,
+ * finalize) once as a hipGraph and replay it on later rc_run calls; the
+ * graph is dropped when fields or the schedule change */
+int rc_set_graph(rc_ctx *c, int on);
+/* time the batched solve of the loaded snapshots over a grid of schedules
+ * (workgroups per CU x refill threshold) and keep the fastest */
+int rc_autotune(rc_ctx *c, int32_t *best_wg, int32_t *best_th, float *best_ms);
 int rc_device_synchronize(void);
 
 #ifdef __cplusplus

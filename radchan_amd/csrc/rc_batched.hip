@@ -298,6 +298,15 @@ __global__ void k_zero_f64(double *p, size_t n) {
     for (size_t i = blockIdx.x * (size_t)blockDim.x + threadIdx.x; i < n; i += (size_t)gridDim.x * blockDim.x) p[i] = 0.0;
 }
 
+__global__ void k_zero_u64(unsigned long long *p, int n) {
+    const int i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < n) p[i] = 0ull;
+}
+
+void launch_zero_u64(hipStream_t st, unsigned long long *p, int n) {
+    hipLaunchKernelGGL(k_zero_u64, dim3((n + 63) / 64), dim3(64), 0, st, p, n);
+}
+
 void launch_zero(hipStream_t st, double *p, size_t n) {
     const int blocks = (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 8192));
     hipLaunchKernelGGL(k_zero_f64, dim3(blocks), dim3(256), 0, st, p, n);
@@ -305,10 +314,13 @@ void launch_zero(hipStream_t st, double *p, size_t n) {
 
 void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int nsnap, int surface, double *de,
                     unsigned long long *stats, int *queue, int wg_per_cu, int threshold) {
-    int dev = 0, ncu = 0;
-    (void)hipGetDevice(&dev);
-    (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
-    if (ncu <= 0) ncu = 256;
+    static int ncu = 0;       /* compute units of the (single) device */
+    if (ncu <= 0) {
+        int dev = 0;
+        (void)hipGetDevice(&dev);
+        (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
+        if (ncu <= 0) ncu = 256;
+    }
     if (wg_per_cu <= 0) wg_per_cu = 32;
     if (threshold <= 0) threshold = 8;
     int n2 = 1;

@@ -111,6 +111,15 @@ struct rc_ctx {
     rc_stats last{};
     bool launched = false;
     int wg_per_cu = 0, threshold = 0;       /* batched schedule; 0 = default */
+    bool use_graph = false;                 /* replay the batched run as a hipGraph */
+    int graph_launches = 0;
+    hipGraph_t graph = nullptr;
+    hipGraphExec_t graph_exec = nullptr;
+    void drop_graph() {
+        if (graph_exec) (void)hipGraphExecDestroy(graph_exec);
+        if (graph) (void)hipGraphDestroy(graph);
+        graph_exec = nullptr; graph = nullptr;
+    }
 };
 
 namespace {
@@ -390,6 +399,7 @@ void rc_destroy(rc_ctx *c) {
     if (c->ev1) (void)hipEventDestroy(c->ev1);
     if (c->evm0) (void)hipEventDestroy(c->evm0);
     if (c->evm1) (void)hipEventDestroy(c->evm1);
+    c->drop_graph();
     hipStream_t st = c->stream;
     delete c;
     if (st) (void)hipStreamDestroy(st);
@@ -416,6 +426,7 @@ int rc_set_fields(rc_ctx *c, int nsnap, const float *const *T, int zstart) {
     if (zstart < 1 || zstart + c->kloc - 1 > p.kmax) { set_error("zstart outside 1..kmax-kmax/p_col+1"); return -2; }
     if (!p.grey && !c->have_tables) { set_error("narrow-band run without tables (rc_load_tables)"); return -3; }
     if (c->launched) { (void)hipStreamSynchronize(c->stream); c->launched = false; }
+    c->drop_graph();
     c->zstart = zstart;
     const size_t nT0 = (size_t)(p.imax + 2) * (p.jmax + 2) * (p.kmax + 2);
     c->snaps.resize(nsnap);
@@ -467,69 +478,131 @@ int rc_set_fields(rc_ctx *c, int nsnap, const float *const *T, int zstart) {
     return 0;
 }
 
-int rc_run(rc_ctx *c, int variant) {
-    if (!c || c->snaps.empty()) { set_error("rc_run before rc_set_fields"); return -1; }
-    if (variant != RC_VARIANT_PLAIN && variant != RC_VARIANT_BATCHED) { set_error("unknown variant"); return -2; }
+/* the device work of one solve (no events, no host synchronisation), so
+ * that it can also be recorded into a graph; returns the launch count */
+static int enqueue_solve(rc_ctx *c, int variant, int *launches) {
     const rc_params &p = c->p;
     const DevCommon cm = make_common(c);
     const int nsnap = (int)c->snaps.size();
     const DevSnap *ds = c->dsnap.as<DevSnap>();
     const bool walls = p.calcwalls == 1 && (p.adiabw == 1 || p.adiabe == 1);
-    int launches = 0;
-    RC_HIP(hipMemsetAsync(c->stats.p, 0, 8 * sizeof(unsigned long long), c->stream));
-    RC_HIP(hipEventRecord(c->ev0, c->stream));
-    RC_HIP(hipEventRecord(c->evm0, c->stream));
+    int n = 0;
+    launch_zero_u64(c->stream, c->stats.as<unsigned long long>(), 8);
+    ++n;
     if (variant == RC_VARIANT_PLAIN) {
-        if (cm.srt) {
-            const size_t nk = (size_t)std::max(c->ncell, 2 * c->nsurf) * p.photmax;
-            if (int e = c->keys.ensure(nk * 2)) return e;
-        }
         for (int s = 0; s < nsnap; ++s) {
             double *de = c->de.as<double>() + (size_t)s * c->ncell * p.nVar;
             for (int v = 0; v < p.nVar; ++v) {
-                if (cm.srt) { launch_sort(c->stream, cm, ds, s, v, 0, c->ncell, nullptr, c->keys.as<uint16_t>()); ++launches; }
+                if (cm.srt) { launch_sort(c->stream, cm, ds, s, v, 0, c->ncell, nullptr, c->keys.as<uint16_t>()); ++n; }
                 launch_fluid_plain(c->stream, cm, ds, s, v, cm.srt ? c->keys.as<uint16_t>() : nullptr, de,
                                    c->stats.as<unsigned long long>());
-                ++launches;
+                ++n;
             }
             if (walls) {
                 double *dw = c->ds.as<double>() + (size_t)s * 2 * c->nsurf * p.nVar;
                 for (int v = 0; v < p.nVar; ++v) {
-                    if (cm.srt) { launch_sort(c->stream, cm, ds, s, v, 1, 2 * c->nsurf, nullptr, c->keys.as<uint16_t>()); ++launches; }
+                    if (cm.srt) { launch_sort(c->stream, cm, ds, s, v, 1, 2 * c->nsurf, nullptr, c->keys.as<uint16_t>()); ++n; }
                     launch_surface_plain(c->stream, cm, ds, s, v, cm.srt ? c->keys.as<uint16_t>() : nullptr, dw,
                                          c->stats.as<unsigned long long>());
-                    ++launches;
+                    ++n;
                 }
             }
         }
     } else {
         launch_batched(c->stream, cm, ds, nsnap, 0, c->de.as<double>(), c->stats.as<unsigned long long>(),
                        c->queue.as<int>(), c->wg_per_cu, c->threshold);
-        ++launches;
+        n += 3;
         if (walls) {
             launch_batched(c->stream, cm, ds, nsnap, 1, c->ds.as<double>(), c->stats.as<unsigned long long>(),
                            c->queue.as<int>() + 16, c->wg_per_cu, c->threshold);
-            ++launches;
+            n += 3;
         }
     }
-    RC_HIP(hipEventRecord(c->evm1, c->stream));
     for (int s = 0; s < nsnap; ++s) {
         launch_finalize(c->stream, c->de.as<double>() + (size_t)s * c->ncell * p.nVar, c->ncell, p.nVar,
                         c->mean.as<float>() + (size_t)s * c->ncell, c->sdev.as<float>() + (size_t)s * c->ncell);
-        ++launches;
+        ++n;
         if (walls) {
             launch_finalize(c->stream, c->ds.as<double>() + (size_t)s * 2 * c->nsurf * p.nVar, 2 * c->nsurf, p.nVar,
                             c->smean.as<float>() + (size_t)s * 2 * c->nsurf,
                             c->ssdev.as<float>() + (size_t)s * 2 * c->nsurf);
-            ++launches;
+            ++n;
         }
     }
+    *launches = n;
     RC_HIP(hipGetLastError());
+    return 0;
+}
+
+int rc_run(rc_ctx *c, int variant) {
+    if (!c || c->snaps.empty()) { set_error("rc_run before rc_set_fields"); return -1; }
+    if (variant != RC_VARIANT_PLAIN && variant != RC_VARIANT_BATCHED) { set_error("unknown variant"); return -2; }
+    const rc_params &p = c->p;
+    if (variant == RC_VARIANT_PLAIN && p.srt == 1 && !p.grey) {
+        const size_t nk = (size_t)std::max(c->ncell, 2 * c->nsurf) * p.photmax;
+        if (int e = c->keys.ensure(nk * 2)) return e;
+    }
+    int launches = 0;
+    const bool graph = c->use_graph && variant == RC_VARIANT_BATCHED;
+    if (graph && !c->graph_exec) {
+        RC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
+        const int e = enqueue_solve(c, variant, &launches);
+        hipGraph_t g = nullptr;
+        const hipError_t ce = hipStreamEndCapture(c->stream, &g);
+        if (e) { if (g) (void)hipGraphDestroy(g); return e; }
+        if (ce != hipSuccess) { set_error(std::string("graph capture failed: ") + hipGetErrorString(ce)); return -12; }
+        c->graph = g;
+        RC_HIP(hipGraphInstantiate(&c->graph_exec, c->graph, nullptr, nullptr, 0));
+        c->graph_launches = launches;
+    }
+    RC_HIP(hipEventRecord(c->ev0, c->stream));
+    RC_HIP(hipEventRecord(c->evm0, c->stream));
+    if (graph) {
+        RC_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+        launches = c->graph_launches;
+    } else if (int e = enqueue_solve(c, variant, &launches)) {
+        return e;
+    }
+    RC_HIP(hipEventRecord(c->evm1, c->stream));
     RC_HIP(hipEventRecord(c->ev1, c->stream));
     c->last = rc_stats{};
     c->last.launches = launches;
     c->last.variant = variant;
     c->launched = true;
+    return 0;
+}
+
+int rc_set_graph(rc_ctx *c, int on) {
+    if (!c) { set_error("null context"); return -1; }
+    if (c->launched) (void)rc_sync(c, nullptr);
+    c->use_graph = on != 0;
+    c->drop_graph();
+    return 0;
+}
+
+/* try a small grid of schedules on the loaded snapshots (batched variant,
+ * device-event timing of the march) and keep the fastest */
+int rc_autotune(rc_ctx *c, int32_t *best_wg, int32_t *best_th, float *best_ms) {
+    if (!c || c->snaps.empty()) { set_error("rc_autotune before rc_set_fields"); return -1; }
+    static const int wgs[] = {8, 12, 16, 24, 32};
+    static const int ths[] = {4, 8, 16, 32};
+    const int keep_wg = c->wg_per_cu, keep_th = c->threshold;
+    float best = 1e30f;
+    int bw = keep_wg, bt = keep_th;
+    for (int wg : wgs)
+        for (int th : ths) {
+            c->wg_per_cu = wg; c->threshold = th;
+            c->drop_graph();
+            if (int e = rc_run(c, RC_VARIANT_BATCHED)) { c->wg_per_cu = keep_wg; c->threshold = keep_th; return e; }
+            rc_stats st;
+            if (int e = rc_sync(c, &st)) return e;
+            if ((float)st.march_ms < best) { best = (float)st.march_ms; bw = wg; bt = th; }
+        }
+    c->wg_per_cu = bw; c->threshold = bt;
+    c->drop_graph();
+    if (best_wg) *best_wg = bw;
+    if (best_th) *best_th = bt;
+    if (best_ms) *best_ms = best;
     return 0;
 }
 
@@ -586,6 +659,7 @@ int rc_set_schedule(rc_ctx *c, int wg_per_cu, int threshold) {
     }
     c->wg_per_cu = wg_per_cu;
     c->threshold = threshold;
+    c->drop_graph();
     return 0;
 }
 

@@ -68,6 +68,7 @@ void launch_trace(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int
                   const int32_t *unit, int n, const uint16_t *keys, rc_ray *out);
 void launch_finalize(hipStream_t st, const double *de, int n, int nVar, float *mean, float *sdev);
 void launch_zero(hipStream_t st, double *p, size_t n);
+void launch_zero_u64(hipStream_t st, unsigned long long *p, int n);
 
 }  // namespace rc
 

@@ -102,6 +102,14 @@ class Solver:
     def set_schedule(self, wg_per_cu: int = 0, threshold: int = 0) -> None:
         L.check(self.lib.rc_set_schedule(self.ctx, wg_per_cu, threshold), "rc_set_schedule")
 
+    def set_graph(self, on: bool = True) -> None:
+        L.check(self.lib.rc_set_graph(self.ctx, int(on)), "rc_set_graph")
+
+    def autotune(self) -> dict:
+        wg, th, ms = C.c_int32(), C.c_int32(), C.c_float()
+        L.check(self.lib.rc_autotune(self.ctx, C.byref(wg), C.byref(th), C.byref(ms)), "rc_autotune")
+        return {"wg_per_cu": wg.value, "threshold": th.value, "march_ms": ms.value}
+
     def info(self, s: int = 0) -> dict:
         f = np.zeros(5, np.float32)
         i = np.zeros(2, np.int32)

@@ -122,3 +122,81 @@ def test_field_multigrid_and_walls(nb_tables, variant):
 @pytest.mark.parametrize("variant", [VARIANT_PLAIN, VARIANT_BATCHED])
 def test_field_boundary_conditions(nb_tables, variant):
     _field_parity(*cases.bc_case(nb_tables), variant)
+
+
+def test_snapshot_batch_one_launch(nb_tables):
+    """Three temperature snapshots solved in one batched launch: each matches
+    the oracle run on that snapshot's own Philox streams."""
+    p, tab, _ = cases.nb_case(nb_tables, n=12, photmax=48, nVar=2)
+    from radchan_amd import fields as F
+    Ts = [F.channel_field(p, 300.0, 1500.0 - 200.0 * s, noise=0.03, seed=40 + s) for s in range(3)]
+    print("snapshot batch: 3 x 12^3", flush=True)
+    s = Solver(p, tab)
+    s.set_fields(Ts)
+    s.solve(VARIANT_BATCHED)
+    for q, T in enumerate(Ts):
+        o = O.Oracle(p, tab, T, snapshot=q)
+        de32, de32d, de64, steps = o.solve(O.all_cells(p))
+        got = s.tallies(q)
+        assert np.abs(got - de32d).max() <= 1e-12 * np.abs(de32d).max(), q
+        info, oi = s.info(q), o.info()
+        for k in ("Tmax", "kappamax", "Ibmax", "Ibmax_w", "Ibmax_e"):
+            assert np.float32(info[k]) == np.float32(oi[k]), k
+        assert (info["tm"], info["tstar"]) == (oi["tm"], oi["tstar"])
+    s.solve(VARIANT_PLAIN)
+    for q, T in enumerate(Ts):
+        o = O.Oracle(p, tab, T, snapshot=q)
+        de32, de32d, de64, steps = o.solve(O.all_cells(p))
+        assert np.array_equal(s.tallies(q), de32.astype(np.float64)), q
+    s.close()
+
+
+def test_results_are_mean_and_reference_sdev(nb_tables):
+    """mean over runs, and sqrt of sum (x-m)^2 / ((n-1/2)(n-1)) as the
+    reference reports it (mc_gpu_noint.cu:234-239, :984)."""
+    p, tab, T = cases.nb_case(nb_tables, n=8, photmax=32, nVar=5)
+    s = Solver(p, tab)
+    s.set_fields([T])
+    s.solve(VARIANT_BATCHED)
+    de = s.tallies()
+    mean, sd = s.results()
+    m = de.mean(axis=1)
+    var = ((de - m[:, None]) ** 2).sum(axis=1) / ((p.nVar - 0.5) * (p.nVar - 1))
+    assert np.allclose(mean.reshape(-1), m.astype(np.float32), rtol=1e-6, atol=0)
+    assert np.allclose(sd.reshape(-1), np.sqrt(var).astype(np.float32), rtol=1e-6, atol=0)
+    s.close()
+
+
+def test_graph_replay_matches_eager(nb_tables):
+    """The batched solve recorded as a hipGraph: several replays, with eager
+    solves and host synchronisation between them, each equal to the eager
+    result (FP64 atomic sums: equal to 1e-12 relative)."""
+    p, tab, T = cases.mg_case(nb_tables)
+    s = Solver(p, tab)
+    s.set_fields([T])
+    s.solve(VARIANT_BATCHED)
+    ref = s.tallies()
+    wref = s.surface()[2]
+    s.set_graph(True)
+    for rep in range(4):
+        st = s.solve(VARIANT_BATCHED)
+        got = s.tallies()
+        assert np.abs(got - ref).max() <= 1e-12 * np.abs(ref).max(), rep
+        assert np.abs(s.surface()[2] - wref).max() <= 1e-12 * np.abs(wref).max(), rep
+        s.set_graph(False) if rep == 1 else None
+        s.solve(VARIANT_PLAIN)                     # eager work between replays
+        s.set_graph(True) if rep == 1 else None
+    s.close()
+
+
+def test_autotune_keeps_results(nb_tables):
+    p, tab, T = cases.nb_case(nb_tables, n=12, photmax=40, nVar=2)
+    s = Solver(p, tab)
+    s.set_fields([T])
+    s.solve(VARIANT_BATCHED)
+    ref = s.tallies()
+    best = s.autotune()
+    assert best["wg_per_cu"] in (8, 12, 16, 24, 32) and best["threshold"] in (4, 8, 16, 32)
+    s.solve(VARIANT_BATCHED)
+    assert np.abs(s.tallies() - ref).max() <= 1e-12 * np.abs(ref).max()
+    s.close()
