@@ -183,6 +183,10 @@ int rc_info(rc_ctx *c, int s, float *f5, int32_t *i2);
  * refill; 0 selects the default.  Chosen by the autotuner. */
 int rc_set_schedule(rc_ctx *c, int wg_per_cu, int threshold);
 int rc_get_schedule(rc_ctx *c, int32_t *wg_per_cu, int32_t *threshold);
+/* storage of the k tables on the device: 0 = FP32 (default), 1 = BF16 k
+ * pairs with a power-of-two scale per band (Ib stays FP32); rebuilds the
+ * loaded tables */
+int rc_set_table_precision(rc_ctx *c, int bf16);
 /* record the batched solve (queue reset, tally reset, persistent kernel,
  * finalize) once as a hipGraph and replay it on later rc_run calls; the
  * graph is dropped when fields or the schedule change */

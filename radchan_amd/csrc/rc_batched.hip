@@ -146,7 +146,7 @@ __device__ __forceinline__ void q_get(const uint32_t *q, int slot, const DevComm
     e.Bi = __uint_as_float(q[Q_BI * QCAP + slot]);
     e.invBi = __uint_as_float(q[Q_IBI * QCAP + slot]);
     const uint32_t key = q[Q_KEY * QCAP + slot];
-    e.kb = sn.kb + (size_t)key * (cm.nT + 1);
+    e.row = key;
     e.rank = (int)(key / (uint32_t)cm.nQ);
     tidx = (int)q[Q_TIDX * QCAP + slot];
 }

@@ -77,9 +77,18 @@ template <class T> int upload(DevBuf &b, const std::vector<T> &v, hipStream_t st
 /* device tables in sorted band order for one temperature table */
 struct TableSet {
     float Tnb0 = 0.f, invdT = 0.f;
+    int bf16 = 0;
     std::vector<float> Tnb;
-    DevBuf kb, bfix, cdf_gs, rank_of;
+    DevBuf kb, bfix, cdf_gs, rank_of, kp16, bp, kscale;
 };
+
+/* round-to-nearest-even FP32 -> BF16 (finite inputs) */
+inline uint16_t to_bf16(float x) {
+    uint32_t u;
+    std::memcpy(&u, &x, 4);
+    u += 0x7FFFu + ((u >> 16) & 1u);
+    return (uint16_t)(u >> 16);
+}
 
 struct SnapState {
     float Tmax = 0.f, kappamax = 0.f, Ibmax = 0.f, Ibmax_w = 0.f, Ibmax_e = 0.f;
@@ -112,6 +121,7 @@ struct rc_ctx {
     bool launched = false;
     int wg_per_cu = 0, threshold = 0;       /* batched schedule; 0 = default */
     bool use_graph = false;                 /* replay the batched run as a hipGraph */
+    int table_bf16 = 0;                     /* k tables stored as BF16 */
     int graph_launches = 0;
     hipGraph_t graph = nullptr;
     hipGraphExec_t graph_exec = nullptr;
@@ -148,6 +158,33 @@ int build_tableset(rc_ctx *c, const Tables &t, TableSet &ts, std::vector<int32_t
             for (int q = 0; q < nT; ++q) row[q] = make_float2(t.kq[((size_t)b * nT + q) * nQ + g], brow[q]);
             row[nT] = row[nT - 1];
         }
+    }
+    ts.bf16 = c->table_bf16;
+    if (ts.bf16) {
+        /* k as BF16 pairs with a power-of-two scale per band: 2^floor(log2 max k) */
+        std::vector<uint32_t> kp((size_t)nB * nQ * nT);
+        std::vector<float2> bp((size_t)nB * nT);
+        std::vector<float> ksc(nB);
+        for (int r = 0; r < nB; ++r) {
+            float mx = 0.f;
+            for (int g = 0; g < nQ; ++g)
+                for (int q = 0; q < nT; ++q) mx = std::max(mx, kb[((size_t)r * nQ + g) * (nT + 1) + q].x);
+            int ex = 0;
+            (void)std::frexp(mx > 0.f ? mx : 1.f, &ex);
+            const float sc = std::ldexp(1.0f, ex - 1), inv = std::ldexp(1.0f, 1 - ex);
+            ksc[r] = sc;
+            for (int g = 0; g < nQ; ++g) {
+                const float2 *row = &kb[((size_t)r * nQ + g) * (nT + 1)];
+                for (int q = 0; q < nT; ++q)
+                    kp[((size_t)r * nQ + g) * nT + q] =
+                        (uint32_t)to_bf16(row[q].x * inv) | ((uint32_t)to_bf16(row[q + 1].x * inv) << 16);
+            }
+            const float2 *row0 = &kb[(size_t)r * nQ * (nT + 1)];
+            for (int q = 0; q < nT; ++q) bp[(size_t)r * nT + q] = make_float2(row0[q].y, row0[q + 1].y);
+        }
+        if (int e = upload(ts.kp16, kp, c->stream)) return e;
+        if (int e = upload(ts.bp, bp, c->stream)) return e;
+        if (int e = upload(ts.kscale, ksc, c->stream)) return e;
     }
     std::vector<float> bfix((size_t)6 * nB);
     for (int f = 0; f < 6; ++f)
@@ -252,6 +289,8 @@ int push_snaps(rc_ctx *c) {
         d.Tmax = S.Tmax; d.Ibmax = S.Ibmax; d.Ibmax_w = S.Ibmax_w; d.Ibmax_e = S.Ibmax_e;
         d.Tnb0 = ts.Tnb0; d.invdT = ts.invdT; d.tstar = S.tstar; d.snap = (int)s;
         d.kb = ts.kb.as<float2>(); d.bfix = ts.bfix.as<float>();
+        d.kbf16 = ts.bf16;
+        d.kp16 = ts.kp16.as<uint32_t>(); d.bp = ts.bp.as<float2>(); d.kscale = ts.kscale.as<float>();
         d.cdf_b = S.cdf_b.as<float>(); d.cdf_g = S.cdf_g.as<float>(); d.cdf_bs = S.cdf_bs.as<float>();
         d.cdf_gs = ts.cdf_gs.as<float>(); d.rank_of = ts.rank_of.as<int>();
         d.guide_b = S.guide_b.as<int>(); d.guide_bs = S.guide_bs.as<int>(); d.gM = kGuideM;
@@ -569,6 +608,22 @@ int rc_run(rc_ctx *c, int variant) {
     c->last.launches = launches;
     c->last.variant = variant;
     c->launched = true;
+    return 0;
+}
+
+int rc_set_table_precision(rc_ctx *c, int bf16) {
+    if (!c || (bf16 != 0 && bf16 != 1)) { set_error("table precision: 0 (FP32) or 1 (BF16)"); return -1; }
+    if (c->launched) (void)rc_sync(c, nullptr);
+    c->table_bf16 = bf16;
+    c->drop_graph();
+    if (c->have_tables)
+        if (int e = build_tableset(c, c->tab, c->nbset, c->idx, c->rank_of)) return e;
+    for (auto &sp : c->snaps)
+        if (sp && sp->grey) {
+            std::vector<int32_t> idx, ro;
+            if (int e = build_tableset(c, sp->greyhost, *sp->grey, idx, ro)) return e;
+        }
+    if (!c->snaps.empty()) return push_snaps(c);
     return 0;
 }
 

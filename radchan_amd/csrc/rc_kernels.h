@@ -43,6 +43,10 @@ struct DevSnap {
     float Tnb0, invdT;
     int tstar, snap;
     const float2 *kb;              /* [(rank*nQ+g)*(nT+1)+t] = {k, Ib}    */
+    const uint32_t *kp16;          /* BF16 mode: [(rank*nQ+g)*nT+t] = k_t | k_t+1 << 16 */
+    const float2 *bp;              /* BF16 mode: [rank*nT+t] = {Ib_t, Ib_t+1}         */
+    const float *kscale;           /* BF16 mode: [rank] power-of-two band scale       */
+    int kbf16, pad3;               /* 1: k tables stored as BF16                       */
     const float *bfix;             /* [6][nB] fixed-temperature wall Ib   */
     const float *cdf_b;            /* [nB] fluid band CDF, row t*         */
     const float *cdf_g;            /* [nB][nQ] g CDFs, row t*             */

@@ -40,17 +40,33 @@ struct Ray {
 };
 
 struct En {
-    const float2 *__restrict__ kb;   /* (rank, g) row: nT+1 pairs {k, Ib} */
-    float W, Bi, invBi;
+    uint32_t row;   /* table row: sorted band rank * nQ + g */
     int rank;
+    float W, Bi, invBi;
 };
 
-__device__ __forceinline__ void kb_at(const DevSnap &sn, const float2 *__restrict__ kb, int nT, float T, float &K, float &B) {
+/* k and Ib of table row `row` (band rank * nQ + g) at temperature T:
+ * clamp-to-edge linear interpolation in T (the reference's linear-filtered
+ * textures, memory_copy_noint.cu:241-247, mc_gpu_noint.cu:784-791).  FP32
+ * tables hold {k, Ib} pairs; BF16 tables hold k_t and k_t+1 as a BF16 pair
+ * times a power-of-two band scale, and Ib pairs per band in FP32. */
+__device__ __forceinline__ void kb_at(const DevSnap &sn, uint32_t row, int rank, int nT, float T, float &K, float &B) {
     int t; float a;
     tpos(T, sn.Tnb0, sn.invdT, nT, t, a);
-    const float2 q0 = gld(kb, t), q1 = gld(kb, t + 1);
-    K = lerp(q0.x, q1.x, a);
-    B = lerp(q0.y, q1.y, a);
+    if (sn.kbf16) {
+        const uint32_t w = gld(sn.kp16, (size_t)row * nT + t);
+        const float2 b = gld(sn.bp, (size_t)rank * nT + t);
+        const float sc = gld(sn.kscale, rank);
+        const float k0 = __uint_as_float(w << 16) * sc;
+        const float k1 = __uint_as_float(w & 0xFFFF0000u) * sc;
+        K = lerp(k0, k1, a);
+        B = lerp(b.x, b.y, a);
+    } else {
+        const size_t base = (size_t)row * (nT + 1) + t;
+        const float2 q0 = gld(sn.kb, base), q1 = gld(sn.kb, base + 1);
+        K = lerp(q0.x, q1.x, a);
+        B = lerp(q0.y, q1.y, a);
+    }
 }
 
 __device__ __forceinline__ void set_dir(Ray &r, float sx, float sy, float sz) {
@@ -94,7 +110,7 @@ __device__ __forceinline__ void boundary(const DevCommon &cm, const DevSnap &sn,
         if ((face == FW && cm.adiab[0]) || (face == FE && cm.adiab[1])) {
             const float Tw = gld(L.Tw, (size_t)(face == FE) * L.jm * L.km + (size_t)(r.jc - 1) + (size_t)L.jm * (r.kc - 1));
             float Kw;
-            kb_at(sn, e.kb, nT, Tw, Kw, Bw);
+            kb_at(sn, e.row, e.rank, nT, Tw, Kw, Bw);
         } else {
             Bw = gld(sn.bfix, (size_t)face * cm.nB + e.rank);
         }
@@ -112,7 +128,7 @@ __device__ __forceinline__ void boundary(const DevCommon &cm, const DevSnap &sn,
     } else if (bd == 4) {
         const float Tw = gld(L.Tz, (size_t)(face == FT) * L.im * L.jm + (size_t)(r.ic - 1) + (size_t)L.im * (r.jc - 1));
         float Kw, Bw;
-        kb_at(sn, e.kb, nT, Tw, Kw, Bw);
+        kb_at(sn, e.row, e.rank, nT, Tw, Kw, Bw);
         wall_hit(sn, e, nT, r, Bw, de);
         status = RC_RAY_WALL;
     } else {
@@ -144,7 +160,7 @@ __device__ __forceinline__ void march_step(const DevCommon &cm, const DevSnap &s
     r.yp = (ds == dsy) ? fy : fmaf(ds, r.sy, r.yp);
     r.zp = (ds == dsz) ? fz : fmaf(ds, r.sz, r.zp);
     float K, B;
-    kb_at(sn, e.kb, nT, Tc, K, B);
+    kb_at(sn, e.row, e.rank, nT, Tc, K, B);
     float em1, ex;
     expm1_exp(-(K * ds), em1, ex);
     const float alpha = -em1;
@@ -213,10 +229,10 @@ __device__ __forceinline__ void energy(const DevCommon &cm, const DevSnap &sn, u
                                        int bound, En &e) {
     const int rank = (int)(key / (uint32_t)cm.nQ);
     e.rank = rank;
-    e.kb = sn.kb + (size_t)key * (cm.nT + 1);
+    e.row = key;
     float Ki, Bi, Km, Bm;
-    kb_at(sn, e.kb, cm.nT, Ti, Ki, Bi);
-    kb_at(sn, e.kb, cm.nT, sn.Tmax, Km, Bm);
+    kb_at(sn, e.row, rank, cm.nT, Ti, Ki, Bi);
+    kb_at(sn, e.row, rank, cm.nT, sn.Tmax, Km, Bm);
     e.Bi = Bi;
     e.invBi = 1.0f / Bi;
     if (!surface) e.W = sn.Ibmax * (((Bi / Bm) * Ki) / Km);

@@ -102,6 +102,9 @@ class Solver:
     def set_schedule(self, wg_per_cu: int = 0, threshold: int = 0) -> None:
         L.check(self.lib.rc_set_schedule(self.ctx, wg_per_cu, threshold), "rc_set_schedule")
 
+    def set_table_precision(self, bf16: bool) -> None:
+        L.check(self.lib.rc_set_table_precision(self.ctx, int(bf16)), "rc_set_table_precision")
+
     def set_graph(self, on: bool = True) -> None:
         L.check(self.lib.rc_set_graph(self.ctx, int(on)), "rc_set_graph")
 

@@ -134,3 +134,22 @@ def synthetic_nb(nB: int = 360, nT: int = 20, nQ: int = 16, T_lo: float = 300.0,
 def grey_sizes(true_grey: bool):
     """(nB, nT, nQ) of the grey tables (NarrowBand.h:35-37)."""
     return (1, 65, 1) if true_grey else (1000, 65, 1)
+
+
+def bf16_dequantized(t: Tables) -> Tables:
+    """The tables as the device holds them in BF16 mode: every k rounded to
+    BF16 (nearest even) after division by the band's power-of-two scale
+    2^floor(log2 max k), then scaled back (both scalings exact).  Feeding
+    these tables to an FP32 solver reproduces the BF16 solver exactly."""
+    q = t.copy()
+    k = t.kq.astype(np.float32)                                    # [nB][nT][nQ]
+    mx = k.reshape(t.nB, -1).max(axis=1)
+    mx = np.where(mx > 0, mx, np.float32(1.0))
+    _, ex = np.frexp(mx)
+    sc = np.ldexp(np.float32(1.0), ex - 1).astype(np.float32)
+    inv = np.ldexp(np.float32(1.0), 1 - ex).astype(np.float32)
+    x = (k * inv[:, None, None]).astype(np.float32)
+    u = x.view(np.uint32).astype(np.uint64)
+    u = (u + 0x7FFF + ((u >> 16) & 1)) >> 16 << 16
+    q.kq[:] = (u.astype(np.uint32).view(np.float32) * sc[:, None, None]).astype(np.float32)
+    return q
