@@ -9,10 +9,15 @@ Workload (default): 64x64x64 cells, x stretched (fact_mesh 1.5), synthetic
 H2O-like tables of 360 bands x 16 g-points x 20 temperatures, photmax 200,
 sorted bands (srt=1), nVar 8, FP32 march with FP64 tallies.
 
-Printed (one JSON line on stdout): rays/s of the batched solver, the plain
-solver's rays/s beside it, the measured std-error eps of div(q_r) and the
-projected time to eps = 1% for both, the roofline of the dominant kernel and
-a CPU baseline (the C++ oracle in FP32 on the host cores, a bounded sample).
+Legs: an untimed autotune of the batched schedule; the batched solver with
+FP32 and with BF16 k-tables, each replayed as a hipGraph; the batched march
+kernel alone (HIP events, no graph) for the roofline; the plain solver.
+
+Printed (one JSON line on stdout): rays/s of the batched solver (BF16
+k-tables unless --tables fp32), the plain solver's rays/s beside it, the
+measured std-error eps of div(q_r) and the projected time to eps = 1% for
+every leg, the roofline of the dominant kernel and a CPU baseline (the C++
+oracle in FP32 on the host cores, a bounded sample).
 """
 from __future__ import annotations
 
@@ -30,6 +35,8 @@ sys.path.insert(0, ROOT)
 HBM_PEAK_GBS = 8000.0           # MI355X HBM3E
 B_STEP = 32                     # 3 face coords + T + 2 k + 2 Ib texels (FP32)
 B_RAY = 32                      # 2 x (2 k + 2 Ib texels) at emission and Tmax
+B_STEP_BF16 = 28                # the same with the two k texels in BF16
+B_RAY_BF16 = 28
 B_CELL = 20                     # T in, FP64 tally pair out
 
 
@@ -78,6 +85,22 @@ def cpu_baseline(p, tab, T, seconds: float):
             "steps_per_s": float(steps.sum() / dt)}
 
 
+def timed_leg(s, lib, variant: int, steps: int, warmup: int):
+    """warmup untimed solves, then `steps` timed ones (device-synchronised
+    on both sides); returns (wall seconds per step, last stats, march ms
+    of every step)."""
+    for _ in range(warmup):
+        s.solve(variant)
+    lib.rc_device_synchronize()
+    march = []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        st = s.solve(variant)
+        march.append(st["march_ms"])
+    lib.rc_device_synchronize()
+    return (time.perf_counter() - t0) / steps, st, march
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
@@ -85,10 +108,17 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="nb64", choices=["nb64", "nb128", "big256", "grey32", "mg192"])
     ap.add_argument("--nvar", type=int, default=8)
+    ap.add_argument("--tables", default="bf16", choices=["bf16", "fp32"],
+                    help="k-table storage of the headline (batched) leg")
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-autotune", action="store_true")
+    ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--probe", action="store_true",
+                    help="no timing: one elementwise copy of 512 MiB (counter calibration) and one batched "
+                         "solve, for hardware-counter runs")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU path (cell slabs per GPU) is not "
@@ -112,52 +142,71 @@ def main() -> int:
     s = Solver(p, tab)
     s.set_fields([T])
     lib = L.load()
+    bf16 = args.tables == "bf16" and not p.grey
 
-    for _ in range(args.warmup):
-        s.solve(VARIANT_BATCHED)
-    lib.rc_device_synchronize()
-    march = []
-    t0 = time.perf_counter()
-    for _ in range(args.steps):
+    if args.probe:
+        import torch
+        n = (512 << 20) // 4
+        x = torch.full((n,), 1.0, dtype=torch.float32, device="cuda")
+        y = torch.empty_like(x)
+        torch.mul(x, 1.0, out=y)                   # reads 512 MiB, writes 512 MiB
+        torch.cuda.synchronize()
+        s.set_table_precision(bf16)
         st = s.solve(VARIANT_BATCHED)
-        march.append(st["march_ms"])
-    lib.rc_device_synchronize()
-    wall = time.perf_counter() - t0
-    ms_per_step = 1e3 * wall / args.steps
-    rays = st["rays"]
-    steps = st["steps"]
-    value = rays / (wall / args.steps)
-    de = s.tallies()
-    eps, p95 = std_error(de)
-    log(f"batched: {ms_per_step:.2f} ms/step, {value / 1e9:.3f} Grays/s, {steps / rays:.2f} steps/ray, eps {eps:.4g}")
+        print(json.dumps({"probe": args.workload, "tables": args.tables, "copy_bytes": 4 * n,
+                          "rays": st["rays"], "steps": st["steps"], "march_ms": st["march_ms"]}), flush=True)
+        s.close()
+        return 0
+
+    sched = None
+    if not args.no_autotune:
+        sched = s.autotune()                       # untimed: picks wavefronts per CU and refill threshold
+        log(f"autotune: {sched}")
+
+    legs = {}
+    for name, use_bf16 in (("fp32_tables", False), ("bf16_tables", True)):
+        if use_bf16 and p.grey:
+            continue
+        s.set_table_precision(use_bf16)
+        s.set_graph(not args.no_graph)
+        wall, st, _ = timed_leg(s, lib, VARIANT_BATCHED, args.steps, args.warmup)
+        eps, p95 = std_error(s.tallies())
+        legs[name] = {"rays_per_s": st["rays"] / wall, "ms_per_step": 1e3 * wall, "eps": eps, "eps_p95_cell": p95,
+                      "time_to_1pct_s": wall * (eps / 0.01) ** 2, "rays": st["rays"], "steps": st["steps"]}
+        log(f"batched {name}: {1e3 * wall:.2f} ms/step, {st['rays'] / wall / 1e9:.3f} Grays/s, "
+            f"{st['steps'] / st['rays']:.2f} steps/ray, eps {eps:.4g}")
+    head = legs["bf16_tables" if bf16 else "fp32_tables"]
+
+    # the march kernel alone, outside a graph: HIP events on the solver's stream
+    s.set_table_precision(bf16)
+    s.set_graph(False)
+    _, st, march = timed_leg(s, lib, VARIANT_BATCHED, max(3, args.steps // 2), 1)
+    rays, steps = st["rays"], st["steps"]
 
     plain = None
     if not args.no_plain and args.plain_steps > 0:
-        s.solve(VARIANT_PLAIN)
-        lib.rc_device_synchronize()
-        t0 = time.perf_counter()
-        for _ in range(args.plain_steps):
-            sp = s.solve(VARIANT_PLAIN)
-        lib.rc_device_synchronize()
-        pwall = (time.perf_counter() - t0) / args.plain_steps
+        s.set_table_precision(False)
+        pwall, sp, _ = timed_leg(s, lib, VARIANT_PLAIN, args.plain_steps, 1)
         eps_p, _ = std_error(s.tallies())
         plain = {"rays_per_s": sp["rays"] / pwall, "ms_per_step": 1e3 * pwall, "eps": eps_p,
                  "time_to_1pct_s": pwall * (eps_p / 0.01) ** 2}
         log(f"plain:   {1e3 * pwall:.2f} ms/step, {sp['rays'] / pwall / 1e9:.3f} Grays/s, eps {eps_p:.4g}")
 
     ncell = s.ncell
-    alg_bytes = steps * B_STEP + rays * B_RAY + ncell * B_CELL
+    b_step, b_ray = (B_STEP_BF16, B_RAY_BF16) if bf16 else (B_STEP, B_RAY)
+    alg_bytes = steps * b_step + rays * b_ray + ncell * B_CELL
     march_ms = float(np.mean(march))
     achieved = alg_bytes / (march_ms * 1e-3) / 1e9
     traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}.json")
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.tables}.json")
     if os.path.exists(tpath):
         with open(tpath) as f:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_batched<false> (march)", "kernel_ms": round(march_ms, 4),
-            "algorithmic_bytes": int(alg_bytes)}
+            "kernel": "k_batched<false,false> (persistent march)", "kernel_ms": round(march_ms, 4),
+            "algorithmic_bytes": int(alg_bytes),
+            "bytes_per_step": b_step, "bytes_per_ray": b_ray, "bytes_per_cell": B_CELL}
 
     cpu = None
     if not args.no_cpu:
@@ -167,6 +216,7 @@ def main() -> int:
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline unavailable: {e}")
 
+    value = head["rays_per_s"]
     out = {
         "metric": "rays/s (batched solver) and time to 1% std-error of div q_r; batched vs plain, 1 MI355X",
         "value": value,
@@ -174,22 +224,25 @@ def main() -> int:
         "n_gpus": 1,
         "steps": args.steps,
         "warmup": args.warmup,
-        "ms_per_step": ms_per_step,
+        "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
         "scaling": "weak",
         "vs_baseline": None,
-        "dtype": "fp32 march, fp64 tallies",
+        "dtype": "fp32 march, bf16 k-tables, fp64 tallies" if bf16 else "fp32 march, fp32 k-tables, fp64 tallies",
         "data": "synthetic (H2O-like tables, parabolic channel T field with 2% smoothed noise)",
         "config": {"workload": f"narrow-band correlated-k OERM {p.imax}x{p.jmax}x{p.kmax}, {p.nB} bands x "
                                f"{p.nQ} g x {p.nT} T, photmax {p.photmax}, nVar {p.nVar}, srt {p.srt}, "
                                f"grid_num {p.grid_num}" if not p.grey else
                                f"grey OERM {p.imax}x{p.jmax}x{p.kmax}, photmax {p.photmax}, nVar {p.nVar}",
-                   "cells": ncell, "rays_per_step": int(rays), "steps_per_ray": steps / rays},
-        "eps": eps,
-        "eps_p95_cell": p95,
-        "time_to_1pct_s": (wall / args.steps) * (eps / 0.01) ** 2,
+                   "cells": ncell, "rays_per_step": int(rays), "steps_per_ray": steps / rays,
+                   "schedule": sched, "graph": not args.no_graph},
+        "eps": head["eps"],
+        "eps_p95_cell": head["eps_p95_cell"],
+        "time_to_1pct_s": head["time_to_1pct_s"],
+        "batched": legs,
         "plain": plain,
         "speedup_vs_plain": (value / plain["rays_per_s"]) if plain else None,
+        "time_to_1pct_speedup_vs_plain": (plain["time_to_1pct_s"] / head["time_to_1pct_s"]) if plain else None,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

@@ -96,7 +96,8 @@ typedef struct rc_ray {
 typedef struct rc_stats {
     double   kernel_ms;         /* device time of the last run (HIP events)   */
     double   march_ms;          /* device time of the march kernels alone     */
-                                /* (band sort and march, not finalize)        */
+                                /* (band sort and march, not finalize); the   */
+                                /* whole graph when replayed as a hipGraph    */
     int64_t  rays;              /* rays marched in the last run               */
     int64_t  steps;             /* march steps executed in the last run       */
     int64_t  capped;            /* rays that hit the last level's step cap    */

@@ -23,7 +23,7 @@
 #include "rc_kernels.h"
 
 namespace rc {
-void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int nsnap, int surface,
+void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int nsnap, int surface, int snaps_bf16,
                     double *de, unsigned long long *stats, int *queue, int wg_per_cu, int threshold);
 }
 
@@ -517,9 +517,10 @@ int rc_set_fields(rc_ctx *c, int nsnap, const float *const *T, int zstart) {
     return 0;
 }
 
-/* the device work of one solve (no events, no host synchronisation), so
- * that it can also be recorded into a graph; returns the launch count */
-static int enqueue_solve(rc_ctx *c, int variant, int *launches) {
+/* the device work of one solve (no host synchronisation), so that it can
+ * also be recorded into a graph; returns the launch count.  With `timed`,
+ * the march events bracket the march kernels alone (never inside a graph). */
+static int enqueue_solve(rc_ctx *c, int variant, int *launches, bool timed) {
     const rc_params &p = c->p;
     const DevCommon cm = make_common(c);
     const int nsnap = (int)c->snaps.size();
@@ -528,6 +529,7 @@ static int enqueue_solve(rc_ctx *c, int variant, int *launches) {
     int n = 0;
     launch_zero_u64(c->stream, c->stats.as<unsigned long long>(), 8);
     ++n;
+    if (timed) RC_HIP(hipEventRecord(c->evm0, c->stream));
     if (variant == RC_VARIANT_PLAIN) {
         for (int s = 0; s < nsnap; ++s) {
             double *de = c->de.as<double>() + (size_t)s * c->ncell * p.nVar;
@@ -548,15 +550,16 @@ static int enqueue_solve(rc_ctx *c, int variant, int *launches) {
             }
         }
     } else {
-        launch_batched(c->stream, cm, ds, nsnap, 0, c->de.as<double>(), c->stats.as<unsigned long long>(),
+        launch_batched(c->stream, cm, ds, nsnap, 0, c->table_bf16, c->de.as<double>(), c->stats.as<unsigned long long>(),
                        c->queue.as<int>(), c->wg_per_cu, c->threshold);
-        n += 3;
+        n += 2;
         if (walls) {
-            launch_batched(c->stream, cm, ds, nsnap, 1, c->ds.as<double>(), c->stats.as<unsigned long long>(),
+            launch_batched(c->stream, cm, ds, nsnap, 1, c->table_bf16, c->ds.as<double>(), c->stats.as<unsigned long long>(),
                            c->queue.as<int>() + 16, c->wg_per_cu, c->threshold);
-            n += 3;
+            n += 2;
         }
     }
+    if (timed) RC_HIP(hipEventRecord(c->evm1, c->stream));
     for (int s = 0; s < nsnap; ++s) {
         launch_finalize(c->stream, c->de.as<double>() + (size_t)s * c->ncell * p.nVar, c->ncell, p.nVar,
                         c->mean.as<float>() + (size_t)s * c->ncell, c->sdev.as<float>() + (size_t)s * c->ncell);
@@ -585,7 +588,7 @@ int rc_run(rc_ctx *c, int variant) {
     const bool graph = c->use_graph && variant == RC_VARIANT_BATCHED;
     if (graph && !c->graph_exec) {
         RC_HIP(hipStreamBeginCapture(c->stream, hipStreamCaptureModeThreadLocal));
-        const int e = enqueue_solve(c, variant, &launches);
+        const int e = enqueue_solve(c, variant, &launches, false);
         hipGraph_t g = nullptr;
         const hipError_t ce = hipStreamEndCapture(c->stream, &g);
         if (e) { if (g) (void)hipGraphDestroy(g); return e; }
@@ -595,14 +598,15 @@ int rc_run(rc_ctx *c, int variant) {
         c->graph_launches = launches;
     }
     RC_HIP(hipEventRecord(c->ev0, c->stream));
-    RC_HIP(hipEventRecord(c->evm0, c->stream));
     if (graph) {
+        /* a replayed graph is timed as a whole */
+        RC_HIP(hipEventRecord(c->evm0, c->stream));
         RC_HIP(hipGraphLaunch(c->graph_exec, c->stream));
+        RC_HIP(hipEventRecord(c->evm1, c->stream));
         launches = c->graph_launches;
-    } else if (int e = enqueue_solve(c, variant, &launches)) {
+    } else if (int e = enqueue_solve(c, variant, &launches, true)) {
         return e;
     }
-    RC_HIP(hipEventRecord(c->evm1, c->stream));
     RC_HIP(hipEventRecord(c->ev1, c->stream));
     c->last = rc_stats{};
     c->last.launches = launches;

@@ -45,15 +45,32 @@ struct En {
     float W, Bi, invBi;
 };
 
+/* storage of the k tables: read from the snapshot at run time, or fixed at
+ * compile time (FP32 {k, Ib} pairs / BF16 k pairs) */
+enum { TAB_RT = -1, TAB_F32 = 0, TAB_BF16 = 1 };
+
 /* k and Ib of table row `row` (band rank * nQ + g) at temperature T:
  * clamp-to-edge linear interpolation in T (the reference's linear-filtered
  * textures, memory_copy_noint.cu:241-247, mc_gpu_noint.cu:784-791).  FP32
  * tables hold {k, Ib} pairs; BF16 tables hold k_t and k_t+1 as a BF16 pair
  * times a power-of-two band scale, and Ib pairs per band in FP32. */
-__device__ __forceinline__ void kb_at(const DevSnap &sn, uint32_t row, int rank, int nT, float T, float &K, float &B) {
+/* the snapshot fields the march reads, held as wavefront-uniform values by
+ * the batched kernel (the march functions accept it or a DevSnap) */
+struct SnapView {
+    float Tnb0, invdT;
+    int kbf16;
+    const float2 *kb;
+    const uint32_t *kp16;
+    const float2 *bp;
+    const float *kscale;
+    const float *bfix;
+};
+
+template <int TM = TAB_RT, class S>
+__device__ __forceinline__ void kb_at(const S &sn, uint32_t row, int rank, int nT, float T, float &K, float &B) {
     int t; float a;
     tpos(T, sn.Tnb0, sn.invdT, nT, t, a);
-    if (sn.kbf16) {
+    if (TM == TAB_RT ? sn.kbf16 != 0 : TM == TAB_BF16) {
         const uint32_t w = gld(sn.kp16, (size_t)row * nT + t);
         const float2 b = gld(sn.bp, (size_t)rank * nT + t);
         const float sc = gld(sn.kscale, rank);
@@ -76,7 +93,7 @@ __device__ __forceinline__ void set_dir(Ray &r, float sx, float sy, float sz) {
 }
 
 /* wall emission tally W tra (Bw - Bi)/Bi, then the ray ends */
-__device__ __forceinline__ void wall_hit(const DevSnap &sn, const En &e, int nT, Ray &r, float Bw, float &de) {
+__device__ __forceinline__ void wall_hit(const En &e, Ray &r, float Bw, float &de) {
     de -= (e.W * r.tra) * ((Bw - e.Bi) * e.invBi);
     r.tra = 0.f;
 }
@@ -85,7 +102,8 @@ __device__ __forceinline__ void wall_hit(const DevSnap &sn, const En &e, int nT,
  * left 1..n): periodic, black (reference wall, adiabatic midpoint or fixed
  * temperature), specular, black at the adjacent-cell temperature (z), lost
  * (z) -- mc_gpu_noint.cu:375-654 */
-__device__ __forceinline__ void boundary(const DevCommon &cm, const DevSnap &sn, const DevLevel &L, Ray &r, const En &e,
+template <int TM = TAB_RT, class S>
+__device__ __forceinline__ void boundary(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r, const En &e,
                                       float &de, int &status) {
     const int nT = cm.nT;
     int face;
@@ -110,11 +128,11 @@ __device__ __forceinline__ void boundary(const DevCommon &cm, const DevSnap &sn,
         if ((face == FW && cm.adiab[0]) || (face == FE && cm.adiab[1])) {
             const float Tw = gld(L.Tw, (size_t)(face == FE) * L.jm * L.km + (size_t)(r.jc - 1) + (size_t)L.jm * (r.kc - 1));
             float Kw;
-            kb_at(sn, e.row, e.rank, nT, Tw, Kw, Bw);
+            kb_at<TM>(sn, e.row, e.rank, nT, Tw, Kw, Bw);
         } else {
             Bw = gld(sn.bfix, (size_t)face * cm.nB + e.rank);
         }
-        wall_hit(sn, e, nT, r, Bw, de);
+        wall_hit(e, r, Bw, de);
         status = RC_RAY_WALL;
     } else if (bd == 3) {
         switch (face) {
@@ -128,8 +146,8 @@ __device__ __forceinline__ void boundary(const DevCommon &cm, const DevSnap &sn,
     } else if (bd == 4) {
         const float Tw = gld(L.Tz, (size_t)(face == FT) * L.im * L.jm + (size_t)(r.ic - 1) + (size_t)L.im * (r.jc - 1));
         float Kw, Bw;
-        kb_at(sn, e.row, e.rank, nT, Tw, Kw, Bw);
-        wall_hit(sn, e, nT, r, Bw, de);
+        kb_at<TM>(sn, e.row, e.rank, nT, Tw, Kw, Bw);
+        wall_hit(e, r, Bw, de);
         status = RC_RAY_WALL;
     } else {
         r.tra = 0.f;
@@ -142,7 +160,8 @@ __device__ __forceinline__ void boundary(const DevCommon &cm, const DevSnap &sn,
  * reciprocity tally in the current cell, transmissivity cut-off, then one
  * index move; the boundary condition only when an index leaves the grid.
  * Written with selects so that the common path has no divergent branch. */
-__device__ __forceinline__ void march_step(const DevCommon &cm, const DevSnap &sn, const DevLevel &L, Ray &r,
+template <int TM = TAB_RT, class S>
+__device__ __forceinline__ void march_step(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r,
                                            const En &e, float tol, float &de, int &status) {
     const int nT = cm.nT;
     const float fx = gld(L.xu, r.ic - r.f0);
@@ -160,7 +179,7 @@ __device__ __forceinline__ void march_step(const DevCommon &cm, const DevSnap &s
     r.yp = (ds == dsy) ? fy : fmaf(ds, r.sy, r.yp);
     r.zp = (ds == dsz) ? fz : fmaf(ds, r.sz, r.zp);
     float K, B;
-    kb_at(sn, e.row, e.rank, nT, Tc, K, B);
+    kb_at<TM>(sn, e.row, e.rank, nT, Tc, K, B);
     float em1, ex;
     expm1_exp(-(K * ds), em1, ex);
     const float alpha = -em1;
@@ -181,7 +200,7 @@ __device__ __forceinline__ void march_step(const DevCommon &cm, const DevSnap &s
     r.kc += za ? (r.f2 ? -1 : 1) : 0;
     const bool out = (uint32_t)(r.ic - 1) >= (uint32_t)L.im || (uint32_t)(r.jc - 1) >= (uint32_t)L.jm ||
                      (uint32_t)(r.kc - 1) >= (uint32_t)L.km;
-    if (out) boundary(cm, sn, L, r, e, de, status);
+    if (out) boundary<TM>(cm, sn, L, r, e, de, status);
 }
 
 /* one grid level of the march (march_ray, mc_gpu_noint.cu:245-658);
@@ -225,14 +244,15 @@ __device__ __forceinline__ void march_all(const DevCommon &cm, const DevSnap &sn
 
 /* per-ray energy constants: W = Ibmax B(Ti)/B(Tmax) K(Ti)/K(Tmax) for the
  * fluid (:181-186), W = Ibmax_s B(Tw)/B(Tmax) for walls (:1419-1424) */
+template <int TM = TAB_RT>
 __device__ __forceinline__ void energy(const DevCommon &cm, const DevSnap &sn, uint32_t key, float Ti, bool surface,
                                        int bound, En &e) {
     const int rank = (int)(key / (uint32_t)cm.nQ);
     e.rank = rank;
     e.row = key;
     float Ki, Bi, Km, Bm;
-    kb_at(sn, e.row, rank, cm.nT, Ti, Ki, Bi);
-    kb_at(sn, e.row, rank, cm.nT, sn.Tmax, Km, Bm);
+    kb_at<TM>(sn, e.row, rank, cm.nT, Ti, Ki, Bi);
+    kb_at<TM>(sn, e.row, rank, cm.nT, sn.Tmax, Km, Bm);
     e.Bi = Bi;
     e.invBi = 1.0f / Bi;
     if (!surface) e.W = sn.Ibmax * (((Bi / Bm) * Ki) / Km);
@@ -246,6 +266,36 @@ __device__ __forceinline__ void cell_ijk(const DevCommon &cm, int n, int &i, int
     k = n / (cm.imax * cm.jmax) + cm.zstart;
 }
 
+/* isotropic emission direction of a fluid ray from its uniforms (a, b),
+ * with exact zeros moved off the axes (kernel_fluid :131-160) */
+__device__ __forceinline__ void fluid_dir(float a, float b, float &sx, float &sy, float &sz) {
+    iso_dir(a, b, sx, sy, sz);
+    if (sx == 0.f) sx = 1e-10f;
+    if (sy == 0.f) sy = 1e-10f;
+    if (sz == 0.f) sz = 1e-10f;
+}
+
+/* Lambertian emission direction from wall `bound` (kernel_surface
+ * :1398-1417) */
+__device__ __forceinline__ void surface_dir(float a, float b, int bound, float &sx, float &sy, float &sz) {
+    lamb_dir(a, b, bound, sx, sy, sz);
+    if (fabsf(sx) <= 1e-10) sx = (float)(1 - 2 * bound) * 1e-10f;
+    if (fabsf(sy) <= 1e-10) sy = (float)((sy > 0.f) - (sy <= 0.f)) * 1e-10f;
+    if (fabsf(sz) <= 1e-10) sz = (float)((sz > 0.f) - (sz <= 0.f)) * 1e-10f;
+}
+
+/* on-face index adjustment of a wall ray placed on the wall cell face
+ * (:1429-1441): an index moves when the ray starts on a face it leaves */
+__device__ __forceinline__ void surface_adjust(const DevSnap &sn, Ray &r) {
+    const DevLevel &L0 = sn.lev[0];
+    if (r.f0 && r.xp == gld(L0.xu, r.ic - 1)) r.ic -= 1;
+    if (r.f1 && r.yp == (float)(r.jc - 1) * L0.dy) r.jc -= 1;
+    if (r.f2 && r.zp == (float)(r.kc - 1) * L0.dz) r.kc -= 1;
+    if (!r.f0 && r.xp == gld(L0.xu, r.ic)) r.ic += 1;
+    if (!r.f1 && r.yp == (float)r.jc * L0.dy) r.jc += 1;
+    if (!r.f2 && r.zp == (float)r.kc * L0.dz) r.kc += 1;
+}
+
 /* set-up of one fluid ray from the centre of cell (i,j,k) (kernel_fluid
  * :115-186): direction, band, per-ray energy constants */
 __device__ __forceinline__ uint32_t fluid_setup(const DevCommon &cm, const DevSnap &sn, int i, int j, int k,
@@ -254,10 +304,7 @@ __device__ __forceinline__ uint32_t fluid_setup(const DevCommon &cm, const DevSn
     const U4 u = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)v, cid, TAG_FLUID + 16u * (uint32_t)sn.snap);
     if (!have_key) key = band_key(cm, sn, false, u.c, u.d);
     float sx, sy, sz;
-    iso_dir(u.a, u.b, sx, sy, sz);
-    if (sx == 0.f) sx = 1e-10f;
-    if (sy == 0.f) sy = 1e-10f;
-    if (sz == 0.f) sz = 1e-10f;
+    fluid_dir(u.a, u.b, sx, sy, sz);
     set_dir(r, sx, sy, sz);
     r.ic = i; r.jc = j; r.kc = k;
     r.xp = gld(sn.x0, i); r.yp = gld(sn.y0, j); r.zp = gld(sn.z0, k);
@@ -275,23 +322,13 @@ __device__ __forceinline__ uint32_t surface_setup(const DevCommon &cm, const Dev
                        (bound ? TAG_EAST : TAG_WEST) + 16u * (uint32_t)sn.snap);
     if (!have_key) key = band_key(cm, sn, true, u.c, u.d);
     float sx, sy, sz;
-    lamb_dir(u.a, u.b, bound, sx, sy, sz);
-    if (fabsf(sx) <= 1e-10) sx = (float)(1 - 2 * bound) * 1e-10f;
-    if (fabsf(sy) <= 1e-10) sy = (float)((sy > 0.f) - (sy <= 0.f)) * 1e-10f;
-    if (fabsf(sz) <= 1e-10) sz = (float)((sz > 0.f) - (sz <= 0.f)) * 1e-10f;
+    surface_dir(u.a, u.b, bound, sx, sy, sz);
     set_dir(r, sx, sy, sz);
     r.ic = bound ? cm.imax : 1; r.jc = j; r.kc = k;
     r.xp = bound ? cm.Lx : 0.f; r.yp = gld(sn.y0, j); r.zp = gld(sn.z0, k);
     r.tra = 1.0f;
     energy(cm, sn, key, Tw, true, bound, e);
-    /* on-face index adjustment (:1429-1441) */
-    const DevLevel &L0 = sn.lev[0];
-    if (r.f0 && r.xp == gld(L0.xu, r.ic - 1)) r.ic -= 1;
-    if (r.f1 && r.yp == (float)(r.jc - 1) * L0.dy) r.jc -= 1;
-    if (r.f2 && r.zp == (float)(r.kc - 1) * L0.dz) r.kc -= 1;
-    if (!r.f0 && r.xp == gld(L0.xu, r.ic)) r.ic += 1;
-    if (!r.f1 && r.yp == (float)r.jc * L0.dy) r.jc += 1;
-    if (!r.f2 && r.zp == (float)r.kc * L0.dz) r.kc += 1;
+    surface_adjust(sn, r);
     return key;
 }
 
