@@ -145,15 +145,22 @@ def main() -> int:
     bf16 = args.tables == "bf16" and not p.grey
 
     if args.probe:
-        import torch
-        n = (512 << 20) // 4
-        x = torch.full((n,), 1.0, dtype=torch.float32, device="cuda")
-        y = torch.empty_like(x)
-        torch.mul(x, 1.0, out=y)                   # reads 512 MiB, writes 512 MiB
-        torch.cuda.synchronize()
+        # a device-to-device copy of 512 MiB through the HIP runtime (a blit
+        # kernel: reads and writes 512 MiB), then one batched solve
+        import ctypes
+        hip = ctypes.CDLL("libamdhip64.so")
+        n = 512 << 20
+        a, b = ctypes.c_void_p(), ctypes.c_void_p()
+        if hip.hipMalloc(ctypes.byref(a), ctypes.c_size_t(n)) or hip.hipMalloc(ctypes.byref(b), ctypes.c_size_t(n)):
+            raise RuntimeError("hipMalloc failed")
+        hip.hipMemset(a, 1, ctypes.c_size_t(n))
+        hip.hipMemcpy(b, a, ctypes.c_size_t(n), 3)           # hipMemcpyDeviceToDevice
+        hip.hipDeviceSynchronize()
         s.set_table_precision(bf16)
         st = s.solve(VARIANT_BATCHED)
-        print(json.dumps({"probe": args.workload, "tables": args.tables, "copy_bytes": 4 * n,
+        hip.hipFree(a)
+        hip.hipFree(b)
+        print(json.dumps({"probe": args.workload, "tables": args.tables, "copy_bytes": n,
                           "rays": st["rays"], "steps": st["steps"], "march_ms": st["march_ms"]}), flush=True)
         s.close()
         return 0

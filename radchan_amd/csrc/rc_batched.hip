@@ -35,13 +35,14 @@ using namespace rcd;
 namespace rc {
 
 /* bitonic sort of n2 (power of two) keys in LDS by one 64-lane workgroup */
-__device__ __forceinline__ void lds_sort(uint32_t *lds, int n2) {
+template <class K>
+__device__ __forceinline__ void lds_sort(K *lds, int n2) {
     for (int k = 2; k <= n2; k <<= 1)
         for (int j = k >> 1; j > 0; j >>= 1) {
             for (int i = threadIdx.x; i < n2; i += 64) {
                 const int ixj = i ^ j;
                 if (ixj > i) {
-                    const uint32_t a = lds[i], b = lds[ixj];
+                    const K a = lds[i], b = lds[ixj];
                     const bool up = (i & k) == 0;
                     if ((a > b) == up) { lds[i] = b; lds[ixj] = a; }
                 }
@@ -146,7 +147,6 @@ __device__ __forceinline__ SnapView snap_view(const DevSnap *snp) {
     v.kb = uni(snp->kb);
     v.kp16 = uni(snp->kp16);
     v.bp = uni(snp->bp);
-    v.kscale = uni(snp->kscale);
     v.bfix = uni(snp->bfix);
     return v;
 }
@@ -155,7 +155,7 @@ __device__ __forceinline__ DevLevel level_view(const DevLevel *lp) {
     DevLevel L;
     L.im = uni(lp->im); L.jm = uni(lp->jm); L.km = uni(lp->km); L.maxs = uni(lp->maxs);
     L.dy = uni(lp->dy); L.dz = uni(lp->dz);
-    L.pad0 = 0; L.pad1 = 0;
+    L.ncell = uni(lp->ncell); L.pad1 = 0;
     L.xu = uni(lp->xu); L.T = uni(lp->T); L.Tw = uni(lp->Tw); L.Tz = uni(lp->Tz);
     return L;
 }
@@ -169,120 +169,142 @@ __device__ __forceinline__ Unit uni(const Unit &a) {
 }
 
 /* LDS ray queue: rays generated in bulk (all 64 lanes) wait here until an
- * idle lane picks them up.  Structure of arrays, QCAP (power of two) slots
- * per wavefront; a queued ray holds its direction, energy constants, table
- * row and the unit slot it belongs to.  The start cell and position come
- * from the unit slot when a lane picks the ray up. */
-constexpr int QCAP = 128;
-enum { Q_SX, Q_SY, Q_SZ, Q_ISX, Q_ISY, Q_ISZ, Q_W, Q_BI, Q_IBI, Q_KEY, Q_SLOT, Q_WORDS };
+ * idle lane picks them up.  QCAP records of 12 words per
+ * wavefront, written and read as three 16-byte words: direction,
+ * reciprocals, W, Bi, 1/Bi, table row and the unit slot the ray belongs to.
+ * The start cell and position come from the unit slot at pick-up. */
+constexpr int QCAP = 96;
 
-/* Unit slots: the units a wavefront has in flight (rays still queued or
- * marching).  Each holds the FP64 tally of its rays, the number of rays not
- * yet finished, the tally index and the start cell and position of its
- * rays.  The ray that finishes last stores the tally, so every (cell, run)
- * total is written once, by the one wavefront that owns the unit, with a
- * plain store.  A slot is reused only once its count is zero. */
-constexpr int NSLOT = 32;
-constexpr int SLOT_WORDS = 10;     /* acc (2 words), cnt, tid, ic, jc, kc, xp, yp, zp */
+/* ring index: i < 2 QCAP */
+__device__ __forceinline__ int qwrap(int i) { return i >= QCAP ? i - QCAP : i; }
+constexpr int QREC = 12;
 
+/* Unit slots: the units a wavefront has in flight.  The wavefront numbers
+ * the units it starts (0, 1, 2, ...); unit n lives in slot n mod NSLOT
+ * until unit n + NSLOT starts.  A tally record {FP64 tally, rays finished,
+ * tally index} and a placement record {start cell, start position} per
+ * slot.  A finishing ray adds its tally and its count to its unit's slot
+ * with LDS atomics that return nothing; whether the slot is still its
+ * unit's follows from the unit numbers alone.  When a slot is taken over
+ * (and for the last units, at the end) the previous unit's total goes to
+ * HBM: a plain store when all its rays have finished -- the one wavefront
+ * that owns a unit writes its total once -- or else (a long ray still in
+ * flight) an FP64 atomic add of the partial total, after which the unit's
+ * remaining rays add theirs to HBM the same way. */
+constexpr int NSLOT = 16;
+constexpr int SLOT_WORDS = 12;     /* tally record 4 words + placement record 8 words */
+
+struct SlotTally {
+    double acc;
+    int done, tid;
+};
+struct SlotPlace {
+    int ic, jc, kc;
+    float xp, yp, zp;
+    int pad0, pad1;
+};
 struct Slots {
-    double *acc;
-    int *cnt, *tid, *ic, *jc, *kc;
-    float *xp, *yp, *zp;
+    SlotTally *tal;
+    SlotPlace *pl;
 };
 
 __device__ __forceinline__ Slots slot_view(uint32_t *lds) {
     Slots s;
-    s.acc = reinterpret_cast<double *>(lds);
-    s.cnt = reinterpret_cast<int *>(lds + 2 * NSLOT);
-    s.tid = s.cnt + NSLOT;
-    s.ic = s.tid + NSLOT;
-    s.jc = s.ic + NSLOT;
-    s.kc = s.jc + NSLOT;
-    s.xp = reinterpret_cast<float *>(s.kc + NSLOT);
-    s.yp = s.xp + NSLOT;
-    s.zp = s.yp + NSLOT;
+    s.tal = reinterpret_cast<SlotTally *>(lds);
+    s.pl = reinterpret_cast<SlotPlace *>(lds + 4 * NSLOT);
     return s;
 }
 
 __device__ __forceinline__ void q_put(uint32_t *q, int qs, float sx, float sy, float sz, const En &e, uint32_t key,
-                                      int slot) {
-    q[Q_SX * QCAP + qs] = __float_as_uint(sx);
-    q[Q_SY * QCAP + qs] = __float_as_uint(sy);
-    q[Q_SZ * QCAP + qs] = __float_as_uint(sz);
-    q[Q_ISX * QCAP + qs] = __float_as_uint(1.0f / sx);
-    q[Q_ISY * QCAP + qs] = __float_as_uint(1.0f / sy);
-    q[Q_ISZ * QCAP + qs] = __float_as_uint(1.0f / sz);
-    q[Q_W * QCAP + qs] = __float_as_uint(e.W);
-    q[Q_BI * QCAP + qs] = __float_as_uint(e.Bi);
-    q[Q_IBI * QCAP + qs] = __float_as_uint(e.invBi);
-    q[Q_KEY * QCAP + qs] = key;
-    q[Q_SLOT * QCAP + qs] = (uint32_t)slot;
+                                      int unit, int tid) {
+    uint4 *rec = reinterpret_cast<uint4 *>(q + qs * QREC);
+    rec[0] = make_uint4(__float_as_uint(sx), __float_as_uint(sy), __float_as_uint(sz), __float_as_uint(1.0f / sx));
+    rec[1] = make_uint4(__float_as_uint(1.0f / sy), __float_as_uint(1.0f / sz), __float_as_uint(e.W),
+                        __float_as_uint(e.Bi));
+    rec[2] = make_uint4(__float_as_uint(e.invBi), key | ((uint32_t)e.rank << 16), (uint32_t)unit, (uint32_t)tid);
 }
 
 template <bool SURF>
 __device__ __forceinline__ void q_get(const uint32_t *q, int qs, const Slots &sl, const DevCommon &cm,
-                                      const DevSnap &sn, Ray &r, En &e, int &slot) {
-    r.sx = __uint_as_float(q[Q_SX * QCAP + qs]);
-    r.sy = __uint_as_float(q[Q_SY * QCAP + qs]);
-    r.sz = __uint_as_float(q[Q_SZ * QCAP + qs]);
-    r.isx = __uint_as_float(q[Q_ISX * QCAP + qs]);
-    r.isy = __uint_as_float(q[Q_ISY * QCAP + qs]);
-    r.isz = __uint_as_float(q[Q_ISZ * QCAP + qs]);
+                                      const DevSnap &sn, Ray &r, En &e, int &unit, int &tid) {
+    const uint4 *rec = reinterpret_cast<const uint4 *>(q + qs * QREC);
+    const uint4 a = rec[0], b = rec[1], c = rec[2];
+    r.sx = __uint_as_float(a.x);
+    r.sy = __uint_as_float(a.y);
+    r.sz = __uint_as_float(a.z);
+    r.isx = __uint_as_float(a.w);
+    r.isy = __uint_as_float(b.x);
+    r.isz = __uint_as_float(b.y);
     r.f0 = r.sx < 0.f; r.f1 = r.sy < 0.f; r.f2 = r.sz < 0.f;
     r.tra = 1.0f;
-    e.W = __uint_as_float(q[Q_W * QCAP + qs]);
-    e.Bi = __uint_as_float(q[Q_BI * QCAP + qs]);
-    e.invBi = __uint_as_float(q[Q_IBI * QCAP + qs]);
-    const uint32_t key = q[Q_KEY * QCAP + qs];
-    e.row = key;
-    e.rank = (int)(key / (uint32_t)cm.nQ);
-    slot = (int)q[Q_SLOT * QCAP + qs];
-    r.ic = sl.ic[slot]; r.jc = sl.jc[slot]; r.kc = sl.kc[slot];
-    r.xp = sl.xp[slot]; r.yp = sl.yp[slot]; r.zp = sl.zp[slot];
+    e.W = __uint_as_float(b.z);
+    e.Bi = __uint_as_float(b.w);
+    e.invBi = __uint_as_float(c.x);
+    e.row = c.y & 0xFFFFu;
+    e.rank = (int)(c.y >> 16);
+    unit = (int)c.z;
+    tid = (int)c.w;
+    const uint4 *pr = reinterpret_cast<const uint4 *>(&sl.pl[unit & (NSLOT - 1)]);
+    const uint4 p0 = pr[0], p1 = pr[1];
+    r.ic = (int)p0.x; r.jc = (int)p0.y; r.kc = (int)p0.z;
+    r.xp = __uint_as_float(p0.w); r.yp = __uint_as_float(p1.x); r.zp = __uint_as_float(p1.y);
     if (SURF) surface_adjust(sn, r);
 }
 
 /* LDS words of one wavefront: unit slots, ray queue, (multigrid) the grid
- * levels of the current snapshot, then (srt) the unit's sorted keys and the
- * direction uniforms of its rays */
+ * levels of the current snapshot, then (srt) the direction uniforms of the
+ * unit's rays and its sorted keys (16-bit: keys are below nB nQ <= 65535) */
 constexpr int LEV_WORDS = (int)(sizeof(DevLevel) / 4);
 __host__ __device__ __forceinline__ int batched_lds_words(int n2, int srt, int multi) {
-    return SLOT_WORDS * NSLOT + Q_WORDS * QCAP + (multi ? LEV_WORDS * RC_MAXLEV : 0) + (srt ? 3 * n2 : 0);
+    return SLOT_WORDS * NSLOT + QREC * QCAP + (multi ? LEV_WORDS * RC_MAXLEV : 0) + (srt ? 2 * n2 + (n2 + 1) / 2 : 0);
 }
 
 template <bool SURF, bool MULTI, int TM>
-__global__ void __launch_bounds__(64) k_batched(DevCommon cm, const DevSnap *__restrict__ snaps, int nsnap,
-                                                double *__restrict__ de, unsigned long long *__restrict__ stats,
-                                                int *__restrict__ queue, int threshold) {
+__device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap *__restrict__ snaps, int nsnap,
+                                             double *__restrict__ de, unsigned long long *__restrict__ stats,
+                                             int *__restrict__ queue, int threshold) {
     extern __shared__ uint32_t lds[];
     const Slots sl = slot_view(lds);
     uint32_t *rq = lds + SLOT_WORDS * NSLOT;
-    DevLevel *llev = reinterpret_cast<DevLevel *>(rq + Q_WORDS * QCAP);   /* 8-byte aligned */
+    DevLevel *llev = reinterpret_cast<DevLevel *>(rq + QREC * QCAP);   /* 16-byte aligned */
     const int P = cm.photmax;
     int n2 = 1;
     while (n2 < P) n2 <<= 1;
-    uint32_t *keys = rq + Q_WORDS * QCAP + (MULTI ? LEV_WORDS * RC_MAXLEV : 0);
-    float *dira = reinterpret_cast<float *>(keys + n2);
+    float *dira = reinterpret_cast<float *>(rq + QREC * QCAP + (MULTI ? LEV_WORDS * RC_MAXLEV : 0));
     float *dirb = dira + n2;
+    uint16_t *keys = reinterpret_cast<uint16_t *>(dirb + n2);
     const int lane = threadIdx.x;
     const int per_snap = SURF ? 2 * cm.nsurf : cm.ncell;
     const int total = per_snap * nsnap * cm.nVar;
     const float tol = SURF ? cm.tollS : cm.toll;
-    if (lane < NSLOT) sl.cnt[lane] = 0;
+
     __syncthreads();
 
     /* wavefront-uniform state: dispatch of units, unit slots, ray queue */
-    int cur_h = P, nxt = -1, nxt_s = -1, qdone = 0, snap = -1, seq = 0, cslot = 0;
+    int cur_h = P, nxt = -1, nxt_s = -1, qdone = 0, snap = -1, seq = 0, cunit = 0, ctid = 0;
     int qhead = 0, qcount = 0;
+    const int chunk = min(max(cm.nVar, 1), 16);
+    int cnext = 0, cleft = 0;
     Unit U{};
 
-    /* lane state */
+    /* lane state; an idle lane keeps in-range values, since the march step
+     * runs in every lane */
     bool active = false;
     Ray r;
+    r.ic = r.jc = r.kc = 1;
+    r.f0 = r.f1 = r.f2 = 0;
+    r.xp = r.yp = r.zp = 0.f;
+    r.sx = r.sy = r.sz = 1.f;
+    r.isx = r.isy = r.isz = 1.f;
+    r.tra = 0.f;
     En e;
+    e.row = 0u;
+    e.rank = 0;
+    e.W = 0.f;
+    e.Bi = 1.f;
+    e.invBi = 1.f;
     float dacc = 0.f;
-    int rslot = 0, lev = 0, counter = 0;
+    int runit = 0, rtid = 0, lev = 0, counter = 0;
     int nsteps = 0, nrays = 0;
 
     /* one back-edge: every iteration either starts a unit or generates,
@@ -292,9 +314,18 @@ __global__ void __launch_bounds__(64) k_batched(DevCommon cm, const DevSnap *__r
          * and (srt) draw and sort its band keys, keeping the direction
          * uniforms of every ray */
         if (nxt < 0 && !qdone && cur_h >= P) {
-            int u = 0;
-            if (lane == 0) u = atomicAdd(queue, 1);
-            u = uni(u);
+            /* units are taken from the device queue in chunks of nVar
+             * (one cell's runs), so that one wavefront writes the cell's
+             * consecutive tallies (whole cache lines in one L2) */
+            if (cleft == 0) {
+                int u0 = 0;
+                if (lane == 0) u0 = atomicAdd(queue, chunk);
+                cnext = uni(u0);
+                cleft = chunk;
+            }
+            const int u = cnext;
+            cnext += 1;
+            cleft -= 1;
             if (u >= total) {
                 qdone = 1;
             } else {
@@ -305,11 +336,10 @@ __global__ void __launch_bounds__(64) k_batched(DevCommon cm, const DevSnap *__r
                     const DevSnap &sn = snaps[N.s];
                     const uint32_t tagx = N.tag + 16u * (uint32_t)sn.snap;
                     if (n2 <= 256) {
-                        /* up to 256 keys: sorted in registers */
-                        uint32_t x[4];
-#pragma unroll
-                        for (int q4 = 0; q4 < 4; ++q4) {
-                            const int h = q4 * 64 + lane;
+                        /* up to 256 keys: drawn one round of 64 at a time,
+                         * then sorted in registers */
+#pragma unroll 1
+                        for (int h = lane; h < 256; h += 64) {
                             uint32_t key = 0xFFFFFFFFu;
                             if (h < P) {
                                 const U4 q = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)N.v, N.cid, tagx);
@@ -317,12 +347,16 @@ __global__ void __launch_bounds__(64) k_batched(DevCommon cm, const DevSnap *__r
                                 dira[h] = q.a;
                                 dirb[h] = q.b;
                             }
-                            x[q4] = key;
+                            if (h < n2) keys[h] = (uint16_t)key;
                         }
+                        __syncthreads();
+                        uint32_t x[4];
+#pragma unroll
+                        for (int q4 = 0; q4 < 4; ++q4) x[q4] = (q4 * 64 + lane < n2) ? (uint32_t)keys[q4 * 64 + lane] : 0xFFFFu;
                         wave_sort256(x, lane);
 #pragma unroll
                         for (int q4 = 0; q4 < 4; ++q4)
-                            if (q4 * 64 + lane < n2) keys[q4 * 64 + lane] = x[q4];
+                            if (q4 * 64 + lane < n2) keys[q4 * 64 + lane] = (uint16_t)x[q4];
                         __syncthreads();
                     } else {
                         for (int h = lane; h < n2; h += 64) {
@@ -333,7 +367,7 @@ __global__ void __launch_bounds__(64) k_batched(DevCommon cm, const DevSnap *__r
                                 dira[h] = q.a;
                                 dirb[h] = q.b;
                             }
-                            keys[h] = key;
+                            keys[h] = (uint16_t)key;
                         }
                         __syncthreads();
                         lds_sort(keys, n2);
@@ -343,9 +377,10 @@ __global__ void __launch_bounds__(64) k_batched(DevCommon cm, const DevSnap *__r
         }
         const unsigned long long live = __ballot(active);
         if (cur_h >= P && nxt >= 0 && (nxt_s == snap || (live == 0ull && qcount == 0)) &&
-            uni(sl.cnt[seq & (NSLOT - 1)]) == 0) {
-            /* (b) start the prepared unit once its slot is free (a new
-             * snapshot only once every lane and the queue are empty) */
+            (qcount == 0 || seq - uni((int)rq[qhead * QREC + 10]) < NSLOT)) {
+            /* (b) start the prepared unit (a new snapshot only once every
+             * lane and the queue are empty; never over the slot of a ray
+             * still queued) */
             U = uni(decode<SURF>(cm, snaps, nxt, per_snap));
             nxt = -1;
             if (MULTI && U.s != snap) {
@@ -355,24 +390,30 @@ __global__ void __launch_bounds__(64) k_batched(DevCommon cm, const DevSnap *__r
             }
             snap = U.s;
             const int tidx = (U.s * per_snap + U.n) * cm.nVar + U.v;
-            if (U.skip) {
-                if (lane == 0) de[tidx] = 0.0;
-            } else {
-                cslot = seq & (NSLOT - 1);
-                seq += 1;
+            if (!U.skip) {
+                const int cslot = seq & (NSLOT - 1);
+                cunit = seq;
+                ctid = tidx;
                 cur_h = 0;
                 if (lane == 0) {
                     const DevSnap &sn = snaps[U.s];
-                    sl.acc[cslot] = 0.0;
-                    sl.cnt[cslot] = P;
-                    sl.tid[cslot] = tidx;
-                    sl.ic[cslot] = SURF ? (U.bound ? cm.imax : 1) : U.i;
-                    sl.jc[cslot] = U.j;
-                    sl.kc[cslot] = U.k;
-                    sl.xp[cslot] = SURF ? (U.bound ? cm.Lx : 0.f) : gld(sn.x0, U.i);
-                    sl.yp[cslot] = gld(sn.y0, U.j);
-                    sl.zp[cslot] = gld(sn.z0, U.k);
+                    SlotTally &t = sl.tal[cslot];
+                    if (seq >= NSLOT) {
+                        if (t.done == P) de[t.tid] = t.acc;
+                        else atomicAdd(&de[t.tid], t.acc);
+                    }
+                    t.acc = 0.0;
+                    t.done = 0;
+                    t.tid = tidx;
+                    SlotPlace &pl = sl.pl[cslot];
+                    pl.ic = SURF ? (U.bound ? cm.imax : 1) : U.i;
+                    pl.jc = U.j;
+                    pl.kc = U.k;
+                    pl.xp = SURF ? (U.bound ? cm.Lx : 0.f) : gld(sn.x0, U.i);
+                    pl.yp = gld(sn.y0, U.j);
+                    pl.zp = gld(sn.z0, U.k);
                 }
+                seq += 1;
             }
             __syncthreads();
         } else {
@@ -386,7 +427,7 @@ __global__ void __launch_bounds__(64) k_batched(DevCommon cm, const DevSnap *__r
                     float a, b;
                     uint32_t key;
                     if (cm.srt) {
-                        key = keys[h];
+                        key = (uint32_t)keys[h];
                         a = dira[h];
                         b = dirb[h];
                     } else {
@@ -400,7 +441,7 @@ __global__ void __launch_bounds__(64) k_batched(DevCommon cm, const DevSnap *__r
                     else fluid_dir(a, b, sx, sy, sz);
                     En ge;
                     energy<TM>(cm, sn, key, U.Ti, SURF, U.bound, ge);
-                    q_put(rq, (qhead + qcount + lane) & (QCAP - 1), sx, sy, sz, ge, key, cslot);
+                    q_put(rq, qwrap(qhead + qcount + lane), sx, sy, sz, ge, key, cunit, ctid);
                 }
                 __syncthreads();
                 qcount += ng;
@@ -413,13 +454,13 @@ __global__ void __launch_bounds__(64) k_batched(DevCommon cm, const DevSnap *__r
                 const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
                 const int npop = min(nidle, qcount);
                 if (!active && rank < npop) {
-                    q_get<SURF>(rq, (qhead + rank) & (QCAP - 1), sl, cm, snaps[snap], r, e, rslot);
+                    q_get<SURF>(rq, qwrap(qhead + rank), sl, cm, snaps[snap], r, e, runit, rtid);
                     active = true;
                     dacc = 0.f;
                     lev = 0;
                     counter = 0;
                 }
-                qhead = (qhead + npop) & (QCAP - 1);
+                qhead = qwrap(qhead + npop);
                 qcount -= npop;
             }
             if (__ballot(active) == 0ull) {
@@ -432,21 +473,30 @@ __global__ void __launch_bounds__(64) k_batched(DevCommon cm, const DevSnap *__r
                 DevLevel l0;
                 if (!MULTI) l0 = level_view(&snaps[snap].lev[0]);
                 const DevLevel L = MULTI ? llev[lev] : l0;
+                /* Every active lane holds a live ray under its level's step
+                 * cap: a ray is finished (or coarsened) right after the step
+                 * that ends it, so that its lane is refilled in the next
+                 * iteration.  Idle lanes step too, on stale state that the
+                 * next pick-up overwrites; the step bounds its loads. */
+                int status = 0;
+                const bool out = march_core<TM>(cm, sv, L, r, e, tol, dacc, status);
+                if (active && out) boundary<TM>(cm, sv, L, r, e, dacc, status);
+                counter += 1;
+                nsteps += active ? 1 : 0;
                 const bool alive = r.tra > tol;
-                const bool go = active && alive && counter < L.maxs;
-                const bool fin = active && !go && (!MULTI || !alive || lev == cm.grid_num - 1);
-                const bool crs = MULTI && active && !go && !fin;
-                if (go) {
-                    int status;
-                    march_step<TM>(cm, sv, L, r, e, tol, dacc, status);
-                    counter += 1;
-                    nsteps += 1;
-                }
+                const bool done = active && (!alive || counter >= L.maxs);
+                const bool fin = done && (!MULTI || !alive || lev == cm.grid_num - 1);
+                const bool crs = MULTI && done && !fin;
                 if (fin) {
-                    /* the ray ends: add its tally to its unit; the last ray
-                     * of the unit stores the unit's total */
-                    atomicAdd(&sl.acc[rslot], (double)dacc);
-                    if (atomicSub(&sl.cnt[rslot], 1) == 1) de[sl.tid[rslot]] = sl.acc[rslot];
+                    /* the ray ends: its tally and count go to its unit's
+                     * slot, or to HBM once the slot has been taken over */
+                    if (seq - runit <= NSLOT) {
+                        SlotTally &t = sl.tal[runit & (NSLOT - 1)];
+                        atomicAdd(&t.acc, (double)dacc);
+                        atomicAdd(&t.done, 1);
+                    } else {
+                        atomicAdd(&de[rtid], (double)dacc);
+                    }
                     nrays += 1;
                     active = false;
                 }
@@ -463,6 +513,13 @@ __global__ void __launch_bounds__(64) k_batched(DevCommon cm, const DevSnap *__r
             }
         }
     }
+    /* the last units' totals (every ray has finished) */
+    __syncthreads();
+    if (lane < NSLOT && lane < seq) {
+        const SlotTally &t = sl.tal[lane];
+        if (t.done == P) de[t.tid] = t.acc;
+        else atomicAdd(&de[t.tid], t.acc);
+    }
     for (int o = 32; o > 0; o >>= 1) {
         nsteps += __shfl_xor(nsteps, o);
         nrays += __shfl_xor(nrays, o);
@@ -471,6 +528,22 @@ __global__ void __launch_bounds__(64) k_batched(DevCommon cm, const DevSnap *__r
         atomicAdd(&stats[0], (unsigned long long)(uint32_t)nrays);
         atomicAdd(&stats[1], (unsigned long long)(uint32_t)nsteps);
     }
+}
+
+/* single grid: registers and LDS sized for 5 wavefronts per SIMD */
+template <bool SURF, int TM>
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8)))
+k_batched(DevCommon cm, const DevSnap *__restrict__ snaps, int nsnap, double *__restrict__ de,
+          unsigned long long *__restrict__ stats, int *__restrict__ queue, int threshold) {
+    batched_body<SURF, false, TM>(cm, snaps, nsnap, de, stats, queue, threshold);
+}
+
+/* multigrid */
+template <bool SURF, int TM>
+__global__ void __launch_bounds__(64)
+k_batched_mg(DevCommon cm, const DevSnap *__restrict__ snaps, int nsnap, double *__restrict__ de,
+             unsigned long long *__restrict__ stats, int *__restrict__ queue, int threshold) {
+    batched_body<SURF, true, TM>(cm, snaps, nsnap, de, stats, queue, threshold);
 }
 
 __global__ void k_zero_int(int *p, int n) {
@@ -515,16 +588,17 @@ void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, i
     const long total = (long)nde;
     const int grid = (int)std::max<long>(1, std::min<long>((long)ncu * wg_per_cu, total));
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, queue, 1);
+    launch_zero(st, de, nde);
     const bool multi = cm.grid_num > 1;
     const bool bf = snaps_bf16;
-#define RC_LAUNCH(S, M, T) hipLaunchKernelGGL((k_batched<S, M, T>), dim3(grid), dim3(64), lds, st, cm, snaps, nsnap, de, \
-                                           stats, queue, threshold)
+#define RC_LAUNCH(K, S, T) hipLaunchKernelGGL((K<S, T>), dim3(grid), dim3(64), lds, st, cm, snaps, nsnap, de, stats, \
+                                           queue, threshold)
     if (!surface) {
-        if (multi) { if (bf) RC_LAUNCH(false, true, TAB_BF16); else RC_LAUNCH(false, true, TAB_F32); }
-        else { if (bf) RC_LAUNCH(false, false, TAB_BF16); else RC_LAUNCH(false, false, TAB_F32); }
+        if (multi) { if (bf) RC_LAUNCH(k_batched_mg, false, TAB_BF16); else RC_LAUNCH(k_batched_mg, false, TAB_F32); }
+        else { if (bf) RC_LAUNCH(k_batched, false, TAB_BF16); else RC_LAUNCH(k_batched, false, TAB_F32); }
     } else {
-        if (multi) { if (bf) RC_LAUNCH(true, true, TAB_BF16); else RC_LAUNCH(true, true, TAB_F32); }
-        else { if (bf) RC_LAUNCH(true, false, TAB_BF16); else RC_LAUNCH(true, false, TAB_F32); }
+        if (multi) { if (bf) RC_LAUNCH(k_batched_mg, true, TAB_BF16); else RC_LAUNCH(k_batched_mg, true, TAB_F32); }
+        else { if (bf) RC_LAUNCH(k_batched, true, TAB_BF16); else RC_LAUNCH(k_batched, true, TAB_F32); }
     }
 #undef RC_LAUNCH
 }

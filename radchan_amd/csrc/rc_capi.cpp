@@ -79,7 +79,7 @@ struct TableSet {
     float Tnb0 = 0.f, invdT = 0.f;
     int bf16 = 0;
     std::vector<float> Tnb;
-    DevBuf kb, bfix, cdf_gs, rank_of, kp16, bp, kscale;
+    DevBuf kb, bfix, cdf_gs, rank_of, kp16, bp;
 };
 
 /* round-to-nearest-even FP32 -> BF16 (finite inputs) */
@@ -163,8 +163,7 @@ int build_tableset(rc_ctx *c, const Tables &t, TableSet &ts, std::vector<int32_t
     if (ts.bf16) {
         /* k as BF16 pairs with a power-of-two scale per band: 2^floor(log2 max k) */
         std::vector<uint32_t> kp((size_t)nB * nQ * nT);
-        std::vector<float2> bp((size_t)nB * nT);
-        std::vector<float> ksc(nB);
+        std::vector<float4> bp((size_t)nB * nT);
         for (int r = 0; r < nB; ++r) {
             float mx = 0.f;
             for (int g = 0; g < nQ; ++g)
@@ -172,7 +171,6 @@ int build_tableset(rc_ctx *c, const Tables &t, TableSet &ts, std::vector<int32_t
             int ex = 0;
             (void)std::frexp(mx > 0.f ? mx : 1.f, &ex);
             const float sc = std::ldexp(1.0f, ex - 1), inv = std::ldexp(1.0f, 1 - ex);
-            ksc[r] = sc;
             for (int g = 0; g < nQ; ++g) {
                 const float2 *row = &kb[((size_t)r * nQ + g) * (nT + 1)];
                 for (int q = 0; q < nT; ++q)
@@ -180,11 +178,10 @@ int build_tableset(rc_ctx *c, const Tables &t, TableSet &ts, std::vector<int32_t
                         (uint32_t)to_bf16(row[q].x * inv) | ((uint32_t)to_bf16(row[q + 1].x * inv) << 16);
             }
             const float2 *row0 = &kb[(size_t)r * nQ * (nT + 1)];
-            for (int q = 0; q < nT; ++q) bp[(size_t)r * nT + q] = make_float2(row0[q].y, row0[q + 1].y);
+            for (int q = 0; q < nT; ++q) bp[(size_t)r * nT + q] = make_float4(row0[q].y, row0[q + 1].y, sc, 0.f);
         }
         if (int e = upload(ts.kp16, kp, c->stream)) return e;
         if (int e = upload(ts.bp, bp, c->stream)) return e;
-        if (int e = upload(ts.kscale, ksc, c->stream)) return e;
     }
     std::vector<float> bfix((size_t)6 * nB);
     for (int f = 0; f < 6; ++f)
@@ -290,7 +287,7 @@ int push_snaps(rc_ctx *c) {
         d.Tnb0 = ts.Tnb0; d.invdT = ts.invdT; d.tstar = S.tstar; d.snap = (int)s;
         d.kb = ts.kb.as<float2>(); d.bfix = ts.bfix.as<float>();
         d.kbf16 = ts.bf16;
-        d.kp16 = ts.kp16.as<uint32_t>(); d.bp = ts.bp.as<float2>(); d.kscale = ts.kscale.as<float>();
+        d.kp16 = ts.kp16.as<uint32_t>(); d.bp = ts.bp.as<float4>();
         d.cdf_b = S.cdf_b.as<float>(); d.cdf_g = S.cdf_g.as<float>(); d.cdf_bs = S.cdf_bs.as<float>();
         d.cdf_gs = ts.cdf_gs.as<float>(); d.rank_of = ts.rank_of.as<int>();
         d.guide_b = S.guide_b.as<int>(); d.guide_bs = S.guide_bs.as<int>(); d.gM = kGuideM;
@@ -299,6 +296,7 @@ int push_snaps(rc_ctx *c) {
         for (int g = 0; g < p.grid_num; ++g) {
             DevLevel &L = d.lev[g];
             L.im = p.maxi[g]; L.jm = p.maxj[g]; L.km = p.maxk[g]; L.maxs = p.maxs[g];
+            L.ncell = L.im * L.jm * L.km;
             L.dy = p.Ly * 1.0f / (float)L.jm;
             L.dz = p.Lz * 1.0f / (float)L.km;
             L.xu = c->xu[g].as<float>();
@@ -552,11 +550,11 @@ static int enqueue_solve(rc_ctx *c, int variant, int *launches, bool timed) {
     } else {
         launch_batched(c->stream, cm, ds, nsnap, 0, c->table_bf16, c->de.as<double>(), c->stats.as<unsigned long long>(),
                        c->queue.as<int>(), c->wg_per_cu, c->threshold);
-        n += 2;
+        n += 3;
         if (walls) {
             launch_batched(c->stream, cm, ds, nsnap, 1, c->table_bf16, c->ds.as<double>(), c->stats.as<unsigned long long>(),
                            c->queue.as<int>() + 16, c->wg_per_cu, c->threshold);
-            n += 2;
+            n += 3;
         }
     }
     if (timed) RC_HIP(hipEventRecord(c->evm1, c->stream));
@@ -643,7 +641,7 @@ int rc_set_graph(rc_ctx *c, int on) {
  * device-event timing of the march) and keep the fastest */
 int rc_autotune(rc_ctx *c, int32_t *best_wg, int32_t *best_th, float *best_ms) {
     if (!c || c->snaps.empty()) { set_error("rc_autotune before rc_set_fields"); return -1; }
-    static const int wgs[] = {8, 12, 16, 24, 32};
+    static const int wgs[] = {8, 12, 16, 20, 24, 32, 40};
     static const int ths[] = {4, 8, 16, 32};
     const int keep_wg = c->wg_per_cu, keep_th = c->threshold;
     float best = 1e30f;

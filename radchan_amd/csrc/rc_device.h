@@ -20,10 +20,36 @@ namespace rcd {
 template <class T> __device__ __forceinline__ T gld(const T *p, size_t i) {
     return ((const __attribute__((address_space(1))) T *)p)[i];
 }
+/* the same for arrays under 4 GiB, with the byte offset formed in 32 bits:
+ * with a wavefront-uniform base the load takes a scalar base register and
+ * a 32-bit vector offset */
+typedef const __attribute__((address_space(1))) char rc_gchar;
+template <class T> __device__ __forceinline__ T gld32(const T *p, uint32_t i) {
+    return *(const __attribute__((address_space(1))) T *)((rc_gchar *)p + i * (uint32_t)sizeof(T));
+}
 typedef float rc_v2f __attribute__((ext_vector_type(2)));
 __device__ __forceinline__ float2 gld(const float2 *p, size_t i) {
     const rc_v2f v = ((const __attribute__((address_space(1))) rc_v2f *)p)[i];
     return make_float2(v.x, v.y);
+}
+__device__ __forceinline__ float2 gld32(const float2 *p, uint32_t i) {
+    const rc_v2f v = *(const __attribute__((address_space(1))) rc_v2f *)((rc_gchar *)p + i * 8u);
+    return make_float2(v.x, v.y);
+}
+/* two consecutive float2 (8-byte aligned) in one 16-byte load */
+typedef float rc_v4f_a8 __attribute__((ext_vector_type(4), aligned(8)));
+__device__ __forceinline__ float4 gld32_pair(const float2 *p, uint32_t i) {
+    const rc_v4f_a8 v = *(const __attribute__((address_space(1))) rc_v4f_a8 *)((rc_gchar *)p + i * 8u);
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+typedef float rc_v4f __attribute__((ext_vector_type(4)));
+__device__ __forceinline__ float4 gld(const float4 *p, size_t i) {
+    const rc_v4f v = ((const __attribute__((address_space(1))) rc_v4f *)p)[i];
+    return make_float4(v.x, v.y, v.z, v.w);
+}
+__device__ __forceinline__ float4 gld32(const float4 *p, uint32_t i) {
+    const rc_v4f v = *(const __attribute__((address_space(1))) rc_v4f *)((rc_gchar *)p + i * 16u);
+    return make_float4(v.x, v.y, v.z, v.w);
 }
 
 /* ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers:
@@ -141,10 +167,19 @@ __device__ __forceinline__ int upper_bound_guided(const float *__restrict__ cdf,
 }
 
 /* upper_bound over a short non-decreasing CDF: count of entries <= R, all
- * loads issued together */
+ * loads issued together; a row of a multiple of 4 entries (16-byte aligned,
+ * as every g CDF row is) is read as 16-byte words */
 __device__ __forceinline__ int upper_bound_short(const float *__restrict__ cdf, int n, float R) {
     int c = 0;
-    for (int q = 0; q < n; ++q) c += (gld(cdf, q) <= R) ? 1 : 0;
+    if ((n & 3) == 0) {
+        const float4 *c4 = reinterpret_cast<const float4 *>(cdf);
+        for (int q = 0; q < (n >> 2); ++q) {
+            const float4 v = gld(c4, q);
+            c += ((v.x <= R) ? 1 : 0) + ((v.y <= R) ? 1 : 0) + ((v.z <= R) ? 1 : 0) + ((v.w <= R) ? 1 : 0);
+        }
+    } else {
+        for (int q = 0; q < n; ++q) c += (gld(cdf, q) <= R) ? 1 : 0;
+    }
     if (R == 1.0f) c = n - 1;
     if (c > n - 1) c = n - 1;
     return c;

@@ -30,7 +30,7 @@ struct DevCommon {
 struct DevLevel {
     int im, jm, km, maxs;
     float dy, dz;                  /* uniform y/z spacing: face j at j*dy */
-    int pad0, pad1;
+    int ncell, pad1;               /* ncell = im jm km                    */
     const float *xu;               /* im+1 x faces                        */
     const float *T;                /* [km][jm][im] cell temperatures      */
     const float *Tw;               /* [2][km][jm] adiabatic x-wall T      */
@@ -44,8 +44,8 @@ struct DevSnap {
     int tstar, snap;
     const float2 *kb;              /* [(rank*nQ+g)*(nT+1)+t] = {k, Ib}    */
     const uint32_t *kp16;          /* BF16 mode: [(rank*nQ+g)*nT+t] = k_t | k_t+1 << 16 */
-    const float2 *bp;              /* BF16 mode: [rank*nT+t] = {Ib_t, Ib_t+1}         */
-    const float *kscale;           /* BF16 mode: [rank] power-of-two band scale       */
+    const float4 *bp;              /* BF16 mode: [rank*nT+t] = {Ib_t, Ib_t+1, scale, 0}: */
+                                   /* Ib pair and the band's power-of-two k scale       */
     int kbf16, pad3;               /* 1: k tables stored as BF16                       */
     const float *bfix;             /* [6][nB] fixed-temperature wall Ib   */
     const float *cdf_b;            /* [nB] fluid band CDF, row t*         */

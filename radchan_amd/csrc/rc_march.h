@@ -53,7 +53,8 @@ enum { TAB_RT = -1, TAB_F32 = 0, TAB_BF16 = 1 };
  * clamp-to-edge linear interpolation in T (the reference's linear-filtered
  * textures, memory_copy_noint.cu:241-247, mc_gpu_noint.cu:784-791).  FP32
  * tables hold {k, Ib} pairs; BF16 tables hold k_t and k_t+1 as a BF16 pair
- * times a power-of-two band scale, and Ib pairs per band in FP32. */
+ * times a power-of-two band scale, and per band the Ib pairs in FP32 beside
+ * the scale (one 16-byte word). */
 /* the snapshot fields the march reads, held as wavefront-uniform values by
  * the batched kernel (the march functions accept it or a DevSnap) */
 struct SnapView {
@@ -61,8 +62,7 @@ struct SnapView {
     int kbf16;
     const float2 *kb;
     const uint32_t *kp16;
-    const float2 *bp;
-    const float *kscale;
+    const float4 *bp;
     const float *bfix;
 };
 
@@ -71,18 +71,16 @@ __device__ __forceinline__ void kb_at(const S &sn, uint32_t row, int rank, int n
     int t; float a;
     tpos(T, sn.Tnb0, sn.invdT, nT, t, a);
     if (TM == TAB_RT ? sn.kbf16 != 0 : TM == TAB_BF16) {
-        const uint32_t w = gld(sn.kp16, (size_t)row * nT + t);
-        const float2 b = gld(sn.bp, (size_t)rank * nT + t);
-        const float sc = gld(sn.kscale, rank);
-        const float k0 = __uint_as_float(w << 16) * sc;
-        const float k1 = __uint_as_float(w & 0xFFFF0000u) * sc;
+        const uint32_t w = gld32(sn.kp16, row * (uint32_t)nT + (uint32_t)t);
+        const float4 b = gld32(sn.bp, (uint32_t)rank * (uint32_t)nT + (uint32_t)t);
+        const float k0 = __uint_as_float(w << 16) * b.z;
+        const float k1 = __uint_as_float(w & 0xFFFF0000u) * b.z;
         K = lerp(k0, k1, a);
         B = lerp(b.x, b.y, a);
     } else {
-        const size_t base = (size_t)row * (nT + 1) + t;
-        const float2 q0 = gld(sn.kb, base), q1 = gld(sn.kb, base + 1);
-        K = lerp(q0.x, q1.x, a);
-        B = lerp(q0.y, q1.y, a);
+        const float4 q = gld32_pair(sn.kb, row * (uint32_t)(nT + 1) + (uint32_t)t);
+        K = lerp(q.x, q.z, a);
+        B = lerp(q.y, q.w, a);
     }
 }
 
@@ -158,20 +156,24 @@ __device__ __forceinline__ void boundary(const DevCommon &cm, const S &sn, const
 /* one march step (one iteration of march_ray's loop, mc_gpu_noint.cu:252-654):
  * distances to the three candidate faces, position update, attenuation and
  * reciprocity tally in the current cell, transmissivity cut-off, then one
- * index move; the boundary condition only when an index leaves the grid.
- * Written with selects so that the common path has no divergent branch. */
+ * index move.  Written with selects, no branch; returns whether an index
+ * left the grid (the boundary condition is then due). */
 template <int TM = TAB_RT, class S>
-__device__ __forceinline__ void march_step(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r,
+__device__ __forceinline__ bool march_core(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r,
                                            const En &e, float tol, float &de, int &status) {
     const int nT = cm.nT;
-    const float fx = gld(L.xu, r.ic - r.f0);
+    /* both indices are in range for a ray inside the grid; the bounds only
+     * matter to a lane whose state is stale (the batched kernel steps idle
+     * lanes too and discards what they compute) */
+    const float fx = gld32(L.xu, min((uint32_t)(r.ic - r.f0), (uint32_t)L.im));
     const float fy = (float)(r.jc - r.f1) * L.dy;
     const float fz = (float)(r.kc - r.f2) * L.dz;
     const float dsx = (fx - r.xp) * r.isx;
     const float dsy = (fy - r.yp) * r.isy;
     const float dsz = (fz - r.zp) * r.isz;
-    const uint32_t cell = (uint32_t)(r.ic - 1) + (uint32_t)L.im * ((uint32_t)(r.jc - 1) + (uint32_t)L.jm * (uint32_t)(r.kc - 1));
-    const float Tc = gld(L.T, cell);
+    const uint32_t cell = min((uint32_t)(r.ic - 1) + (uint32_t)L.im * ((uint32_t)(r.jc - 1) + (uint32_t)L.jm * (uint32_t)(r.kc - 1)),
+                              (uint32_t)L.ncell - 1u);
+    const float Tc = gld32(L.T, cell);
     float ds = dsx;
     ds = (ds > dsy) ? dsy : ds;
     ds = (ds > dsz) ? dsz : ds;
@@ -198,9 +200,15 @@ __device__ __forceinline__ void march_step(const DevCommon &cm, const S &sn, con
     r.ic += xa ? (r.f0 ? -1 : 1) : 0;
     r.jc += ya ? (r.f1 ? -1 : 1) : 0;
     r.kc += za ? (r.f2 ? -1 : 1) : 0;
-    const bool out = (uint32_t)(r.ic - 1) >= (uint32_t)L.im || (uint32_t)(r.jc - 1) >= (uint32_t)L.jm ||
-                     (uint32_t)(r.kc - 1) >= (uint32_t)L.km;
-    if (out) boundary<TM>(cm, sn, L, r, e, de, status);
+    return (uint32_t)(r.ic - 1) >= (uint32_t)L.im || (uint32_t)(r.jc - 1) >= (uint32_t)L.jm ||
+           (uint32_t)(r.kc - 1) >= (uint32_t)L.km;
+}
+
+/* march_core, then the boundary condition of the face crossed, if any */
+template <int TM = TAB_RT, class S>
+__device__ __forceinline__ void march_step(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r,
+                                           const En &e, float tol, float &de, int &status) {
+    if (march_core<TM>(cm, sn, L, r, e, tol, de, status)) boundary<TM>(cm, sn, L, r, e, de, status);
 }
 
 /* one grid level of the march (march_ray, mc_gpu_noint.cu:245-658);

@@ -132,7 +132,8 @@ int check_params(const rc_params &p) {
     if ((int64_t)p.nB * p.nQ > 65535) return fail("nB*nQ must fit 16 bits");
     if (p.grey && p.nQ != 1) return fail("grey runs use nQ=1");
     if (p.grey && p.grey_true && p.nB != 1) return fail("true grey uses nB=1");
-    if (!(p.toll > 0) || !(p.tollS > 0)) return fail("toll and tollS must be positive");
+    if (!(p.toll > 0) || !(p.tollS > 0) || !(p.toll < 1) || !(p.tollS < 1))
+        return fail("toll and tollS must lie in (0, 1)");
     if (p.calcwalls == 1 && !(p.adiabw == 1 || p.adiabe == 1)) return fail("calcwalls needs an adiabatic x wall");
     return 0;
 }
