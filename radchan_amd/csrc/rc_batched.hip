@@ -51,10 +51,24 @@ __device__ __forceinline__ void lds_sort(K *lds, int n2) {
         }
 }
 
-/* bitonic sort of 256 keys held in registers, element r*64 + lane in x[r]:
- * stages whose partner is in another lane exchange through a lane shuffle,
- * stages whose partner is in another register of the same lane use min/max
- * directly; no LDS traffic and no barrier */
+/* the value of lane (lane ^ m) of a wavefront: quad permutes for m = 1, 2,
+ * LDS-crossbar swizzles within 32 lanes for m = 4, 8, 16, a permute for 32
+ * (m is a constant once the caller's loops are unrolled) */
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v, int m) {
+    switch (m) {
+        case 1: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+        case 2: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);
+        case 4: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x101F);
+        case 8: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x201F);
+        case 16: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F);
+        default: return (uint32_t)__shfl_xor((int)v, m);
+    }
+}
+
+/* bitonic sort of 256 keys held in registers, element 4 lane + r in x[r]:
+ * compare-exchanges at distance 1 and 2 stay inside a lane; the 21 stages
+ * at distance 4..128 exchange with lane (lane ^ distance/4).  No LDS
+ * traffic and no barrier. */
 __device__ __forceinline__ void wave_sort256(uint32_t (&x)[4], int lane) {
     /* the lane index as an opaque value, so that the per-stage lane masks
      * are formed here instead of being hoisted out of the caller's loop and
@@ -64,25 +78,24 @@ __device__ __forceinline__ void wave_sort256(uint32_t (&x)[4], int lane) {
     for (int k = 2; k <= 256; k <<= 1) {
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j >= 64) {
+            if (j < 4) {
+                const bool upl = ((lane * 4) & k) == 0;
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const int p = r ^ (j >> 6);
+                    const int p = r ^ j;
                     if (p > r) {
-                        const bool up = ((r * 64) & k) == 0;
+                        const bool up = (k < 4) ? ((r & k) == 0) : upl;
                         const uint32_t lo = min(x[r], x[p]), hi = max(x[r], x[p]);
                         x[r] = up ? lo : hi;
                         x[p] = up ? hi : lo;
                     }
                 }
             } else {
+                const bool keep_min = (((lane * 4) & j) == 0) == (((lane * 4) & k) == 0);
 #pragma unroll
                 for (int r = 0; r < 4; ++r) {
-                    const uint32_t y = (uint32_t)__shfl_xor((int)x[r], j);
-                    /* all ones where this lane keeps the larger key */
-                    const uint32_t big = 0u - ((uint32_t)(((r * 64 + lane) & k) != 0) ^ (uint32_t)((lane & j) != 0));
-                    const uint32_t mn = min(x[r], y), mx = max(x[r], y);
-                    x[r] = (mn & ~big) | (mx & big);
+                    const uint32_t y = lane_xor(x[r], j >> 2);
+                    x[r] = keep_min ? min(x[r], y) : max(x[r], y);
                 }
             }
         }
@@ -251,12 +264,27 @@ __device__ __forceinline__ void q_get(const uint32_t *q, int qs, const Slots &sl
     if (SURF) surface_adjust(sn, r);
 }
 
+/* a ray's end: its tally and count go to its unit's slot, or to HBM once
+ * the slot has been taken over (seq: units the wavefront has started) */
+__device__ __forceinline__ void finish_ray(const Slots &sl, int seq, int runit, int rtid, float dacc,
+                                           double *__restrict__ de) {
+    if (seq - runit <= NSLOT) {
+        SlotTally &t = sl.tal[runit & (NSLOT - 1)];
+        atomicAdd(&t.acc, (double)dacc);
+        atomicAdd(&t.done, 1);
+    } else {
+        atomicAdd(&de[rtid], (double)dacc);
+    }
+}
+
 /* LDS words of one wavefront: unit slots, ray queue, (multigrid) the grid
  * levels of the current snapshot, then (srt) the direction uniforms of the
- * unit's rays and its sorted keys (16-bit: keys are below nB nQ <= 65535) */
+ * unit's rays and its sorted keys (16-bit: keys are below nB nQ <= 65535;
+ * at least 256, the register sort's width) */
 constexpr int LEV_WORDS = (int)(sizeof(DevLevel) / 4);
 __host__ __device__ __forceinline__ int batched_lds_words(int n2, int srt, int multi) {
-    return SLOT_WORDS * NSLOT + QREC * QCAP + (multi ? LEV_WORDS * RC_MAXLEV : 0) + (srt ? 2 * n2 + (n2 + 1) / 2 : 0);
+    return SLOT_WORDS * NSLOT + QREC * QCAP + (multi ? LEV_WORDS * RC_MAXLEV : 0) +
+           (srt ? 2 * n2 + (n2 < 256 ? 256 : n2) / 2 : 0);
 }
 
 template <bool SURF, bool MULTI, int TM>
@@ -277,6 +305,10 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
     const int per_snap = SURF ? 2 * cm.nsurf : cm.ncell;
     const int total = per_snap * nsnap * cm.nVar;
     const float tol = SURF ? cm.tollS : cm.toll;
+    /* the drain march reads the x faces of every level from LDS */
+    int nxf = 0;
+    for (int g = 0; g < (MULTI ? cm.grid_num : 1); ++g) nxf += snaps[0].lev[g].im + 1;
+    const bool xfit = uni(nxf) <= QREC * QCAP;
 
     __syncthreads();
 
@@ -347,16 +379,15 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                                 dira[h] = q.a;
                                 dirb[h] = q.b;
                             }
-                            if (h < n2) keys[h] = (uint16_t)key;
+                            keys[h] = (uint16_t)key;
                         }
                         __syncthreads();
-                        uint32_t x[4];
-#pragma unroll
-                        for (int q4 = 0; q4 < 4; ++q4) x[q4] = (q4 * 64 + lane < n2) ? (uint32_t)keys[q4 * 64 + lane] : 0xFFFFu;
+                        /* keys 4 lane .. 4 lane + 3: one 8-byte LDS word */
+                        uint2 *kw = reinterpret_cast<uint2 *>(keys) + lane;
+                        const uint2 w = *kw;
+                        uint32_t x[4] = {w.x & 0xFFFFu, w.x >> 16, w.y & 0xFFFFu, w.y >> 16};
                         wave_sort256(x, lane);
-#pragma unroll
-                        for (int q4 = 0; q4 < 4; ++q4)
-                            if (q4 * 64 + lane < n2) keys[q4 * 64 + lane] = (uint16_t)x[q4];
+                        *kw = make_uint2(x[0] | (x[1] << 16), x[2] | (x[3] << 16));
                         __syncthreads();
                     } else {
                         for (int h = lane; h < n2; h += 64) {
@@ -463,9 +494,51 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                 qhead = qwrap(qhead + npop);
                 qcount -= npop;
             }
-            if (__ballot(active) == 0ull) {
-                if (qdone && nxt < 0 && cur_h >= P && qcount == 0) break;
-            } else {
+            const bool drained = qdone && nxt < 0 && cur_h >= P && qcount == 0;
+            if (drained && (__ballot(active) == 0ull || xfit)) {
+                /* (f) no ray left to start: every lane marches its ray to
+                 * the end with the look-ahead march (the rays still live
+                 * here are the longest ones), then the wavefront is done */
+                if (__ballot(active) != 0ull) {
+                    const SnapView sv = snap_view(snaps + snap);
+                    DevLevel l0;
+                    if (!MULTI) l0 = level_view(&snaps[snap].lev[0]);
+                    /* the x faces of every level to LDS (the ray queue's
+                     * space, empty now), level g from xoff(g) */
+                    float *xul = reinterpret_cast<float *>(rq);
+                    int off = 0;
+                    for (int g = 0; g < (MULTI ? cm.grid_num : 1); ++g) {
+                        const DevLevel Lg = MULTI ? llev[g] : l0;
+                        for (int w = lane; w <= Lg.im; w += 64) xul[off + w] = gld(Lg.xu, w);
+                        off += Lg.im + 1;
+                    }
+                    __syncthreads();
+                    if (active) {
+                        /* the lane's ray may be on a coarser level already */
+                        int xoff = 0;
+                        if (MULTI)
+                            for (int g = 0; g < lev; ++g) xoff += llev[g].im + 1;
+                        for (;;) {
+                            const DevLevel L = MULTI ? llev[lev] : l0;
+                            march_ahead<TM, true>(cm, sv, L, r, e, tol, dacc, counter, nsteps, xul + xoff);
+                            if (!MULTI || !(r.tra > tol) || lev == cm.grid_num - 1) break;
+                            const DevLevel Nl = llev[lev + 1];
+                            const int ri = L.im / Nl.im, rj = L.jm / Nl.jm, rk = L.km / Nl.km;
+                            r.ic = (r.ic + ri - 1) / ri;
+                            r.jc = (r.jc + rj - 1) / rj;
+                            r.kc = (r.kc + rk - 1) / rk;
+                            xoff += L.im + 1;
+                            lev += 1;
+                            counter = 0;
+                        }
+                        finish_ray(sl, seq, runit, rtid, dacc, de);
+                        nrays += 1;
+                        active = false;
+                    }
+                }
+                break;
+            }
+            if (__ballot(active) != 0ull) {
                 /* (e) one march step of every active lane; the snapshot's
                  * fields (and the level, single grid) are wavefront-uniform
                  * values, a lane's level comes from LDS (multigrid) */
@@ -488,15 +561,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                 const bool fin = done && (!MULTI || !alive || lev == cm.grid_num - 1);
                 const bool crs = MULTI && done && !fin;
                 if (fin) {
-                    /* the ray ends: its tally and count go to its unit's
-                     * slot, or to HBM once the slot has been taken over */
-                    if (seq - runit <= NSLOT) {
-                        SlotTally &t = sl.tal[runit & (NSLOT - 1)];
-                        atomicAdd(&t.acc, (double)dacc);
-                        atomicAdd(&t.done, 1);
-                    } else {
-                        atomicAdd(&de[rtid], (double)dacc);
-                    }
+                    finish_ray(sl, seq, runit, rtid, dacc, de);
                     nrays += 1;
                     active = false;
                 }
@@ -530,17 +595,17 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
     }
 }
 
-/* single grid: registers and LDS sized for 5 wavefronts per SIMD */
+/* single grid */
 template <bool SURF, int TM>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8)))
+__global__ void __launch_bounds__(64)
 k_batched(DevCommon cm, const DevSnap *__restrict__ snaps, int nsnap, double *__restrict__ de,
           unsigned long long *__restrict__ stats, int *__restrict__ queue, int threshold) {
     batched_body<SURF, false, TM>(cm, snaps, nsnap, de, stats, queue, threshold);
 }
 
-/* multigrid */
+/* multigrid (registers held to 4 wavefronts per SIMD) */
 template <bool SURF, int TM>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_batched_mg(DevCommon cm, const DevSnap *__restrict__ snaps, int nsnap, double *__restrict__ de,
              unsigned long long *__restrict__ stats, int *__restrict__ queue, int threshold) {
     batched_body<SURF, true, TM>(cm, snaps, nsnap, de, stats, queue, threshold);

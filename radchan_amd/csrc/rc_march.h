@@ -84,6 +84,44 @@ __device__ __forceinline__ void kb_at(const S &sn, uint32_t row, int rank, int n
     }
 }
 
+/* kb_at in two halves, for a fixed storage mode: the loads of the two
+ * table rows t, t+1 (issued at once), and the interpolation (when the
+ * values are needed); finish(fetch(...)) computes exactly kb_at */
+struct KbRaw {
+    float4 q;        /* FP32: {k_t, Ib_t, k_t+1, Ib_t+1}; BF16: {Ib_t, Ib_t+1, scale, 0} */
+    uint32_t w;      /* BF16: k_t | k_t+1 << 16 */
+    float a;         /* interpolation weight */
+};
+
+template <int TM, class S>
+__device__ __forceinline__ KbRaw kb_fetch(const S &sn, uint32_t row, int rank, int nT, float T) {
+    static_assert(TM == TAB_F32 || TM == TAB_BF16, "fixed storage mode");
+    KbRaw k;
+    int t;
+    tpos(T, sn.Tnb0, sn.invdT, nT, t, k.a);
+    if (TM == TAB_BF16) {
+        k.w = gld32(sn.kp16, row * (uint32_t)nT + (uint32_t)t);
+        k.q = gld32(sn.bp, (uint32_t)rank * (uint32_t)nT + (uint32_t)t);
+    } else {
+        k.w = 0u;
+        k.q = gld32_pair(sn.kb, row * (uint32_t)(nT + 1) + (uint32_t)t);
+    }
+    return k;
+}
+
+template <int TM>
+__device__ __forceinline__ void kb_finish(const KbRaw &k, float &K, float &B) {
+    if (TM == TAB_BF16) {
+        const float k0 = __uint_as_float(k.w << 16) * k.q.z;
+        const float k1 = __uint_as_float(k.w & 0xFFFF0000u) * k.q.z;
+        K = lerp(k0, k1, k.a);
+        B = lerp(k.q.x, k.q.y, k.a);
+    } else {
+        K = lerp(k.q.x, k.q.z, k.a);
+        B = lerp(k.q.y, k.q.w, k.a);
+    }
+}
+
 __device__ __forceinline__ void set_dir(Ray &r, float sx, float sy, float sz) {
     r.sx = sx; r.sy = sy; r.sz = sz;
     r.f0 = sx < 0.f; r.f1 = sy < 0.f; r.f2 = sz < 0.f;
@@ -96,13 +134,17 @@ __device__ __forceinline__ void wall_hit(const En &e, Ray &r, float Bw, float &d
     r.tra = 0.f;
 }
 
+/* outcome of the boundary condition of a face */
+enum { RB_ON = 0, RB_WALL = 1, RB_LOST = 2 };
+
 /* boundary condition of the face the ray just crossed (the one index that
- * left 1..n): periodic, black (reference wall, adiabatic midpoint or fixed
- * temperature), specular, black at the adjacent-cell temperature (z), lost
- * (z) -- mc_gpu_noint.cu:375-654 */
+ * left 1..n), without its tally: periodic and specular faces move the ray
+ * on (RB_ON); a black face -- reference wall, adiabatic midpoint or fixed
+ * temperature -- or a bd 4 z face ends it with the wall's Ib in Bw
+ * (RB_WALL); a bd 5 face loses it (RB_LOST) -- mc_gpu_noint.cu:375-654 */
 template <int TM = TAB_RT, class S>
-__device__ __forceinline__ void boundary(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r, const En &e,
-                                      float &de, int &status) {
+__device__ __forceinline__ int boundary_move(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r,
+                                             const En &e, float &Bw) {
     const int nT = cm.nT;
     int face;
     if (r.ic == 0) face = FW;
@@ -121,8 +163,8 @@ __device__ __forceinline__ void boundary(const DevCommon &cm, const S &sn, const
             case FB: r.kc = L.km; r.zp = cm.Lz; break;
             default: r.kc = 1; r.zp = 0.f; break;
         }
+        return RB_ON;
     } else if (bd == 2) {
-        float Bw;
         if ((face == FW && cm.adiab[0]) || (face == FE && cm.adiab[1])) {
             const float Tw = gld(L.Tw, (size_t)(face == FE) * L.jm * L.km + (size_t)(r.jc - 1) + (size_t)L.jm * (r.kc - 1));
             float Kw;
@@ -130,8 +172,7 @@ __device__ __forceinline__ void boundary(const DevCommon &cm, const S &sn, const
         } else {
             Bw = gld(sn.bfix, (size_t)face * cm.nB + e.rank);
         }
-        wall_hit(e, r, Bw, de);
-        status = RC_RAY_WALL;
+        return RB_WALL;
     } else if (bd == 3) {
         switch (face) {
             case FW: r.xp = 0.f; r.sx = -r.sx; r.isx = -r.isx; r.ic = 1; r.f0 = r.sx < 0.f; break;
@@ -141,59 +182,60 @@ __device__ __forceinline__ void boundary(const DevCommon &cm, const S &sn, const
             case FB: r.zp = 0.f; r.sz = -r.sz; r.isz = -r.isz; r.kc = 1; r.f2 = r.sz < 0.f; break;
             default: r.zp = cm.Lz; r.sz = -r.sz; r.isz = -r.isz; r.kc = L.km; r.f2 = r.sz < 0.f; break;
         }
+        return RB_ON;
     } else if (bd == 4) {
         const float Tw = gld(L.Tz, (size_t)(face == FT) * L.im * L.jm + (size_t)(r.ic - 1) + (size_t)L.im * (r.jc - 1));
-        float Kw, Bw;
+        float Kw;
         kb_at<TM>(sn, e.row, e.rank, nT, Tw, Kw, Bw);
+        return RB_WALL;
+    }
+    return RB_LOST;
+}
+
+/* the boundary condition with its tally: the wall emission tally ends the
+ * ray at a wall; a lost ray ends with nothing */
+template <int TM = TAB_RT, class S>
+__device__ __forceinline__ void boundary(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r, const En &e,
+                                      float &de, int &status) {
+    float Bw = 0.f;
+    const int b = boundary_move<TM>(cm, sn, L, r, e, Bw);
+    if (b == RB_WALL) {
         wall_hit(e, r, Bw, de);
         status = RC_RAY_WALL;
-    } else {
+    } else if (b == RB_LOST) {
         r.tra = 0.f;
         status = RC_RAY_LOST;
     }
 }
 
-/* one march step (one iteration of march_ray's loop, mc_gpu_noint.cu:252-654):
- * distances to the three candidate faces, position update, attenuation and
- * reciprocity tally in the current cell, transmissivity cut-off, then one
- * index move.  Written with selects, no branch; returns whether an index
- * left the grid (the boundary condition is then due). */
-template <int TM = TAB_RT, class S>
-__device__ __forceinline__ bool march_core(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r,
-                                           const En &e, float tol, float &de, int &status) {
-    const int nT = cm.nT;
+/* The geometry of one march step (one iteration of march_ray's loop,
+ * mc_gpu_noint.cu:252-654): distances to the three candidate faces, the
+ * segment length ds, the position update and the index move along the
+ * crossed axis; `cell` is the cell the segment crosses.  Nothing here
+ * depends on the temperature.  Returns whether an index left the grid
+ * (the boundary condition is then due). */
+template <bool XL = false>
+__device__ __forceinline__ bool geo_core(const DevLevel &L, Ray &r, uint32_t &cell, float &ds,
+                                         const float *xul = nullptr) {
     /* both indices are in range for a ray inside the grid; the bounds only
      * matter to a lane whose state is stale (the batched kernel steps idle
-     * lanes too and discards what they compute) */
-    const float fx = gld32(L.xu, min((uint32_t)(r.ic - r.f0), (uint32_t)L.im));
+     * lanes too and discards what they compute).  With XL the x faces are
+     * read from an LDS copy. */
+    const uint32_t ix = min((uint32_t)(r.ic - r.f0), (uint32_t)L.im);
+    const float fx = XL ? xul[ix] : gld32(L.xu, ix);
     const float fy = (float)(r.jc - r.f1) * L.dy;
     const float fz = (float)(r.kc - r.f2) * L.dz;
     const float dsx = (fx - r.xp) * r.isx;
     const float dsy = (fy - r.yp) * r.isy;
     const float dsz = (fz - r.zp) * r.isz;
-    const uint32_t cell = min((uint32_t)(r.ic - 1) + (uint32_t)L.im * ((uint32_t)(r.jc - 1) + (uint32_t)L.jm * (uint32_t)(r.kc - 1)),
-                              (uint32_t)L.ncell - 1u);
-    const float Tc = gld32(L.T, cell);
-    float ds = dsx;
+    cell = min((uint32_t)(r.ic - 1) + (uint32_t)L.im * ((uint32_t)(r.jc - 1) + (uint32_t)L.jm * (uint32_t)(r.kc - 1)),
+               (uint32_t)L.ncell - 1u);
+    ds = dsx;
     ds = (ds > dsy) ? dsy : ds;
     ds = (ds > dsz) ? dsz : ds;
     r.xp = (ds == dsx) ? fx : fmaf(ds, r.sx, r.xp);
     r.yp = (ds == dsy) ? fy : fmaf(ds, r.sy, r.yp);
     r.zp = (ds == dsz) ? fz : fmaf(ds, r.sz, r.zp);
-    float K, B;
-    kb_at<TM>(sn, e.row, e.rank, nT, Tc, K, B);
-    float em1, ex;
-    expm1_exp(-(K * ds), em1, ex);
-    const float alpha = -em1;
-    const float fac = (B - e.Bi) * e.invBi;
-    de -= ((e.W * r.tra) * alpha) * fac;
-    const float tra = r.tra * ex;
-    const bool dump = tra < tol;
-    const float dd = (e.W * tra) * fac;
-    de = dump ? de - dd : de;
-    r.tra = dump ? 0.f : tra;
-    status = dump ? RC_RAY_ABSORBED : status;
-    /* one index moves along the crossed axis */
     const bool xa = dsx < dsy && dsx < dsz;
     const bool ya = !(dsx < dsy) && dsy < dsz;
     const bool za = !xa && !ya;
@@ -202,6 +244,103 @@ __device__ __forceinline__ bool march_core(const DevCommon &cm, const S &sn, con
     r.kc += za ? (r.f2 ? -1 : 1) : 0;
     return (uint32_t)(r.ic - 1) >= (uint32_t)L.im || (uint32_t)(r.jc - 1) >= (uint32_t)L.jm ||
            (uint32_t)(r.kc - 1) >= (uint32_t)L.km;
+}
+
+/* The absorption of one step given the crossed cell's k and Ib: attenuation
+ * and reciprocity tally, transmissivity cut-off (the dump tally when it
+ * falls below tol). */
+__device__ __forceinline__ void phys_core(float K, float B, float ds, const En &e, float tol, float &tra,
+                                          float &de, int &status) {
+    float em1, ex;
+    expm1_exp(-(K * ds), em1, ex);
+    const float alpha = -em1;
+    const float fac = (B - e.Bi) * e.invBi;
+    de -= ((e.W * tra) * alpha) * fac;
+    const float tn = tra * ex;
+    const bool dump = tn < tol;
+    const float dd = (e.W * tn) * fac;
+    de = dump ? de - dd : de;
+    tra = dump ? 0.f : tn;
+    status = dump ? RC_RAY_ABSORBED : status;
+}
+
+/* one march step: geometry, then the absorption at the crossed cell's
+ * temperature (table interpolation, FP32).  Select-based, no branch;
+ * returns whether an index left the grid. */
+template <int TM = TAB_RT, class S>
+__device__ __forceinline__ bool march_core(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r,
+                                           const En &e, float tol, float &de, int &status) {
+    uint32_t cell;
+    float ds;
+    const bool out = geo_core(L, r, cell, ds);
+    float K, B;
+    kb_at<TM>(sn, e.row, e.rank, cm.nT, gld32(L.T, cell), K, B);
+    phys_core(K, B, ds, e, tol, r.tra, de, status);
+    return out;
+}
+
+/* A ray marched on one grid level to its end, or to the level's step cap,
+ * with the geometry two steps ahead of the absorption: the temperature of
+ * step i+2 and the table entries of step i+1 are loaded while step i is
+ * absorbed.  The arithmetic of every step is that of march_step, so the
+ * tally, transmissivity and step count are bit-identical; only a dumped
+ * ray's final position differs (the geometry ran ahead of it).  Expects a
+ * live ray under the cap (counter < L.maxs).  With XL the level's x faces
+ * come from an LDS copy (xul), so that the geometry never waits on the
+ * vector-memory counter behind the prefetched loads. */
+template <int TM = TAB_RT, bool XL = false, class S>
+__device__ __forceinline__ void march_ahead(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r,
+                                            const En &e, float tol, float &de, int &counter, int &nsteps,
+                                            const float *xul = nullptr) {
+    const int nT = cm.nT;
+    int status = 0;
+    /* step i */
+    uint32_t c0;
+    float d0, w0;
+    int b0 = RB_ON;
+    if (geo_core<XL>(L, r, c0, d0, xul)) b0 = boundary_move<TM>(cm, sn, L, r, e, w0);
+    KbRaw k0 = kb_fetch<TM>(sn, e.row, e.rank, nT, gld32(L.T, c0));
+    /* step i+1 */
+    /* The loads are issued on every path (cell 0 for a step not taken,
+     * whose values are then never used), so that the number of loads in
+     * flight does not depend on the path and the wait for the oldest one
+     * does not have to drain the newer ones. */
+    bool h1 = b0 == RB_ON && counter + 1 < L.maxs;
+    uint32_t c1 = 0u;
+    float d1, w1;
+    int b1 = RB_ON;
+    if (h1)
+        if (geo_core<XL>(L, r, c1, d1, xul)) b1 = boundary_move<TM>(cm, sn, L, r, e, w1);
+    float T1 = gld32(L.T, c1);
+    for (;;) {
+        /* step i+2: geometry and temperature */
+        const bool h2 = h1 && b1 == RB_ON && counter + 2 < L.maxs;
+        uint32_t c2 = 0u;
+        float d2, w2;
+        int b2 = RB_ON;
+        if (h2)
+            if (geo_core<XL>(L, r, c2, d2, xul)) b2 = boundary_move<TM>(cm, sn, L, r, e, w2);
+        const float T2 = gld32(L.T, c2);
+        /* step i+1: table entries */
+        const KbRaw k1 = kb_fetch<TM>(sn, e.row, e.rank, nT, T1);
+        /* step i: absorption, then the end of the ray, if any */
+        float K0, B0;
+        kb_finish<TM>(k0, K0, B0);
+        phys_core(K0, B0, d0, e, tol, r.tra, de, status);
+        counter += 1;
+        nsteps += 1;
+        if (b0 == RB_WALL) {
+            wall_hit(e, r, w0, de);
+            break;
+        }
+        if (b0 == RB_LOST) {
+            r.tra = 0.f;
+            break;
+        }
+        if (!(r.tra > tol) || !h1) break;
+        k0 = k1; d0 = d1; w0 = w1; b0 = b1;
+        h1 = h2; d1 = d2; w1 = w2; b1 = b2; T1 = T2;
+    }
 }
 
 /* march_core, then the boundary condition of the face crossed, if any */
