@@ -13,8 +13,8 @@ Legs: an untimed autotune of the batched schedule; the batched solver with
 FP32 and with BF16 k-tables, each replayed as a hipGraph; the batched march
 kernel alone (HIP events, no graph) for the roofline; the plain solver.
 
-Printed (one JSON line on stdout): rays/s of the batched solver (BF16
-k-tables unless --tables fp32), the plain solver's rays/s beside it, the
+Printed (one JSON line on stdout): rays/s of the batched solver (FP32
+k-tables unless --tables bf16), the plain solver's rays/s beside it, the
 measured std-error eps of div(q_r) and the projected time to eps = 1% for
 every leg, the roofline of the dominant kernel and a CPU baseline (the C++
 oracle in FP32 on the host cores, a bounded sample).
@@ -108,8 +108,8 @@ def main() -> int:
     ap.add_argument("--warmup", type=int, default=2)
     ap.add_argument("--workload", default="nb64", choices=["nb64", "nb128", "big256", "grey32", "mg192"])
     ap.add_argument("--nvar", type=int, default=8)
-    ap.add_argument("--tables", default="bf16", choices=["bf16", "fp32"],
-                    help="k-table storage of the headline (batched) leg")
+    ap.add_argument("--tables", default="fp32", choices=["bf16", "fp32"],
+                    help="k-table storage of the headline (batched) leg; both are measured")
     ap.add_argument("--plain-steps", type=int, default=3)
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
@@ -211,7 +211,8 @@ def main() -> int:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_batched<false,false> (persistent march)", "kernel_ms": round(march_ms, 4),
+            "kernel": "k_batched<false,%d> (persistent march, %s k-tables)" % (1 if bf16 else 0, "BF16" if bf16 else "FP32"),
+            "kernel_ms": round(march_ms, 4),
             "algorithmic_bytes": int(alg_bytes),
             "bytes_per_step": b_step, "bytes_per_ray": b_ray, "bytes_per_cell": B_CELL}
 

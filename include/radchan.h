@@ -180,10 +180,12 @@ int rc_trace(rc_ctx *c, int surface, const int32_t *rays, int n, rc_ray *out);
  * Ibmax_e; ints tm, tstar */
 int rc_info(rc_ctx *c, int s, float *f5, int32_t *i2);
 /* schedule of the batched solver: persistent workgroups (one wavefront
- * each) per compute unit, and the number of idle lanes that triggers a
- * refill; 0 selects the default.  Chosen by the autotuner. */
-int rc_set_schedule(rc_ctx *c, int wg_per_cu, int threshold);
-int rc_get_schedule(rc_ctx *c, int32_t *wg_per_cu, int32_t *threshold);
+ * each) per compute unit, the number of idle lanes that triggers a
+ * refill, and march steps per loop iteration (rays that end on an earlier
+ * step wait for the next refill); 0 selects the default.  Chosen by the
+ * autotuner. */
+int rc_set_schedule(rc_ctx *c, int wg_per_cu, int threshold, int steps_per_iter);
+int rc_get_schedule(rc_ctx *c, int32_t *wg_per_cu, int32_t *threshold, int32_t *steps_per_iter);
 /* storage of the k tables on the device: 0 = FP32 (default), 1 = BF16 k
  * pairs with a power-of-two scale per band (Ib stays FP32); rebuilds the
  * loaded tables */
@@ -192,9 +194,10 @@ int rc_set_table_precision(rc_ctx *c, int bf16);
  * finalize) once as a hipGraph and replay it on later rc_run calls; the
  * graph is dropped when fields or the schedule change */
 int rc_set_graph(rc_ctx *c, int on);
-/* time the batched solve of the loaded snapshots over a grid of schedules
- * (workgroups per CU x refill threshold) and keep the fastest */
-int rc_autotune(rc_ctx *c, int32_t *best_wg, int32_t *best_th, float *best_ms);
+/* time the batched solve of the loaded snapshots over schedules (steps per
+ * iteration, then workgroups per CU, then refill threshold, each at the
+ * best of the previous) and keep the fastest */
+int rc_autotune(rc_ctx *c, int32_t *best_wg, int32_t *best_th, int32_t *best_spi, float *best_ms);
 int rc_device_synchronize(void);
 
 #ifdef __cplusplus

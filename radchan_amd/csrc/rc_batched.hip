@@ -181,6 +181,10 @@ __device__ __forceinline__ Unit uni(const Unit &a) {
     return b;
 }
 
+/* device unit queue: counters of the QPARTS ranges (fluid at queue[0..],
+ * walls at queue[16..]) */
+constexpr int QPARTS = 8;
+
 /* LDS ray queue: rays generated in bulk (all 64 lanes) wait here until an
  * idle lane picks them up.  QCAP records of 12 words per
  * wavefront, written and read as three 16-byte words: direction,
@@ -290,7 +294,7 @@ __host__ __device__ __forceinline__ int batched_lds_words(int n2, int srt, int m
 template <bool SURF, bool MULTI, int TM>
 __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap *__restrict__ snaps, int nsnap,
                                              double *__restrict__ de, unsigned long long *__restrict__ stats,
-                                             int *__restrict__ queue, int threshold) {
+                                             int *__restrict__ queue, int threshold, int spi) {
     extern __shared__ uint32_t lds[];
     const Slots sl = slot_view(lds);
     uint32_t *rq = lds + SLOT_WORDS * NSLOT;
@@ -316,7 +320,9 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
     int cur_h = P, nxt = -1, nxt_s = -1, qdone = 0, snap = -1, seq = 0, cunit = 0, ctid = 0;
     int qhead = 0, qcount = 0;
     const int chunk = min(max(cm.nVar, 1), 16);
+    const int nch = (total + chunk - 1) / chunk;
     int cnext = 0, cleft = 0;
+    int part = (int)(blockIdx.x % QPARTS), ptries = 0;   /* workgroups are dealt round-robin over the XCDs */
     Unit U{};
 
     /* lane state; an idle lane keeps in-range values, since the march step
@@ -337,27 +343,40 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
     e.invBi = 1.f;
     float dacc = 0.f;
     int runit = 0, rtid = 0, lev = 0, counter = 0;
+    int kt = -1;                  /* the lane's cached table pair (t, values) */
+    KbRaw kc;
+    kc.q = make_float4(0.f, 0.f, 0.f, 0.f);
+    kc.w = 0u;
+    kc.a = 0.f;
     int nsteps = 0, nrays = 0;
 
-    /* one back-edge: every iteration either starts a unit or generates,
-     * refills and steps */
     for (;;) {
         /* (a) once the current unit is fully generated, take the next unit
          * and (srt) draw and sort its band keys, keeping the direction
          * uniforms of every ray */
         if (nxt < 0 && !qdone && cur_h >= P) {
-            /* units are taken from the device queue in chunks of nVar
-             * (one cell's runs), so that one wavefront writes the cell's
-             * consecutive tallies (whole cache lines in one L2) */
-            if (cleft == 0) {
-                int u0 = 0;
-                if (lane == 0) u0 = atomicAdd(queue, chunk);
-                cnext = uni(u0);
-                cleft = chunk;
+            /* units are taken in chunks of nVar (one cell's runs, so that
+             * one wavefront writes the cell's consecutive tallies).  The
+             * chunks are split into QPARTS contiguous ranges (z slabs of
+             * the field), one per XCD, each with its own counter, so that
+             * an XCD's L2 holds its own slab; a range that runs out sends
+             * the wavefront to the next one. */
+            while (cleft == 0 && part >= 0) {
+                const int pb = (int)((long)nch * part / QPARTS), pe = (int)((long)nch * (part + 1) / QPARTS);
+                int c0 = 0;
+                if (lane == 0) c0 = atomicAdd(&queue[part], 1);
+                c0 = uni(c0);
+                if (pb + c0 < pe) {
+                    cnext = (pb + c0) * chunk;
+                    cleft = min(chunk, total - cnext);
+                } else {
+                    part = (part + 1) % QPARTS;
+                    if (++ptries >= QPARTS) part = -1;
+                }
             }
-            const int u = cnext;
+            const int u = cleft > 0 ? cnext : total;
             cnext += 1;
-            cleft -= 1;
+            cleft -= (cleft > 0) ? 1 : 0;
             if (u >= total) {
                 qdone = 1;
             } else {
@@ -370,8 +389,9 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                     if (n2 <= 256) {
                         /* up to 256 keys: drawn one round of 64 at a time,
                          * then sorted in registers */
+                        /* rounds of 64 draws up to photmax, then padding */
 #pragma unroll 1
-                        for (int h = lane; h < 256; h += 64) {
+                        for (int h = lane; h < ((P + 63) & ~63); h += 64) {
                             uint32_t key = 0xFFFFFFFFu;
                             if (h < P) {
                                 const U4 q = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)N.v, N.cid, tagx);
@@ -381,6 +401,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                             }
                             keys[h] = (uint16_t)key;
                         }
+                        for (int h = ((P + 63) & ~63) + lane; h < 256; h += 64) keys[h] = (uint16_t)0xFFFFu;
                         __syncthreads();
                         /* keys 4 lane .. 4 lane + 3: one 8-byte LDS word */
                         uint2 *kw = reinterpret_cast<uint2 *>(keys) + lane;
@@ -447,7 +468,10 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                 seq += 1;
             }
             __syncthreads();
-        } else {
+        }
+        /* one back-edge: a unit may start and generate in the same iteration;
+         * an iteration with no live lane steps idle lanes only */
+        {
             /* (c) bulk generation: every lane sets up one ray of the current unit */
             const int left = P - cur_h;
             if (left > 0 && qcount <= QCAP - 64) {
@@ -490,6 +514,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                     dacc = 0.f;
                     lev = 0;
                     counter = 0;
+                    kt = -1;
                 }
                 qhead = qwrap(qhead + npop);
                 qcount -= npop;
@@ -539,41 +564,44 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                 break;
             }
             if (__ballot(active) != 0ull) {
-                /* (e) one march step of every active lane; the snapshot's
+                /* (e) march steps of every active lane; the snapshot's
                  * fields (and the level, single grid) are wavefront-uniform
                  * values, a lane's level comes from LDS (multigrid) */
                 const SnapView sv = snap_view(snaps + snap);
                 DevLevel l0;
                 if (!MULTI) l0 = level_view(&snaps[snap].lev[0]);
-                const DevLevel L = MULTI ? llev[lev] : l0;
-                /* Every active lane holds a live ray under its level's step
-                 * cap: a ray is finished (or coarsened) right after the step
-                 * that ends it, so that its lane is refilled in the next
-                 * iteration.  Idle lanes step too, on stale state that the
-                 * next pick-up overwrites; the step bounds its loads. */
-                int status = 0;
-                const bool out = march_core<TM>(cm, sv, L, r, e, tol, dacc, status);
-                if (active && out) boundary<TM>(cm, sv, L, r, e, dacc, status);
-                counter += 1;
-                nsteps += active ? 1 : 0;
-                const bool alive = r.tra > tol;
-                const bool done = active && (!alive || counter >= L.maxs);
-                const bool fin = done && (!MULTI || !alive || lev == cm.grid_num - 1);
-                const bool crs = MULTI && done && !fin;
-                if (fin) {
-                    finish_ray(sl, seq, runit, rtid, dacc, de);
-                    nrays += 1;
-                    active = false;
-                }
-                if (crs) {
-                    /* coarsen the indices for the next grid level (:215-223) */
-                    const DevLevel Nl = llev[lev + 1];
-                    const int ri = L.im / Nl.im, rj = L.jm / Nl.jm, rk = L.km / Nl.km;
-                    r.ic = (r.ic + ri - 1) / ri;
-                    r.jc = (r.jc + rj - 1) / rj;
-                    r.kc = (r.kc + rk - 1) / rk;
-                    lev += 1;
-                    counter = 0;
+                /* spi march steps.  Every active lane holds a live ray under
+                 * its level's step cap: a ray is finished (or coarsened)
+                 * right after the step that ends it.  Idle lanes step too, on
+                 * stale state that the next pick-up overwrites; the step
+                 * bounds its loads. */
+#pragma unroll 1
+                for (int q = 0; q < spi; ++q) {
+                    const DevLevel L = MULTI ? llev[lev] : l0;
+                    int status = 0;
+                    const bool out = march_core_cached<TM>(cm, sv, L, r, e, tol, dacc, status, kt, kc);
+                    if (active && out) boundary<TM>(cm, sv, L, r, e, dacc, status);
+                    counter += 1;
+                    nsteps += active ? 1 : 0;
+                    const bool alive = r.tra > tol;
+                    const bool done = active && (!alive || counter >= L.maxs);
+                    const bool fin = done && (!MULTI || !alive || lev == cm.grid_num - 1);
+                    const bool crs = MULTI && done && !fin;
+                    if (fin) {
+                        finish_ray(sl, seq, runit, rtid, dacc, de);
+                        nrays += 1;
+                        active = false;
+                    }
+                    if (crs) {
+                        /* coarsen the indices for the next grid level (:215-223) */
+                        const DevLevel Nl = llev[lev + 1];
+                        const int ri = L.im / Nl.im, rj = L.jm / Nl.jm, rk = L.km / Nl.km;
+                        r.ic = (r.ic + ri - 1) / ri;
+                        r.jc = (r.jc + rj - 1) / rj;
+                        r.kc = (r.kc + rk - 1) / rk;
+                        lev += 1;
+                        counter = 0;
+                    }
                 }
             }
         }
@@ -599,16 +627,16 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
 template <bool SURF, int TM>
 __global__ void __launch_bounds__(64)
 k_batched(DevCommon cm, const DevSnap *__restrict__ snaps, int nsnap, double *__restrict__ de,
-          unsigned long long *__restrict__ stats, int *__restrict__ queue, int threshold) {
-    batched_body<SURF, false, TM>(cm, snaps, nsnap, de, stats, queue, threshold);
+          unsigned long long *__restrict__ stats, int *__restrict__ queue, int threshold, int spi) {
+    batched_body<SURF, false, TM>(cm, snaps, nsnap, de, stats, queue, threshold, spi);
 }
 
 /* multigrid (registers held to 4 wavefronts per SIMD) */
 template <bool SURF, int TM>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
 k_batched_mg(DevCommon cm, const DevSnap *__restrict__ snaps, int nsnap, double *__restrict__ de,
-             unsigned long long *__restrict__ stats, int *__restrict__ queue, int threshold) {
-    batched_body<SURF, true, TM>(cm, snaps, nsnap, de, stats, queue, threshold);
+             unsigned long long *__restrict__ stats, int *__restrict__ queue, int threshold, int spi) {
+    batched_body<SURF, true, TM>(cm, snaps, nsnap, de, stats, queue, threshold, spi);
 }
 
 __global__ void k_zero_int(int *p, int n) {
@@ -635,7 +663,7 @@ void launch_zero(hipStream_t st, double *p, size_t n) {
 }
 
 void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int nsnap, int surface, int snaps_bf16,
-                    double *de, unsigned long long *stats, int *queue, int wg_per_cu, int threshold) {
+                    double *de, unsigned long long *stats, int *queue, int wg_per_cu, int threshold, int spi) {
     static int ncu = 0;       /* compute units of the (single) device */
     if (ncu <= 0) {
         int dev = 0;
@@ -645,6 +673,7 @@ void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, i
     }
     if (wg_per_cu <= 0) wg_per_cu = 32;
     if (threshold <= 0) threshold = 8;
+    if (spi <= 0) spi = 1;
     int n2 = 1;
     while (n2 < cm.photmax) n2 <<= 1;
     const size_t lds = (size_t)batched_lds_words(n2, cm.srt, cm.grid_num > 1) * 4;
@@ -652,12 +681,12 @@ void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, i
     const size_t nde = (size_t)per_snap * nsnap * cm.nVar;
     const long total = (long)nde;
     const int grid = (int)std::max<long>(1, std::min<long>((long)ncu * wg_per_cu, total));
-    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, queue, 1);
+    hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, queue, QPARTS);
     launch_zero(st, de, nde);
     const bool multi = cm.grid_num > 1;
     const bool bf = snaps_bf16;
 #define RC_LAUNCH(K, S, T) hipLaunchKernelGGL((K<S, T>), dim3(grid), dim3(64), lds, st, cm, snaps, nsnap, de, stats, \
-                                           queue, threshold)
+                                           queue, threshold, spi)
     if (!surface) {
         if (multi) { if (bf) RC_LAUNCH(k_batched_mg, false, TAB_BF16); else RC_LAUNCH(k_batched_mg, false, TAB_F32); }
         else { if (bf) RC_LAUNCH(k_batched, false, TAB_BF16); else RC_LAUNCH(k_batched, false, TAB_F32); }

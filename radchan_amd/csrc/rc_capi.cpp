@@ -24,7 +24,7 @@
 
 namespace rc {
 void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int nsnap, int surface, int snaps_bf16,
-                    double *de, unsigned long long *stats, int *queue, int wg_per_cu, int threshold);
+                    double *de, unsigned long long *stats, int *queue, int wg_per_cu, int threshold, int spi);
 }
 
 using namespace rc;
@@ -119,7 +119,7 @@ struct rc_ctx {
     DevBuf tr_rays, tr_unit, tr_ulist, tr_out;
     rc_stats last{};
     bool launched = false;
-    int wg_per_cu = 0, threshold = 0;       /* batched schedule; 0 = default */
+    int wg_per_cu = 0, threshold = 0, spi = 0;   /* batched schedule; 0 = default */
     bool use_graph = false;                 /* replay the batched run as a hipGraph */
     int table_bf16 = 0;                     /* k tables stored as BF16 */
     int graph_launches = 0;
@@ -549,11 +549,11 @@ static int enqueue_solve(rc_ctx *c, int variant, int *launches, bool timed) {
         }
     } else {
         launch_batched(c->stream, cm, ds, nsnap, 0, c->table_bf16, c->de.as<double>(), c->stats.as<unsigned long long>(),
-                       c->queue.as<int>(), c->wg_per_cu, c->threshold);
+                       c->queue.as<int>(), c->wg_per_cu, c->threshold, c->spi);
         n += 3;
         if (walls) {
             launch_batched(c->stream, cm, ds, nsnap, 1, c->table_bf16, c->ds.as<double>(), c->stats.as<unsigned long long>(),
-                           c->queue.as<int>() + 16, c->wg_per_cu, c->threshold);
+                           c->queue.as<int>() + 16, c->wg_per_cu, c->threshold, c->spi);
             n += 3;
         }
     }
@@ -637,28 +637,38 @@ int rc_set_graph(rc_ctx *c, int on) {
     return 0;
 }
 
-/* try a small grid of schedules on the loaded snapshots (batched variant,
- * device-event timing of the march) and keep the fastest */
-int rc_autotune(rc_ctx *c, int32_t *best_wg, int32_t *best_th, float *best_ms) {
+/* coordinate search over schedules on the loaded snapshots (batched
+ * variant, device-event timing of the march): steps per iteration at the
+ * default occupancy, then workgroups per CU, then the refill threshold */
+int rc_autotune(rc_ctx *c, int32_t *best_wg, int32_t *best_th, int32_t *best_spi, float *best_ms) {
     if (!c || c->snaps.empty()) { set_error("rc_autotune before rc_set_fields"); return -1; }
+    static const int spis[] = {1, 2, 3, 4};
     static const int wgs[] = {8, 12, 16, 20, 24, 32, 40};
     static const int ths[] = {4, 8, 16, 32};
-    const int keep_wg = c->wg_per_cu, keep_th = c->threshold;
+    const int keep_wg = c->wg_per_cu, keep_th = c->threshold, keep_spi = c->spi;
     float best = 1e30f;
-    int bw = keep_wg, bt = keep_th;
-    for (int wg : wgs)
-        for (int th : ths) {
-            c->wg_per_cu = wg; c->threshold = th;
-            c->drop_graph();
-            if (int e = rc_run(c, RC_VARIANT_BATCHED)) { c->wg_per_cu = keep_wg; c->threshold = keep_th; return e; }
-            rc_stats st;
-            if (int e = rc_sync(c, &st)) return e;
-            if ((float)st.march_ms < best) { best = (float)st.march_ms; bw = wg; bt = th; }
-        }
-    c->wg_per_cu = bw; c->threshold = bt;
+    int bw = 32, bt = 8, bs = 1;
+    auto trial = [&](int wg, int th, int sp) -> int {
+        c->wg_per_cu = wg; c->threshold = th; c->spi = sp;
+        c->drop_graph();
+        if (int e = rc_run(c, RC_VARIANT_BATCHED)) return e;
+        rc_stats st;
+        if (int e = rc_sync(c, &st)) return e;
+        if ((float)st.march_ms < best) { best = (float)st.march_ms; bw = wg; bt = th; bs = sp; }
+        return 0;
+    };
+    int e = 0;
+    for (int sp : spis) if (!e) e = trial(bw, bt, sp);
+    const int s1 = bs;
+    for (int wg : wgs) if (!e) e = trial(wg, bt, s1);
+    const int w1 = bw;
+    for (int th : ths) if (!e) e = trial(w1, th, s1);
+    if (e) { c->wg_per_cu = keep_wg; c->threshold = keep_th; c->spi = keep_spi; c->drop_graph(); return e; }
+    c->wg_per_cu = bw; c->threshold = bt; c->spi = bs;
     c->drop_graph();
     if (best_wg) *best_wg = bw;
     if (best_th) *best_th = bt;
+    if (best_spi) *best_spi = bs;
     if (best_ms) *best_ms = best;
     return 0;
 }
@@ -709,21 +719,24 @@ int rc_surface_results(rc_ctx *c, int s, float *mean, float *sdev, double *de) {
     return 0;
 }
 
-int rc_set_schedule(rc_ctx *c, int wg_per_cu, int threshold) {
-    if (!c || wg_per_cu < 0 || wg_per_cu > 64 || threshold < 0 || threshold > 64) {
-        set_error("schedule: wg_per_cu in 0..64, threshold in 0..64 (0 = default)");
+int rc_set_schedule(rc_ctx *c, int wg_per_cu, int threshold, int steps_per_iter) {
+    if (!c || wg_per_cu < 0 || wg_per_cu > 64 || threshold < 0 || threshold > 64 || steps_per_iter < 0 ||
+        steps_per_iter > 8) {
+        set_error("schedule: wg_per_cu in 0..64, threshold in 0..64, steps_per_iter in 0..8 (0 = default)");
         return -1;
     }
     c->wg_per_cu = wg_per_cu;
     c->threshold = threshold;
+    c->spi = steps_per_iter;
     c->drop_graph();
     return 0;
 }
 
-int rc_get_schedule(rc_ctx *c, int32_t *wg_per_cu, int32_t *threshold) {
+int rc_get_schedule(rc_ctx *c, int32_t *wg_per_cu, int32_t *threshold, int32_t *steps_per_iter) {
     if (!c) { set_error("null context"); return -1; }
     *wg_per_cu = c->wg_per_cu;
     *threshold = c->threshold;
+    if (steps_per_iter) *steps_per_iter = c->spi;
     return 0;
 }
 

@@ -146,25 +146,42 @@ template <int TM = TAB_RT, class S>
 __device__ __forceinline__ int boundary_move(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r,
                                              const En &e, float &Bw) {
     const int nT = cm.nT;
-    int face;
-    if (r.ic == 0) face = FW;
-    else if (r.ic == L.im + 1) face = FE;
-    else if (r.jc == 0) face = FS;
-    else if (r.jc == L.jm + 1) face = FN;
-    else if (r.kc == 0) face = FB;
-    else face = FT;
-    const int bd = cm.bd[face];
-    if (bd == 1) {
-        switch (face) {
-            case FW: r.ic = L.im; r.xp = cm.Lx; break;
-            case FE: r.ic = 1; r.xp = 0.f; break;
-            case FS: r.jc = L.jm; r.yp = cm.Ly; break;
-            case FN: r.jc = 1; r.yp = 0.f; break;
-            case FB: r.kc = L.km; r.zp = cm.Lz; break;
-            default: r.kc = 1; r.zp = 0.f; break;
-        }
+    /* the face crossed: exactly one index left 1..n in the step */
+    const bool xo = (uint32_t)(r.ic - 1) >= (uint32_t)L.im;
+    const bool yo = !xo && (uint32_t)(r.jc - 1) >= (uint32_t)L.jm;
+    const bool zo = !xo && !yo;
+    const bool lo = xo ? r.ic == 0 : (yo ? r.jc == 0 : r.kc == 0);
+    const int face = (xo ? FW : (yo ? FS : FB)) + (lo ? 0 : 1);
+    /* its condition, from the six face codes packed 3 bits each (a
+     * wavefront-uniform word, no per-lane memory access) */
+    const uint32_t pack = (uint32_t)cm.bd[0] | ((uint32_t)cm.bd[1] << 3) | ((uint32_t)cm.bd[2] << 6) |
+                          ((uint32_t)cm.bd[3] << 9) | ((uint32_t)cm.bd[4] << 12) | ((uint32_t)cm.bd[5] << 15);
+    const int bd = (int)((pack >> (3 * face)) & 7u);
+    if (bd == 1 || bd == 3) {
+        /* periodic: to the opposite end (low face -> last cell, coordinate
+         * at the high end); specular: back into the edge cell, coordinate on
+         * the face, direction component reversed */
+        const bool per = bd == 1;
+        const bool hi_end = per == lo;
+        r.ic = xo ? (hi_end ? L.im : 1) : r.ic;
+        r.xp = xo ? (hi_end ? cm.Lx : 0.f) : r.xp;
+        r.jc = yo ? (hi_end ? L.jm : 1) : r.jc;
+        r.yp = yo ? (hi_end ? cm.Ly : 0.f) : r.yp;
+        r.kc = zo ? (hi_end ? L.km : 1) : r.kc;
+        r.zp = zo ? (hi_end ? cm.Lz : 0.f) : r.zp;
+        const bool fx = !per && xo, fy = !per && yo, fz = !per && zo;
+        r.sx = fx ? -r.sx : r.sx;
+        r.isx = fx ? -r.isx : r.isx;
+        r.sy = fy ? -r.sy : r.sy;
+        r.isy = fy ? -r.isy : r.isy;
+        r.sz = fz ? -r.sz : r.sz;
+        r.isz = fz ? -r.isz : r.isz;
+        r.f0 = r.sx < 0.f;
+        r.f1 = r.sy < 0.f;
+        r.f2 = r.sz < 0.f;
         return RB_ON;
-    } else if (bd == 2) {
+    }
+    if (bd == 2) {
         if ((face == FW && cm.adiab[0]) || (face == FE && cm.adiab[1])) {
             const float Tw = gld(L.Tw, (size_t)(face == FE) * L.jm * L.km + (size_t)(r.jc - 1) + (size_t)L.jm * (r.kc - 1));
             float Kw;
@@ -173,17 +190,8 @@ __device__ __forceinline__ int boundary_move(const DevCommon &cm, const S &sn, c
             Bw = gld(sn.bfix, (size_t)face * cm.nB + e.rank);
         }
         return RB_WALL;
-    } else if (bd == 3) {
-        switch (face) {
-            case FW: r.xp = 0.f; r.sx = -r.sx; r.isx = -r.isx; r.ic = 1; r.f0 = r.sx < 0.f; break;
-            case FE: r.xp = cm.Lx; r.sx = -r.sx; r.isx = -r.isx; r.ic = L.im; r.f0 = r.sx < 0.f; break;
-            case FS: r.yp = 0.f; r.sy = -r.sy; r.isy = -r.isy; r.jc = 1; r.f1 = r.sy < 0.f; break;
-            case FN: r.yp = cm.Ly; r.sy = -r.sy; r.isy = -r.isy; r.jc = L.jm; r.f1 = r.sy < 0.f; break;
-            case FB: r.zp = 0.f; r.sz = -r.sz; r.isz = -r.isz; r.kc = 1; r.f2 = r.sz < 0.f; break;
-            default: r.zp = cm.Lz; r.sz = -r.sz; r.isz = -r.isz; r.kc = L.km; r.f2 = r.sz < 0.f; break;
-        }
-        return RB_ON;
-    } else if (bd == 4) {
+    }
+    if (bd == 4) {
         const float Tw = gld(L.Tz, (size_t)(face == FT) * L.im * L.jm + (size_t)(r.ic - 1) + (size_t)L.im * (r.jc - 1));
         float Kw;
         kb_at<TM>(sn, e.row, e.rank, nT, Tw, Kw, Bw);
@@ -275,6 +283,37 @@ __device__ __forceinline__ bool march_core(const DevCommon &cm, const S &sn, con
     const bool out = geo_core(L, r, cell, ds);
     float K, B;
     kb_at<TM>(sn, e.row, e.rank, cm.nT, gld32(L.T, cell), K, B);
+    phys_core(K, B, ds, e, tol, r.tra, de, status);
+    return out;
+}
+
+/* march_core with the ray's last table pair kept in registers: a ray's
+ * table row is fixed, and along its path the temperature row t changes
+ * only every few cells, so the pair is loaded only when t changes (kt:
+ * the cached t, -1 for none; the loads of the other lanes are masked off).
+ * Same values, so the same results as march_core. */
+template <int TM, class S>
+__device__ __forceinline__ bool march_core_cached(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r,
+                                                  const En &e, float tol, float &de, int &status, int &kt,
+                                                  KbRaw &kc) {
+    uint32_t cell;
+    float ds;
+    const bool out = geo_core(L, r, cell, ds);
+    int t;
+    float a;
+    tpos(gld32(L.T, cell), sn.Tnb0, sn.invdT, cm.nT, t, a);
+    if (t != kt) {
+        if (TM == TAB_BF16) {
+            kc.w = gld32(sn.kp16, e.row * (uint32_t)cm.nT + (uint32_t)t);
+            kc.q = gld32(sn.bp, (uint32_t)e.rank * (uint32_t)cm.nT + (uint32_t)t);
+        } else {
+            kc.q = gld32_pair(sn.kb, e.row * (uint32_t)(cm.nT + 1) + (uint32_t)t);
+        }
+        kt = t;
+    }
+    kc.a = a;
+    float K, B;
+    kb_finish<TM>(kc, K, B);
     phys_core(K, B, ds, e, tol, r.tra, de, status);
     return out;
 }

@@ -130,11 +130,11 @@ def load() -> C.CDLL:
         "rc_surface_results": (C.c_int, [vp, C.c_int, P(C.c_float), P(C.c_float), P(C.c_double)]),
         "rc_trace": (C.c_int, [vp, C.c_int, P(C.c_int32), C.c_int, P(Ray)]),
         "rc_info": (C.c_int, [vp, C.c_int, P(C.c_float), P(C.c_int32)]),
-        "rc_set_schedule": (C.c_int, [vp, C.c_int, C.c_int]),
-        "rc_get_schedule": (C.c_int, [vp, P(C.c_int32), P(C.c_int32)]),
+        "rc_set_schedule": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "rc_get_schedule": (C.c_int, [vp, P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
         "rc_set_graph": (C.c_int, [vp, C.c_int]),
         "rc_set_table_precision": (C.c_int, [vp, C.c_int]),
-        "rc_autotune": (C.c_int, [vp, P(C.c_int32), P(C.c_int32), P(C.c_float)]),
+        "rc_autotune": (C.c_int, [vp, P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_float)]),
         "rc_device_synchronize": (C.c_int, []),
     }
     for name, (res, args) in sig.items():

@@ -99,8 +99,8 @@ class Solver:
                                   out.ctypes.data_as(C.POINTER(L.Ray))), "rc_trace")
         return out
 
-    def set_schedule(self, wg_per_cu: int = 0, threshold: int = 0) -> None:
-        L.check(self.lib.rc_set_schedule(self.ctx, wg_per_cu, threshold), "rc_set_schedule")
+    def set_schedule(self, wg_per_cu: int = 0, threshold: int = 0, steps_per_iter: int = 0) -> None:
+        L.check(self.lib.rc_set_schedule(self.ctx, wg_per_cu, threshold, steps_per_iter), "rc_set_schedule")
 
     def set_table_precision(self, bf16: bool) -> None:
         L.check(self.lib.rc_set_table_precision(self.ctx, int(bf16)), "rc_set_table_precision")
@@ -109,9 +109,9 @@ class Solver:
         L.check(self.lib.rc_set_graph(self.ctx, int(on)), "rc_set_graph")
 
     def autotune(self) -> dict:
-        wg, th, ms = C.c_int32(), C.c_int32(), C.c_float()
-        L.check(self.lib.rc_autotune(self.ctx, C.byref(wg), C.byref(th), C.byref(ms)), "rc_autotune")
-        return {"wg_per_cu": wg.value, "threshold": th.value, "march_ms": ms.value}
+        wg, th, sp, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_float()
+        L.check(self.lib.rc_autotune(self.ctx, C.byref(wg), C.byref(th), C.byref(sp), C.byref(ms)), "rc_autotune")
+        return {"wg_per_cu": wg.value, "threshold": th.value, "steps_per_iter": sp.value, "march_ms": ms.value}
 
     def info(self, s: int = 0) -> dict:
         f = np.zeros(5, np.float32)
