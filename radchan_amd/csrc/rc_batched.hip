@@ -65,36 +65,37 @@ __device__ __forceinline__ uint32_t lane_xor(uint32_t v, int m) {
     }
 }
 
-/* bitonic sort of 256 keys held in registers, element 4 lane + r in x[r]:
- * compare-exchanges at distance 1 and 2 stay inside a lane; the 21 stages
- * at distance 4..128 exchange with lane (lane ^ distance/4).  No LDS
- * traffic and no barrier. */
-__device__ __forceinline__ void wave_sort256(uint32_t (&x)[4], int lane) {
+/* bitonic sort of 64 R keys held in registers, element R lane + r in x[r]:
+ * compare-exchanges at distances below R stay inside a lane; the stages at
+ * distance R..32 R exchange with lane (lane ^ distance/R) (21 of them).  No
+ * LDS traffic and no barrier. */
+template <int R>
+__device__ __forceinline__ void wave_sort(uint32_t (&x)[R], int lane) {
     /* the lane index as an opaque value, so that the per-stage lane masks
      * are formed here instead of being hoisted out of the caller's loop and
      * kept live across it */
     asm volatile("" : "+v"(lane));
 #pragma unroll
-    for (int k = 2; k <= 256; k <<= 1) {
+    for (int k = 2; k <= 64 * R; k <<= 1) {
 #pragma unroll
         for (int j = k >> 1; j > 0; j >>= 1) {
-            if (j < 4) {
-                const bool upl = ((lane * 4) & k) == 0;
+            if (j < R) {
+                const bool upl = ((lane * R) & k) == 0;
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
+                for (int r = 0; r < R; ++r) {
                     const int p = r ^ j;
                     if (p > r) {
-                        const bool up = (k < 4) ? ((r & k) == 0) : upl;
+                        const bool up = (k < R) ? ((r & k) == 0) : upl;
                         const uint32_t lo = min(x[r], x[p]), hi = max(x[r], x[p]);
                         x[r] = up ? lo : hi;
                         x[p] = up ? hi : lo;
                     }
                 }
             } else {
-                const bool keep_min = (((lane * 4) & j) == 0) == (((lane * 4) & k) == 0);
+                const bool keep_min = (((lane * R) & j) == 0) == (((lane * R) & k) == 0);
 #pragma unroll
-                for (int r = 0; r < 4; ++r) {
-                    const uint32_t y = lane_xor(x[r], j >> 2);
+                for (int r = 0; r < R; ++r) {
+                    const uint32_t y = lane_xor(x[r], j / R);
                     x[r] = keep_min ? min(x[r], y) : max(x[r], y);
                 }
             }
@@ -286,6 +287,37 @@ __device__ __forceinline__ void finish_ray(const Slots &sl, int seq, int runit, 
  * unit's rays and its sorted keys (16-bit: keys are below nB nQ <= 65535;
  * at least 256, the register sort's width) */
 constexpr int LEV_WORDS = (int)(sizeof(DevLevel) / 4);
+
+/* drain of the last rays: at most COOP_RAYS live rays are marched one at a
+ * time by the whole wavefront; above that, in blocks of DRAIN_BLOCK steps
+ * per lane */
+constexpr int COOP_RAYS = 4;
+constexpr int DRAIN_BLOCK = 64;
+
+/* lane src's value in every lane, as a per-lane value (a crossbar
+ * permute), so that the arithmetic on it stays on the vector ALU */
+__device__ __forceinline__ int bcast(int x, int src) { return __shfl(x, src); }
+__device__ __forceinline__ uint32_t bcast(uint32_t x, int src) { return (uint32_t)__shfl((int)x, src); }
+__device__ __forceinline__ float bcast(float x, int src) { return __shfl(x, src); }
+__device__ __forceinline__ Ray bcast_ray(const Ray &a, int src) {
+    Ray b;
+    b.ic = bcast(a.ic, src); b.jc = bcast(a.jc, src); b.kc = bcast(a.kc, src);
+    b.f0 = bcast(a.f0, src); b.f1 = bcast(a.f1, src); b.f2 = bcast(a.f2, src);
+    b.xp = bcast(a.xp, src); b.yp = bcast(a.yp, src); b.zp = bcast(a.zp, src);
+    b.sx = bcast(a.sx, src); b.sy = bcast(a.sy, src); b.sz = bcast(a.sz, src);
+    b.isx = bcast(a.isx, src); b.isy = bcast(a.isy, src); b.isz = bcast(a.isz, src);
+    b.tra = bcast(a.tra, src);
+    return b;
+}
+__device__ __forceinline__ En bcast_en(const En &a, int src) {
+    En b;
+    b.row = bcast(a.row, src);
+    b.rank = bcast(a.rank, src);
+    b.W = bcast(a.W, src);
+    b.Bi = bcast(a.Bi, src);
+    b.invBi = bcast(a.invBi, src);
+    return b;
+}
 __host__ __device__ __forceinline__ int batched_lds_words(int n2, int srt, int multi) {
     return SLOT_WORDS * NSLOT + QREC * QCAP + (multi ? LEV_WORDS * RC_MAXLEV : 0) +
            (srt ? 2 * n2 + (n2 < 256 ? 256 : n2) / 2 : 0);
@@ -387,9 +419,10 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                     const DevSnap &sn = snaps[N.s];
                     const uint32_t tagx = N.tag + 16u * (uint32_t)sn.snap;
                     if (n2 <= 256) {
-                        /* up to 256 keys: drawn one round of 64 at a time,
-                         * then sorted in registers */
-                        /* rounds of 64 draws up to photmax, then padding */
+                        /* up to 256 keys: drawn one round of 64 at a time
+                         * up to photmax, padded to 128 or 256, then sorted
+                         * in registers, 2 or 4 keys per lane */
+                        const int nr = n2 <= 128 ? 128 : 256;
 #pragma unroll 1
                         for (int h = lane; h < ((P + 63) & ~63); h += 64) {
                             uint32_t key = 0xFFFFFFFFu;
@@ -401,14 +434,23 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                             }
                             keys[h] = (uint16_t)key;
                         }
-                        for (int h = ((P + 63) & ~63) + lane; h < 256; h += 64) keys[h] = (uint16_t)0xFFFFu;
+                        for (int h = ((P + 63) & ~63) + lane; h < nr; h += 64) keys[h] = (uint16_t)0xFFFFu;
                         __syncthreads();
-                        /* keys 4 lane .. 4 lane + 3: one 8-byte LDS word */
-                        uint2 *kw = reinterpret_cast<uint2 *>(keys) + lane;
-                        const uint2 w = *kw;
-                        uint32_t x[4] = {w.x & 0xFFFFu, w.x >> 16, w.y & 0xFFFFu, w.y >> 16};
-                        wave_sort256(x, lane);
-                        *kw = make_uint2(x[0] | (x[1] << 16), x[2] | (x[3] << 16));
+                        if (nr == 128) {
+                            /* keys 2 lane, 2 lane + 1: one 4-byte LDS word */
+                            uint32_t *kw = reinterpret_cast<uint32_t *>(keys) + lane;
+                            const uint32_t w = *kw;
+                            uint32_t x[2] = {w & 0xFFFFu, w >> 16};
+                            wave_sort<2>(x, lane);
+                            *kw = x[0] | (x[1] << 16);
+                        } else {
+                            /* keys 4 lane .. 4 lane + 3: one 8-byte LDS word */
+                            uint2 *kw = reinterpret_cast<uint2 *>(keys) + lane;
+                            const uint2 w = *kw;
+                            uint32_t x[4] = {w.x & 0xFFFFu, w.x >> 16, w.y & 0xFFFFu, w.y >> 16};
+                            wave_sort<4>(x, lane);
+                            *kw = make_uint2(x[0] | (x[1] << 16), x[2] | (x[3] << 16));
+                        }
                         __syncthreads();
                     } else {
                         for (int h = lane; h < n2; h += 64) {
@@ -521,9 +563,15 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
             }
             const bool drained = qdone && nxt < 0 && cur_h >= P && qcount == 0;
             if (drained && (__ballot(active) == 0ull || xfit)) {
-                /* (f) no ray left to start: every lane marches its ray to
-                 * the end with the look-ahead march (the rays still live
-                 * here are the longest ones), then the wavefront is done */
+                /* (f) no ray left to start: the rays still live here are
+                 * the longest ones.  Multigrid: every live lane marches
+                 * its ray to the end with the look-ahead march (the step
+                 * caps of the levels keep the rays short).  Single grid:
+                 * while more than COOP_RAYS lanes are live, every live lane
+                 * marches its ray DRAIN_BLOCK steps with the look-ahead
+                 * march; below that, the wavefront marches the remaining
+                 * rays one at a time, all lanes on one ray.  Then the
+                 * wavefront is done. */
                 if (__ballot(active) != 0ull) {
                     const SnapView sv = snap_view(snaps + snap);
                     DevLevel l0;
@@ -538,27 +586,59 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                         off += Lg.im + 1;
                     }
                     __syncthreads();
-                    if (active) {
-                        /* the lane's ray may be on a coarser level already */
-                        int xoff = 0;
-                        if (MULTI)
+                    if (MULTI) {
+                        if (active) {
+                            /* the lane's ray may be on a coarser level already */
+                            int xoff = 0;
                             for (int g = 0; g < lev; ++g) xoff += llev[g].im + 1;
-                        for (;;) {
-                            const DevLevel L = MULTI ? llev[lev] : l0;
-                            march_ahead<TM, true>(cm, sv, L, r, e, tol, dacc, counter, nsteps, xul + xoff);
-                            if (!MULTI || !(r.tra > tol) || lev == cm.grid_num - 1) break;
-                            const DevLevel Nl = llev[lev + 1];
-                            const int ri = L.im / Nl.im, rj = L.jm / Nl.jm, rk = L.km / Nl.km;
-                            r.ic = (r.ic + ri - 1) / ri;
-                            r.jc = (r.jc + rj - 1) / rj;
-                            r.kc = (r.kc + rk - 1) / rk;
-                            xoff += L.im + 1;
-                            lev += 1;
-                            counter = 0;
+                            for (;;) {
+                                const DevLevel L = llev[lev];
+                                march_ahead<TM, true>(cm, sv, L, r, e, tol, dacc, counter, nsteps, xul + xoff, L.maxs);
+                                if (!(r.tra > tol) || lev == cm.grid_num - 1) break;
+                                /* coarsen the indices for the next grid level */
+                                const DevLevel Nl = llev[lev + 1];
+                                const int ri = L.im / Nl.im, rj = L.jm / Nl.jm, rk = L.km / Nl.km;
+                                r.ic = (r.ic + ri - 1) / ri;
+                                r.jc = (r.jc + rj - 1) / rj;
+                                r.kc = (r.kc + rk - 1) / rk;
+                                xoff += L.im + 1;
+                                lev += 1;
+                                counter = 0;
+                            }
+                            finish_ray(sl, seq, runit, rtid, dacc, de);
+                            nrays += 1;
+                            active = false;
                         }
-                        finish_ray(sl, seq, runit, rtid, dacc, de);
-                        nrays += 1;
-                        active = false;
+                    } else {
+                        for (;;) {
+                            const unsigned long long act = __ballot(active);
+                            if (act == 0ull) break;
+                            if (__popcll(act) > COOP_RAYS) {
+                                if (active) {
+                                    march_ahead<TM, true>(cm, sv, l0, r, e, tol, dacc, counter, nsteps, xul,
+                                                          min(l0.maxs, counter + DRAIN_BLOCK));
+                                    if (!(r.tra > tol) || counter >= l0.maxs) {
+                                        finish_ray(sl, seq, runit, rtid, dacc, de);
+                                        nrays += 1;
+                                        active = false;
+                                    }
+                                }
+                            } else {
+                                /* the lowest live lane's ray, by every lane */
+                                const int src = __builtin_ctzll(act);
+                                Ray cr = bcast_ray(r, src);
+                                const En ce = bcast_en(e, src);
+                                float cd = bcast(dacc, src);
+                                int cc = bcast(counter, src), cs = 0;
+                                march_coop<TM>(cm, sv, l0, cr, ce, tol, cd, cc, cs, xul, lane);
+                                if (lane == src) {
+                                    finish_ray(sl, seq, runit, rtid, cd, de);
+                                    nsteps += cs;
+                                    nrays += 1;
+                                    active = false;
+                                }
+                            }
+                        }
                     }
                 }
                 break;

@@ -217,20 +217,13 @@ __device__ __forceinline__ void boundary(const DevCommon &cm, const S &sn, const
 }
 
 /* The geometry of one march step (one iteration of march_ray's loop,
- * mc_gpu_noint.cu:252-654): distances to the three candidate faces, the
- * segment length ds, the position update and the index move along the
- * crossed axis; `cell` is the cell the segment crosses.  Nothing here
- * depends on the temperature.  Returns whether an index left the grid
- * (the boundary condition is then due). */
-template <bool XL = false>
-__device__ __forceinline__ bool geo_core(const DevLevel &L, Ray &r, uint32_t &cell, float &ds,
-                                         const float *xul = nullptr) {
-    /* both indices are in range for a ray inside the grid; the bounds only
-     * matter to a lane whose state is stale (the batched kernel steps idle
-     * lanes too and discards what they compute).  With XL the x faces are
-     * read from an LDS copy. */
-    const uint32_t ix = min((uint32_t)(r.ic - r.f0), (uint32_t)L.im);
-    const float fx = XL ? xul[ix] : gld32(L.xu, ix);
+ * mc_gpu_noint.cu:252-654), given the x face the ray heads for
+ * (fx = xu[ic - f0]): distances to the three candidate faces, the segment
+ * length ds, the position update and the index move along the crossed
+ * axis; `cell` is the cell the segment crosses.  Nothing here depends on
+ * the temperature.  Returns whether an index left the grid (the boundary
+ * condition is then due). */
+__device__ __forceinline__ bool geo_core_fx(const DevLevel &L, Ray &r, float fx, uint32_t &cell, float &ds) {
     const float fy = (float)(r.jc - r.f1) * L.dy;
     const float fz = (float)(r.kc - r.f2) * L.dz;
     const float dsx = (fx - r.xp) * r.isx;
@@ -252,6 +245,19 @@ __device__ __forceinline__ bool geo_core(const DevLevel &L, Ray &r, uint32_t &ce
     r.kc += za ? (r.f2 ? -1 : 1) : 0;
     return (uint32_t)(r.ic - 1) >= (uint32_t)L.im || (uint32_t)(r.jc - 1) >= (uint32_t)L.jm ||
            (uint32_t)(r.kc - 1) >= (uint32_t)L.km;
+}
+
+/* geo_core_fx with the x face looked up: in global memory, or with XL in
+ * an LDS copy */
+template <bool XL = false>
+__device__ __forceinline__ bool geo_core(const DevLevel &L, Ray &r, uint32_t &cell, float &ds,
+                                         const float *xul = nullptr) {
+    /* both indices are in range for a ray inside the grid; the bounds only
+     * matter to a lane whose state is stale (the batched kernel steps idle
+     * lanes too and discards what they compute) */
+    const uint32_t ix = min((uint32_t)(r.ic - r.f0), (uint32_t)L.im);
+    const float fx = XL ? xul[ix] : gld32(L.xu, ix);
+    return geo_core_fx(L, r, fx, cell, ds);
 }
 
 /* The absorption of one step given the crossed cell's k and Ib: attenuation
@@ -318,19 +324,21 @@ __device__ __forceinline__ bool march_core_cached(const DevCommon &cm, const S &
     return out;
 }
 
-/* A ray marched on one grid level to its end, or to the level's step cap,
- * with the geometry two steps ahead of the absorption: the temperature of
- * step i+2 and the table entries of step i+1 are loaded while step i is
- * absorbed.  The arithmetic of every step is that of march_step, so the
- * tally, transmissivity and step count are bit-identical; only a dumped
- * ray's final position differs (the geometry ran ahead of it).  Expects a
- * live ray under the cap (counter < L.maxs).  With XL the level's x faces
- * come from an LDS copy (xul), so that the geometry never waits on the
- * vector-memory counter behind the prefetched loads. */
+/* A ray marched on one grid level to its end, or until its step counter
+ * reaches `limit` (at most the level's step cap L.maxs), with the geometry
+ * two steps ahead of the absorption: the temperature of step i+2 and the
+ * table entries of step i+1 are loaded while step i is absorbed.  The
+ * arithmetic of every step is that of march_step, so the tally,
+ * transmissivity and step count are bit-identical; only a dumped ray's
+ * final position differs (the geometry ran ahead of it).  A ray stopped by
+ * the limit has taken exactly its steps up to the limit, so the march can
+ * resume from it.  Expects a live ray below the limit.  With XL the
+ * level's x faces come from an LDS copy (xul), so that the geometry never
+ * waits on the vector-memory counter behind the prefetched loads. */
 template <int TM = TAB_RT, bool XL = false, class S>
 __device__ __forceinline__ void march_ahead(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r,
                                             const En &e, float tol, float &de, int &counter, int &nsteps,
-                                            const float *xul = nullptr) {
+                                            const float *xul, int limit) {
     const int nT = cm.nT;
     int status = 0;
     /* step i */
@@ -344,7 +352,7 @@ __device__ __forceinline__ void march_ahead(const DevCommon &cm, const S &sn, co
      * whose values are then never used), so that the number of loads in
      * flight does not depend on the path and the wait for the oldest one
      * does not have to drain the newer ones. */
-    bool h1 = b0 == RB_ON && counter + 1 < L.maxs;
+    bool h1 = b0 == RB_ON && counter + 1 < limit;
     uint32_t c1 = 0u;
     float d1, w1;
     int b1 = RB_ON;
@@ -353,7 +361,7 @@ __device__ __forceinline__ void march_ahead(const DevCommon &cm, const S &sn, co
     float T1 = gld32(L.T, c1);
     for (;;) {
         /* step i+2: geometry and temperature */
-        const bool h2 = h1 && b1 == RB_ON && counter + 2 < L.maxs;
+        const bool h2 = h1 && b1 == RB_ON && counter + 2 < limit;
         uint32_t c2 = 0u;
         float d2, w2;
         int b2 = RB_ON;
@@ -379,6 +387,130 @@ __device__ __forceinline__ void march_ahead(const DevCommon &cm, const S &sn, co
         if (!(r.tra > tol) || !h1) break;
         k0 = k1; d0 = d1; w0 = w1; b0 = b1;
         h1 = h2; d1 = d2; w1 = w2; b1 = b2; T1 = T2;
+    }
+}
+
+/* One ray marched on one grid level by a whole wavefront, every lane
+ * holding the same ray (as per-lane values, so that the geometry stays on
+ * the vector ALU): up to 64 steps of geometry run in every lane (each step
+ * needs the position the one before left), lane s keeping the cell and
+ * length of step s; the temperatures, table entries and exponentials of
+ * those steps are then computed one step per lane, all at once; last, the
+ * transmissivity is carried through the steps in order (lane s receives
+ * the value before its step), the step that ends the ray found by a
+ * ballot, and the tally terms, computed per lane, summed in step order.
+ * Every product and sum is the one march_step makes, on the same values
+ * and in the same order, so the tally, transmissivity and step count are
+ * bit-identical to it.  Returns at the ray's end or at the level's step
+ * cap; expects a live ray under the cap.  `xul`: the level's x faces in
+ * LDS. */
+template <int TM, class S>
+__device__ __forceinline__ void march_coop(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r,
+                                           const En &e, float tol, float &de, int &counter, int &nsteps,
+                                           const float *xul, int lane) {
+    /* Per-ray constants of the geometry (recomputed after a reflection):
+     * index steps along each axis and the matching steps of the linear
+     * cell index; the cell index itself is carried from step to step.  The
+     * x face the ray heads for (fxc) and the next one along x (fxn, index
+     * ixn) are read one step ahead of their use. */
+    int dix, diy, diz, dcy, dcz;
+    uint32_t cur, ixn;
+    float fxc, fxn;
+    auto setup = [&]() {
+        dix = r.f0 ? -1 : 1;
+        diy = r.f1 ? -1 : 1;
+        diz = r.f2 ? -1 : 1;
+        dcy = diy * L.im;
+        dcz = diz * L.im * L.jm;
+        cur = min((uint32_t)(r.ic - 1) + (uint32_t)L.im * ((uint32_t)(r.jc - 1) + (uint32_t)L.jm * (uint32_t)(r.kc - 1)),
+                  (uint32_t)L.ncell - 1u);
+        const uint32_t ix = min((uint32_t)(r.ic - r.f0), (uint32_t)L.im);
+        ixn = min(ix + (uint32_t)dix, (uint32_t)L.im);
+        fxc = xul[ix];
+        fxn = xul[ixn];
+    };
+    setup();
+    for (;;) {
+        /* geometry of up to 64 steps (the arithmetic of geo_core_fx); only
+         * the block's last step can meet a black or open face (the block
+         * stops there) */
+        uint32_t mcell = 0u;
+        float mds = 0.f, wl = 0.f;
+        int bl = RB_ON;
+        const int nmax = min(64, L.maxs - __builtin_amdgcn_readfirstlane(counter));
+        int n = 0;
+        for (;;) {
+            const float fy = (float)(r.jc - r.f1) * L.dy;
+            const float fz = (float)(r.kc - r.f2) * L.dz;
+            const float dsx = (fxc - r.xp) * r.isx;
+            const float dsy = (fy - r.yp) * r.isy;
+            const float dsz = (fz - r.zp) * r.isz;
+            float ds = dsx;
+            ds = (ds > dsy) ? dsy : ds;
+            ds = (ds > dsz) ? dsz : ds;
+            r.xp = (ds == dsx) ? fxc : fmaf(ds, r.sx, r.xp);
+            r.yp = (ds == dsy) ? fy : fmaf(ds, r.sy, r.yp);
+            r.zp = (ds == dsz) ? fz : fmaf(ds, r.sz, r.zp);
+            const bool xa = dsx < dsy && dsx < dsz;
+            const bool ya = !(dsx < dsy) && dsy < dsz;
+            r.ic += xa ? dix : 0;
+            r.jc += ya ? diy : 0;
+            r.kc += (!xa && !ya) ? diz : 0;
+            mcell = lane == n ? cur : mcell;
+            mds = lane == n ? ds : mds;
+            cur += xa ? (uint32_t)dix : (uint32_t)(ya ? dcy : dcz);
+            fxc = xa ? fxn : fxc;
+            ixn = xa ? min(ixn + (uint32_t)dix, (uint32_t)L.im) : ixn;
+            fxn = xul[ixn];
+            n += 1;
+            const bool out = (uint32_t)(r.ic - 1) >= (uint32_t)L.im || (uint32_t)(r.jc - 1) >= (uint32_t)L.jm ||
+                             (uint32_t)(r.kc - 1) >= (uint32_t)L.km;
+            if (__builtin_amdgcn_readfirstlane((int)out)) {
+                bl = boundary_move<TM>(cm, sn, L, r, e, wl);
+                setup();
+            }
+            if (__builtin_amdgcn_readfirstlane(bl) != RB_ON || n >= nmax) break;
+        }
+        /* the absorption terms of the n steps, one per lane (the lanes
+         * from n up work on cell 0 and are not read) */
+        float K, B;
+        kb_finish<TM>(kb_fetch<TM>(sn, e.row, e.rank, cm.nT, gld32(L.T, mcell)), K, B);
+        float em1, ex;
+        expm1_exp(-(K * mds), em1, ex);
+        const float alpha = -em1;
+        const float fac = (B - e.Bi) * e.invBi;
+        /* transmissivity before each step: trl in lane s */
+        float trl = 0.f, t = r.tra;
+        for (int s = 0; s < n; ++s) {
+            trl = lane == s ? t : trl;
+            t = t * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(ex), s));
+        }
+        /* the step that ends the ray: the transmissivity falls to tol or
+         * below (dumped below it), or the block's last step met a black or
+         * open face */
+        const float tn = trl * ex;
+        const bool ends = lane < n && (!(tn > tol) || (lane == n - 1 && bl != RB_ON));
+        const unsigned long long em = __ballot(ends);
+        const int m = em ? (int)__builtin_ctzll(em) + 1 : n;
+        /* the tally terms of the steps taken, in order */
+        const float term = ((e.W * trl) * alpha) * fac;
+        for (int s = 0; s < m; ++s) de -= __int_as_float(__builtin_amdgcn_readlane(__float_as_int(term), s));
+        counter += m;
+        nsteps += m;
+        if (em) {
+            const float te = __int_as_float(__builtin_amdgcn_readlane(__float_as_int(tn), m - 1));
+            if (te < tol) {
+                de -= (e.W * te) * __int_as_float(__builtin_amdgcn_readlane(__float_as_int(fac), m - 1));
+                r.tra = 0.f;
+            } else {
+                r.tra = te;
+            }
+            if (m == n && bl == RB_WALL) wall_hit(e, r, wl, de);
+            else if (m == n && bl == RB_LOST) r.tra = 0.f;
+            return;
+        }
+        r.tra = t;
+        if (counter >= L.maxs) return;
     }
 }
 

@@ -112,6 +112,14 @@ def test_field_narrow_band(nb_tables, variant):
     _field_parity(*cases.nb_case(nb_tables), variant)
 
 
+@pytest.mark.parametrize("photmax", [100, 200, 300])
+@pytest.mark.parametrize("variant", [VARIANT_PLAIN, VARIANT_BATCHED])
+def test_field_narrow_band_sort_widths(nb_tables, variant, photmax):
+    """photmax above 64: the batched kernel's band keys sorted in registers
+    2 per lane (<= 128), 4 per lane (<= 256) and in LDS (above)."""
+    _field_parity(*cases.nb_case(nb_tables, n=8, photmax=photmax), variant)
+
+
 @pytest.mark.parametrize("variant", [VARIANT_PLAIN, VARIANT_BATCHED])
 def test_field_multigrid_and_walls(nb_tables, variant):
     p, tab, T = cases.mg_case(nb_tables)
