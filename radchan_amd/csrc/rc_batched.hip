@@ -703,9 +703,11 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
     }
 }
 
-/* single grid */
+/* single grid (registers held to 5 wavefronts per SIMD: the march is
+ * issue-bound, and a fifth wavefront outweighs the few values the
+ * compiler then keeps in scratch) */
 template <bool SURF, int TM>
-__global__ void __launch_bounds__(64)
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_batched(DevCommon cm, const DevSnap *__restrict__ snaps, int nsnap, double *__restrict__ de,
           unsigned long long *__restrict__ stats, int *__restrict__ queue, int threshold, int spi) {
     batched_body<SURF, false, TM>(cm, snaps, nsnap, de, stats, queue, threshold, spi);
