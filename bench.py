@@ -11,7 +11,9 @@ sorted bands (srt=1), nVar 8, FP32 march with FP64 tallies.
 
 Legs: an untimed autotune of the batched schedule; the batched solver with
 FP32 and with BF16 k-tables, each replayed as a hipGraph; the batched march
-kernel alone (HIP events, no graph) for the roofline; the plain solver.
+kernel alone (HIP events, no graph) for the roofline; the plain solver; one
+solve at 4x photmax (checks eps ~ rays^-1/2, which the time-to-1%
+projection assumes); the 256x128x256 field, batched against plain.
 
 Printed (one JSON line on stdout): rays/s of the batched solver (FP32
 k-tables unless --tables bf16), the plain solver's rays/s beside it, the
@@ -101,12 +103,145 @@ def timed_leg(s, lib, variant: int, steps: int, warmup: int):
     return (time.perf_counter() - t0) / steps, st, march
 
 
+def eps_scaling(p, tab, T, sched, eps, bf16: bool):
+    """eps at 4x photmax on the same field (one solve): the projection to
+    eps = 1% assumes eps ~ rays^(-1/2), so the ratio should be ~0.5.  (The
+    time is not 4x: above 256 rays per cell the band keys are sorted in LDS
+    rather than in registers.)"""
+    from radchan_amd import VARIANT_BATCHED, Solver
+    import copy
+    q = copy.deepcopy(p)
+    q.photmax = 4 * p.photmax
+    s = Solver(q, tab)
+    s.set_fields([T])
+    s.set_table_precision(bf16)
+    if sched:
+        s.set_schedule(sched["wg_per_cu"], sched["threshold"], sched["steps_per_iter"])
+    s.set_graph(False)
+    s.solve(VARIANT_BATCHED)
+    st = s.solve(VARIANT_BATCHED)
+    e4, _ = std_error(s.tallies())
+    s.close()
+    return {"photmax": q.photmax, "eps": e4, "eps_ratio": e4 / eps, "expected_ratio": 0.5,
+            "march_ms": st["march_ms"]}
+
+
+def target_leg(lib, tab):
+    """The 256 x 128 x 256 narrow-band field: batched (autotuned schedule,
+    FP32 k tables, hipGraph) against the plain solver, rays/s and time to
+    eps = 1%."""
+    from radchan_amd import VARIANT_BATCHED, VARIANT_PLAIN, Solver, config
+    from radchan_amd import fields as F
+    p = config("big256")
+    p.nVar = 8
+    T = F.channel_field(p, 300.0, 1500.0, noise=0.02, seed=0)
+    s = Solver(p, tab)
+    s.set_fields([T])
+    sched = s.autotune()
+    s.set_graph(True)
+    wall, st, _ = timed_leg(s, lib, VARIANT_BATCHED, 2, 1)
+    eps, _ = std_error(s.tallies())
+    s.set_graph(False)
+    pwall, sp, _ = timed_leg(s, lib, VARIANT_PLAIN, 1, 1)
+    eps_p, _ = std_error(s.tallies())
+    s.close()
+    t1 = wall * (eps / 0.01) ** 2
+    tp = pwall * (eps_p / 0.01) ** 2
+    log(f"target 256x128x256: batched {1e3 * wall:.1f} ms, plain {1e3 * pwall:.1f} ms, "
+        f"time-to-1% speed-up {tp / t1:.2f}x")
+    return {"workload": f"narrow-band correlated-k OERM {p.imax}x{p.jmax}x{p.kmax}, photmax {p.photmax}, "
+                        f"nVar {p.nVar}, FP32 k tables",
+            "schedule": sched, "rays": int(st["rays"]), "steps": int(st["steps"]),
+            "batched": {"ms_per_step": 1e3 * wall, "rays_per_s": st["rays"] / wall, "eps": eps,
+                        "time_to_1pct_s": t1},
+            "plain": {"ms_per_step": 1e3 * pwall, "rays_per_s": sp["rays"] / pwall, "eps": eps_p,
+                      "time_to_1pct_s": tp},
+            "time_to_1pct_speedup_vs_plain": tp / t1}
+
+
+def bench_c4(args) -> int:
+    """The snapshot ensemble: 8 snapshots of 64x96x64 and 4 of 256x128x256
+    (laminar 300-1800 K profile plus 5% smoothed noise, its own noise seed
+    per snapshot), photmax 100, nVar 8, FP32 k tables.  Each
+    shape is one batched launch over all its snapshots; the two launches go
+    to their own streams and run together.  A step solves all 12 snapshots;
+    eps and the time to 1% are those of the worst large snapshot."""
+    from radchan_amd import VARIANT_BATCHED, VARIANT_PLAIN, Solver, config
+    from radchan_amd import fields as F
+    from radchan_amd import lib as L
+    from radchan_amd.tables import synthetic_nb
+
+    tab = synthetic_nb()
+    lib = L.load()
+    shapes = []
+    for name, n, seed0 in (("big256", 4, 100), ("ens64x96", 8, 200)):
+        p = config(name)
+        p.nVar = args.nvar
+        Ts = [F.channel_field(p, 300.0, 1800.0, noise=0.05, seed=seed0 + i) for i in range(n)]
+        s = Solver(p, tab)
+        s.set_fields(Ts)
+        shapes.append((name, p, n, s))
+        log(f"c4: {n} x {p.imax}x{p.jmax}x{p.kmax}")
+    # schedules: the small shape autotuned on its snapshots; the large one
+    # takes the schedule the autotuner picks for a single 256x128x256 field
+    shapes[1][3].autotune()
+    shapes[0][3].set_schedule(32, 8, 4)
+
+    def step(variant):
+        for _, _, _, s in shapes:
+            s.run(variant)
+        return [s.sync() for _, _, _, s in shapes]
+
+    def timed(variant, steps, warmup):
+        for _ in range(warmup):
+            step(variant)
+        lib.rc_device_synchronize()
+        t0 = time.perf_counter()
+        for _ in range(steps):
+            sts = step(variant)
+        lib.rc_device_synchronize()
+        return (time.perf_counter() - t0) / steps, sts
+
+    for _, _, _, s in shapes:
+        s.set_graph(not args.no_graph)
+    wall, sts = timed(VARIANT_BATCHED, args.steps, args.warmup)
+    rays = sum(st["rays"] for st in sts)
+    eps_l = [std_error(shapes[0][3].tallies(i))[0] for i in range(shapes[0][2])]
+    log(f"c4 batched: {1e3 * wall:.1f} ms/step, {rays / wall / 1e9:.3f} Grays/s, eps (large) {max(eps_l):.4g}")
+    plain = None
+    if not args.no_plain:
+        for _, _, _, s in shapes:
+            s.set_graph(False)
+        pwall, psts = timed(VARIANT_PLAIN, 1, 0)
+        peps = [std_error(shapes[0][3].tallies(i))[0] for i in range(shapes[0][2])]
+        plain = {"ms_per_step": 1e3 * pwall, "rays_per_s": sum(st["rays"] for st in psts) / pwall,
+                 "eps_large_max": max(peps), "time_to_1pct_s": pwall * (max(peps) / 0.01) ** 2}
+        log(f"c4 plain: {1e3 * pwall:.1f} ms/step")
+    t1 = wall * (max(eps_l) / 0.01) ** 2
+    out = {
+        "metric": "rays/s (batched solver) and time to 1% std-error of div q_r; batched vs plain, 1 MI355X",
+        "value": rays / wall, "unit": "rays/s", "n_gpus": 1, "steps": args.steps, "warmup": args.warmup,
+        "ms_per_step": 1e3 * wall, "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 march, fp32 k-tables, fp64 tallies",
+        "data": "synthetic (H2O-like tables, laminar 300-1800 K channel T fields with 5% smoothed noise)",
+        "config": {"workload": "snapshot ensemble: 8 x 64x96x64 + 4 x 256x128x256, photmax 100, nVar %d" % args.nvar,
+                   "rays_per_step": int(rays), "launches_per_step": 2, "graph": not args.no_graph},
+        "eps_large": eps_l, "time_to_1pct_s": t1,
+        "plain": plain,
+        "time_to_1pct_speedup_vs_plain": (plain["time_to_1pct_s"] / t1) if plain else None,
+    }
+    print(json.dumps(out), flush=True)
+    for _, _, _, s in shapes:
+        s.close()
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="nb64", choices=["nb64", "nb128", "big256", "grey32", "mg192"])
+    ap.add_argument("--workload", default="nb64", choices=["nb64", "nb128", "big256", "grey32", "mg192", "c4"])
     ap.add_argument("--nvar", type=int, default=8)
     ap.add_argument("--tables", default="fp32", choices=["bf16", "fp32"],
                     help="k-table storage of the headline (batched) leg; both are measured")
@@ -116,6 +251,8 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-target", action="store_true",
+                    help="skip the 256x128x256 batched-vs-plain leg")
     ap.add_argument("--probe", action="store_true",
                     help="no timing: one elementwise copy of 512 MiB (counter calibration) and one batched "
                          "solve, for hardware-counter runs")
@@ -124,6 +261,9 @@ def main() -> int:
         print("bench.py: this build runs on one GPU; the multi-GPU path (cell slabs per GPU) is not "
               "part of it, so --gpus must be 1", file=sys.stderr)
         return 2
+
+    if args.workload == "c4":
+        return bench_c4(args)
 
     from radchan_amd import VARIANT_BATCHED, VARIANT_PLAIN, Solver, config
     from radchan_amd import fields as F
@@ -211,10 +351,18 @@ def main() -> int:
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "k_batched<false,%d> (persistent march, %s k-tables)" % (1 if bf16 else 0, "BF16" if bf16 else "FP32"),
+            "kernel": "%s<false,%d> (persistent march, %s k-tables)" % (
+                "k_batched_mg" if p.grid_num > 1 else "k_batched", 1 if bf16 else 0, "BF16" if bf16 else "FP32"),
             "kernel_ms": round(march_ms, 4),
             "algorithmic_bytes": int(alg_bytes),
             "bytes_per_step": b_step, "bytes_per_ray": b_ray, "bytes_per_cell": B_CELL}
+
+    scal = eps_scaling(p, tab, T, sched, head["eps"], bf16)
+    log(f"eps at photmax {scal['photmax']}: {scal['eps']:.4g} (ratio {scal['eps_ratio']:.3f}, expected 0.5), "
+        f"{scal['march_ms']:.1f} ms")
+    target = None
+    if not args.no_target and args.workload != "big256" and not p.grey:
+        target = target_leg(lib, tab)
 
     cpu = None
     if not args.no_cpu:
@@ -251,6 +399,8 @@ def main() -> int:
         "plain": plain,
         "speedup_vs_plain": (value / plain["rays_per_s"]) if plain else None,
         "time_to_1pct_speedup_vs_plain": (plain["time_to_1pct_s"] / head["time_to_1pct_s"]) if plain else None,
+        "eps_scaling": scal,
+        "target_256x128x256": target,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

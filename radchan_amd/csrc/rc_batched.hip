@@ -283,9 +283,9 @@ __device__ __forceinline__ void finish_ray(const Slots &sl, int seq, int runit, 
 }
 
 /* LDS words of one wavefront: unit slots, ray queue, (multigrid) the grid
- * levels of the current snapshot, then (srt) the direction uniforms of the
- * unit's rays and its sorted keys (16-bit: keys are below nB nQ <= 65535;
- * at least 256, the register sort's width) */
+ * levels of the current snapshot, then (srt) its sorted keys (16-bit: keys
+ * are below nB nQ <= 65535; at least 256, the register sort's width),
+ * preceded, up to 256 keys, by the direction uniforms of the unit's rays */
 constexpr int LEV_WORDS = (int)(sizeof(DevLevel) / 4);
 
 /* drain of the last rays: at most COOP_RAYS live rays are marched one at a
@@ -320,7 +320,7 @@ __device__ __forceinline__ En bcast_en(const En &a, int src) {
 }
 __host__ __device__ __forceinline__ int batched_lds_words(int n2, int srt, int multi) {
     return SLOT_WORDS * NSLOT + QREC * QCAP + (multi ? LEV_WORDS * RC_MAXLEV : 0) +
-           (srt ? 2 * n2 + (n2 < 256 ? 256 : n2) / 2 : 0);
+           (srt ? (n2 <= 256 ? 2 * n2 + 128 : n2 / 2) : 0);
 }
 
 template <bool SURF, bool MULTI, int TM>
@@ -334,9 +334,13 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
     const int P = cm.photmax;
     int n2 = 1;
     while (n2 < P) n2 <<= 1;
+    /* srt: up to 256 keys the direction uniforms of the unit's rays are
+     * kept from the key draws; above that they are drawn again at the
+     * rays' generation, and only the keys take LDS */
+    const int nkeep = n2 <= 256 ? n2 : 0;
     float *dira = reinterpret_cast<float *>(rq + QREC * QCAP + (MULTI ? LEV_WORDS * RC_MAXLEV : 0));
-    float *dirb = dira + n2;
-    uint16_t *keys = reinterpret_cast<uint16_t *>(dirb + n2);
+    float *dirb = dira + nkeep;
+    uint16_t *keys = reinterpret_cast<uint16_t *>(dirb + nkeep);
     const int lane = threadIdx.x;
     const int per_snap = SURF ? 2 * cm.nsurf : cm.ncell;
     const int total = per_snap * nsnap * cm.nVar;
@@ -458,8 +462,6 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                             if (h < P) {
                                 const U4 q = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)N.v, N.cid, tagx);
                                 key = band_key(cm, sn, SURF, q.c, q.d);
-                                dira[h] = q.a;
-                                dirb[h] = q.b;
                             }
                             keys[h] = (uint16_t)key;
                         }
@@ -523,10 +525,15 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                     const int h = cur_h + lane;
                     float a, b;
                     uint32_t key;
-                    if (cm.srt) {
+                    if (cm.srt && nkeep) {
                         key = (uint32_t)keys[h];
                         a = dira[h];
                         b = dirb[h];
+                    } else if (cm.srt) {
+                        key = (uint32_t)keys[h];
+                        const U4 q = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)U.v, U.cid, U.tag + 16u * (uint32_t)sn.snap);
+                        a = q.a;
+                        b = q.b;
                     } else {
                         const U4 q = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)U.v, U.cid, U.tag + 16u * (uint32_t)sn.snap);
                         key = band_key(cm, sn, SURF, q.c, q.d);

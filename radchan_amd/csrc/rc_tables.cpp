@@ -121,6 +121,8 @@ int check_params(const rc_params &p) {
         if (p.maxs[g] < 1) return fail("maxs must be positive");
     }
     if (p.photmax < 1 || p.nVar < 1) return fail("photmax and nVar must be positive");
+    if (p.srt == 1 && p.photmax > 16384)
+        return fail("srt: photmax above 16384 (the band keys of a cell's rays are sorted in one workgroup's LDS)");
     if (!(p.Lx > 0) || !(p.Ly > 0) || !(p.Lz > 0)) return fail("domain lengths must be positive");
     for (int f = 0; f < 4; ++f)
         if (p.bd[f] < 1 || p.bd[f] > 3) return fail("x and y faces take bd 1 (periodic), 2 (black) or 3 (specular)");
