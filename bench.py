@@ -162,10 +162,10 @@ def target_leg(lib, tab):
 def bench_c4(args) -> int:
     """The snapshot ensemble: 8 snapshots of 64x96x64 and 4 of 256x128x256
     (laminar 300-1800 K profile plus 5% smoothed noise, its own noise seed
-    per snapshot), photmax 100, nVar 8, FP32 k tables.  Each
-    shape is one batched launch over all its snapshots; the two launches go
-    to their own streams and run together.  A step solves all 12 snapshots;
-    eps and the time to 1% are those of the worst large snapshot."""
+    per snapshot), photmax 100, nVar 8, FP32 k tables.  The batched solver
+    runs all 12 snapshots, both grid sizes, in one launch (a group of the
+    two sessions); the plain solver one launch per snapshot and run.  eps
+    and the time to 1% are those of the worst large snapshot."""
     from radchan_amd import VARIANT_BATCHED, VARIANT_PLAIN, Solver, config
     from radchan_amd import fields as F
     from radchan_amd import lib as L
@@ -182,12 +182,14 @@ def bench_c4(args) -> int:
         s.set_fields(Ts)
         shapes.append((name, p, n, s))
         log(f"c4: {n} x {p.imax}x{p.jmax}x{p.kmax}")
-    # schedules: the small shape autotuned on its snapshots; the large one
-    # takes the schedule the autotuner picks for a single 256x128x256 field
-    shapes[1][3].autotune()
-    shapes[0][3].set_schedule(32, 8, 4)
+    from radchan_amd import Group
+    group = Group([s for _, _, _, s in shapes])
+    # the schedule the autotuner picks for a single 256x128x256 field
+    group.set_schedule(32, 8, 4)
 
     def step(variant):
+        if variant == VARIANT_BATCHED:
+            return [group.solve()]
         for _, _, _, s in shapes:
             s.run(variant)
         return [s.sync() for _, _, _, s in shapes]
@@ -202,16 +204,12 @@ def bench_c4(args) -> int:
         lib.rc_device_synchronize()
         return (time.perf_counter() - t0) / steps, sts
 
-    for _, _, _, s in shapes:
-        s.set_graph(not args.no_graph)
     wall, sts = timed(VARIANT_BATCHED, args.steps, args.warmup)
     rays = sum(st["rays"] for st in sts)
     eps_l = [std_error(shapes[0][3].tallies(i))[0] for i in range(shapes[0][2])]
     log(f"c4 batched: {1e3 * wall:.1f} ms/step, {rays / wall / 1e9:.3f} Grays/s, eps (large) {max(eps_l):.4g}")
     plain = None
     if not args.no_plain:
-        for _, _, _, s in shapes:
-            s.set_graph(False)
         pwall, psts = timed(VARIANT_PLAIN, 1, 0)
         peps = [std_error(shapes[0][3].tallies(i))[0] for i in range(shapes[0][2])]
         plain = {"ms_per_step": 1e3 * pwall, "rays_per_s": sum(st["rays"] for st in psts) / pwall,
@@ -225,12 +223,14 @@ def bench_c4(args) -> int:
         "dtype": "fp32 march, fp32 k-tables, fp64 tallies",
         "data": "synthetic (H2O-like tables, laminar 300-1800 K channel T fields with 5% smoothed noise)",
         "config": {"workload": "snapshot ensemble: 8 x 64x96x64 + 4 x 256x128x256, photmax 100, nVar %d" % args.nvar,
-                   "rays_per_step": int(rays), "launches_per_step": 2, "graph": not args.no_graph},
+                   "rays_per_step": int(rays), "batched_launches_per_step": 1,
+                   "schedule": {"wg_per_cu": 32, "threshold": 8, "steps_per_iter": 4}},
         "eps_large": eps_l, "time_to_1pct_s": t1,
         "plain": plain,
         "time_to_1pct_speedup_vs_plain": (plain["time_to_1pct_s"] / t1) if plain else None,
     }
     print(json.dumps(out), flush=True)
+    group.close()
     for _, _, _, s in shapes:
         s.close()
     return 0

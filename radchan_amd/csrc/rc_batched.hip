@@ -112,31 +112,36 @@ struct Unit {
     int skip;             /* wall unit of a non-adiabatic wall           */
 };
 
+/* unit u of the launch: (cell, run) of the snapshot whose range of
+ * emitting (wall) cells holds cell u / nVar; every grid size from the
+ * snapshot */
 template <bool SURF>
-__device__ __forceinline__ Unit decode(const DevCommon &cm, const DevSnap *__restrict__ snaps, int u, int per_snap) {
+__device__ __forceinline__ Unit decode(const DevCommon &cm, const DevSnap *__restrict__ snaps, int nsnap, int u) {
     Unit U;
     U.v = u % cm.nVar;
     const int cs = u / cm.nVar;
-    U.s = cs / per_snap;
-    U.n = cs % per_snap;
-    const DevSnap &sn = snaps[U.s];
+    int s = 0;
+    while (s + 1 < nsnap && cs >= (SURF ? snaps[s + 1].sbase : snaps[s + 1].ubase)) ++s;
+    const DevSnap &sn = snaps[s];
+    U.s = s;
+    U.n = cs - (SURF ? sn.sbase : sn.ubase);
     U.skip = 0;
     if (!SURF) {
-        cell_ijk(cm, U.n, U.i, U.j, U.k);
+        cell_ijk(sn, U.n, U.i, U.j, U.k);
         U.bound = 0;
-        U.cid = (uint32_t)U.n + (uint32_t)cm.imax * cm.jmax * (cm.zstart - 1);
+        U.cid = (uint32_t)U.n + (uint32_t)sn.imax * sn.jmax * (sn.zstart - 1);
         U.tag = TAG_FLUID;
-        U.Ti = cell_T(cm, sn, U.i, U.j, U.k);
+        U.Ti = cell_T(sn, sn, U.i, U.j, U.k);
     } else {
-        U.bound = U.n / cm.nsurf;
-        const int m = U.n % cm.nsurf;
+        U.bound = U.n / sn.nsurf;
+        const int m = U.n % sn.nsurf;
         U.i = 0;
-        U.j = m % cm.jmax + 1;
-        U.k = m / cm.jmax + cm.zstart;
-        U.cid = (uint32_t)m + (uint32_t)cm.jmax * (cm.zstart - 1);
+        U.j = m % sn.jmax + 1;
+        U.k = m / sn.jmax + sn.zstart;
+        U.cid = (uint32_t)m + (uint32_t)sn.jmax * (sn.zstart - 1);
         U.tag = U.bound ? TAG_EAST : TAG_WEST;
         U.skip = (U.bound == 0 && !cm.adiab[0]) || (U.bound == 1 && !cm.adiab[1]);
-        U.Ti = U.skip ? 0.f : wall_T(cm, sn, U.bound, U.j, U.k);
+        U.Ti = U.skip ? 0.f : wall_T(sn, sn, U.bound, U.j, U.k);
     }
     return U;
 }
@@ -342,12 +347,16 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
     float *dirb = dira + nkeep;
     uint16_t *keys = reinterpret_cast<uint16_t *>(dirb + nkeep);
     const int lane = threadIdx.x;
-    const int per_snap = SURF ? 2 * cm.nsurf : cm.ncell;
-    const int total = per_snap * nsnap * cm.nVar;
+    const DevSnap &slast = snaps[nsnap - 1];
+    const int total = (SURF ? slast.sbase + 2 * slast.nsurf : slast.ubase + slast.ncell) * cm.nVar;
     const float tol = SURF ? cm.tollS : cm.toll;
     /* the drain march reads the x faces of every level from LDS */
     int nxf = 0;
-    for (int g = 0; g < (MULTI ? cm.grid_num : 1); ++g) nxf += snaps[0].lev[g].im + 1;
+    for (int s = 0; s < nsnap; ++s) {
+        int n = 0;
+        for (int g = 0; g < (MULTI ? cm.grid_num : 1); ++g) n += snaps[s].lev[g].im + 1;
+        nxf = max(nxf, n);
+    }
     const bool xfit = uni(nxf) <= QREC * QCAP;
 
     __syncthreads();
@@ -417,7 +426,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                 qdone = 1;
             } else {
                 nxt = u;
-                const Unit N = uni(decode<SURF>(cm, snaps, u, per_snap));
+                const Unit N = uni(decode<SURF>(cm, snaps, nsnap, u));
                 nxt_s = N.s;
                 if (cm.srt && !N.skip) {
                     const DevSnap &sn = snaps[N.s];
@@ -477,7 +486,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
             /* (b) start the prepared unit (a new snapshot only once every
              * lane and the queue are empty; never over the slot of a ray
              * still queued) */
-            U = uni(decode<SURF>(cm, snaps, nxt, per_snap));
+            U = uni(decode<SURF>(cm, snaps, nsnap, nxt));
             nxt = -1;
             if (MULTI && U.s != snap) {
                 const uint32_t *src = reinterpret_cast<const uint32_t *>(snaps[U.s].lev);
@@ -485,7 +494,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                     reinterpret_cast<uint32_t *>(llev)[w] = gld(src, w);
             }
             snap = U.s;
-            const int tidx = (U.s * per_snap + U.n) * cm.nVar + U.v;
+            const int tidx = ((SURF ? snaps[U.s].sbase : snaps[U.s].ubase) + U.n) * cm.nVar + U.v;
             if (!U.skip) {
                 const int cslot = seq & (NSLOT - 1);
                 cunit = seq;
@@ -502,7 +511,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                     t.done = 0;
                     t.tid = tidx;
                     SlotPlace &pl = sl.pl[cslot];
-                    pl.ic = SURF ? (U.bound ? cm.imax : 1) : U.i;
+                    pl.ic = SURF ? (U.bound ? sn.imax : 1) : U.i;
                     pl.jc = U.j;
                     pl.kc = U.k;
                     pl.xp = SURF ? (U.bound ? cm.Lx : 0.f) : gld(sn.x0, U.i);
@@ -751,8 +760,9 @@ void launch_zero(hipStream_t st, double *p, size_t n) {
     hipLaunchKernelGGL(k_zero_f64, dim3(blocks), dim3(256), 0, st, p, n);
 }
 
-void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int nsnap, int surface, int snaps_bf16,
-                    double *de, unsigned long long *stats, int *queue, int wg_per_cu, int threshold, int spi) {
+void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int nsnap, int nunit_cells, int surface,
+                    int snaps_bf16, double *de, unsigned long long *stats, int *queue, int wg_per_cu, int threshold,
+                    int spi) {
     static int ncu = 0;       /* compute units of the (single) device */
     if (ncu <= 0) {
         int dev = 0;
@@ -766,8 +776,7 @@ void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, i
     int n2 = 1;
     while (n2 < cm.photmax) n2 <<= 1;
     const size_t lds = (size_t)batched_lds_words(n2, cm.srt, cm.grid_num > 1) * 4;
-    const int per_snap = surface ? 2 * cm.nsurf : cm.ncell;
-    const size_t nde = (size_t)per_snap * nsnap * cm.nVar;
+    const size_t nde = (size_t)nunit_cells * cm.nVar;
     const long total = (long)nde;
     const int grid = (int)std::max<long>(1, std::min<long>((long)ncu * wg_per_cu, total));
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, queue, QPARTS);

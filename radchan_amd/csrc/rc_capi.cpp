@@ -23,8 +23,9 @@
 #include "rc_kernels.h"
 
 namespace rc {
-void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int nsnap, int surface, int snaps_bf16,
-                    double *de, unsigned long long *stats, int *queue, int wg_per_cu, int threshold, int spi);
+void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int nsnap, int nunit_cells, int surface,
+                    int snaps_bf16, double *de, unsigned long long *stats, int *queue, int wg_per_cu, int threshold,
+                    int spi);
 }
 
 using namespace rc;
@@ -113,6 +114,7 @@ struct rc_ctx {
     std::vector<Grid> grids;
     DevBuf xu[RC_MAXLEV], x0, y0, z0;
     std::vector<std::unique_ptr<SnapState>> snaps;
+    std::vector<DevSnap> hsnap;             /* host copy of the descriptors */
     DevBuf dsnap;
     DevBuf de, ds, mean, sdev, smean, ssdev;
     DevBuf keys, stats, queue;
@@ -293,6 +295,9 @@ int push_snaps(rc_ctx *c) {
         d.guide_b = S.guide_b.as<int>(); d.guide_bs = S.guide_bs.as<int>(); d.gM = kGuideM;
         d.T0 = S.T0.as<float>();
         d.x0 = c->x0.as<float>(); d.y0 = c->y0.as<float>(); d.z0 = c->z0.as<float>();
+        d.imax = p.imax; d.jmax = p.jmax; d.kmax = p.kmax; d.zstart = c->zstart;
+        d.ncell = c->ncell; d.nsurf = c->nsurf;
+        d.ubase = (int)s * c->ncell; d.sbase = (int)s * 2 * c->nsurf;
         for (int g = 0; g < p.grid_num; ++g) {
             DevLevel &L = d.lev[g];
             L.im = p.maxi[g]; L.jm = p.maxj[g]; L.km = p.maxk[g]; L.maxs = p.maxs[g];
@@ -306,7 +311,8 @@ int push_snaps(rc_ctx *c) {
         }
         v[s] = d;
     }
-    return upload(c->dsnap, v, c->stream);
+    c->hsnap = v;
+    return upload(c->dsnap, c->hsnap, c->stream);
 }
 
 }  // namespace
@@ -548,12 +554,13 @@ static int enqueue_solve(rc_ctx *c, int variant, int *launches, bool timed) {
             }
         }
     } else {
-        launch_batched(c->stream, cm, ds, nsnap, 0, c->table_bf16, c->de.as<double>(), c->stats.as<unsigned long long>(),
-                       c->queue.as<int>(), c->wg_per_cu, c->threshold, c->spi);
+        launch_batched(c->stream, cm, ds, nsnap, nsnap * c->ncell, 0, c->table_bf16, c->de.as<double>(),
+                       c->stats.as<unsigned long long>(), c->queue.as<int>(), c->wg_per_cu, c->threshold, c->spi);
         n += 3;
         if (walls) {
-            launch_batched(c->stream, cm, ds, nsnap, 1, c->table_bf16, c->ds.as<double>(), c->stats.as<unsigned long long>(),
-                           c->queue.as<int>() + 16, c->wg_per_cu, c->threshold, c->spi);
+            launch_batched(c->stream, cm, ds, nsnap, nsnap * 2 * c->nsurf, 1, c->table_bf16, c->ds.as<double>(),
+                           c->stats.as<unsigned long long>(), c->queue.as<int>() + 16, c->wg_per_cu, c->threshold,
+                           c->spi);
             n += 3;
         }
     }
@@ -797,6 +804,154 @@ int rc_trace(rc_ctx *c, int surface, const int32_t *rays, int n, rc_ray *out) {
 
 /* ------------------------------------------------------------------------- */
 /* reference entry points                                                     */
+
+/* ---- groups: snapshots of several sessions in one batched launch ------- */
+
+struct rc_group {
+    std::vector<rc_ctx *> cs;
+    hipStream_t stream = nullptr;
+    hipEvent_t ev0 = nullptr, ev1 = nullptr, evm0 = nullptr, evm1 = nullptr;
+    std::vector<DevSnap> h;
+    DevBuf dsnap, de, stats, queue;
+    int wg_per_cu = 0, threshold = 0, spi = 0;
+    rc_stats last{};
+    bool launched = false;
+};
+
+/* the parameters one batched launch shares: everything in DevCommon that
+ * the batched kernel reads (the grid sizes travel with each snapshot) */
+static int group_compatible(const rc_ctx *a, const rc_ctx *b) {
+    const DevCommon x = make_common(a), y = make_common(b);
+    bool ok = x.nVar == y.nVar && x.photmax == y.photmax && x.srt == y.srt && x.nB == y.nB && x.nQ == y.nQ &&
+              x.nT == y.nT && x.grid_num == y.grid_num && x.Lx == y.Lx && x.Ly == y.Ly && x.Lz == y.Lz &&
+              x.toll == y.toll && x.tollS == y.tollS && x.k0 == y.k0 && x.k1 == y.k1 &&
+              x.adiab[0] == y.adiab[0] && x.adiab[1] == y.adiab[1] && a->p.grey == b->p.grey &&
+              a->table_bf16 == b->table_bf16;
+    for (int f = 0; f < 6; ++f) ok = ok && x.bd[f] == y.bd[f];
+    return ok ? 0 : -1;
+}
+
+rc_group *rc_group_create(rc_ctx *const *cs, int n) {
+    if (!cs || n < 1) { set_error("rc_group_create: no sessions"); return nullptr; }
+    for (int i = 0; i < n; ++i) {
+        if (!cs[i]) { set_error("rc_group_create: null session"); return nullptr; }
+        if (group_compatible(cs[0], cs[i])) {
+            set_error("rc_group_create: the sessions differ in a parameter other than the grid (nVar, photmax, "
+                      "srt, table sizes, tolerances, boundary codes, domain lengths, seed or table precision)");
+            return nullptr;
+        }
+    }
+    auto *g = new rc_group();
+    g->cs.assign(cs, cs + n);
+    if (hipStreamCreateWithFlags(&g->stream, hipStreamNonBlocking) != hipSuccess ||
+        hipEventCreate(&g->ev0) != hipSuccess || hipEventCreate(&g->ev1) != hipSuccess ||
+        hipEventCreate(&g->evm0) != hipSuccess || hipEventCreate(&g->evm1) != hipSuccess) {
+        set_error("rc_group_create: stream or event creation failed");
+        rc_group_destroy(g);
+        return nullptr;
+    }
+    return g;
+}
+
+void rc_group_destroy(rc_group *g) {
+    if (!g) return;
+    if (g->stream) (void)hipStreamSynchronize(g->stream);
+    for (hipEvent_t e : {g->ev0, g->ev1, g->evm0, g->evm1})
+        if (e) (void)hipEventDestroy(e);
+    if (g->stream) (void)hipStreamDestroy(g->stream);
+    delete g;
+}
+
+int rc_group_set_schedule(rc_group *g, int wg_per_cu, int threshold, int steps_per_iter) {
+    if (!g) { set_error("null group"); return -1; }
+    if (wg_per_cu < 0 || wg_per_cu > 64 || threshold < 0 || threshold > 64 || steps_per_iter < 0 || steps_per_iter > 8) {
+        set_error("schedule out of range (wg_per_cu 0..64, threshold 0..64, steps_per_iter 0..8)");
+        return -2;
+    }
+    g->wg_per_cu = wg_per_cu; g->threshold = threshold; g->spi = steps_per_iter;
+    return 0;
+}
+
+int rc_group_run(rc_group *g) {
+    if (!g) { set_error("null group"); return -1; }
+    /* the sessions' own queued work on the buffers the group writes */
+    for (rc_ctx *c : g->cs) {
+        if (c->snaps.empty()) { set_error("rc_group_run: a session has no fields"); return -2; }
+        if (c->p.calcwalls == 1 && (c->p.adiabw == 1 || c->p.adiabe == 1)) {
+            set_error("rc_group_run: wall fluxes (calcwalls) are solved per session; run such sessions alone");
+            return -3;
+        }
+        if (group_compatible(g->cs[0], c)) { set_error("rc_group_run: sessions no longer compatible"); return -4; }
+        RC_HIP(hipStreamSynchronize(c->stream));
+    }
+    rc_ctx *c0 = g->cs[0];
+    const int nVar = c0->p.nVar;
+    g->h.clear();
+    int cells = 0;
+    for (rc_ctx *c : g->cs)
+        for (const DevSnap &d0 : c->hsnap) {
+            DevSnap d = d0;
+            d.ubase = cells;
+            d.sbase = 0;
+            cells += c->ncell;
+            g->h.push_back(d);
+        }
+    if (int e = upload(g->dsnap, g->h, g->stream)) return e;
+    if (int e = g->de.ensure((size_t)cells * nVar * 8)) return e;
+    if (int e = g->stats.ensure(8 * sizeof(unsigned long long))) return e;
+    if (int e = g->queue.ensure(32 * sizeof(int))) return e;
+    const DevCommon cm = make_common(c0);
+    int n = 0;
+    RC_HIP(hipEventRecord(g->ev0, g->stream));
+    launch_zero_u64(g->stream, g->stats.as<unsigned long long>(), 8);
+    ++n;
+    RC_HIP(hipEventRecord(g->evm0, g->stream));
+    launch_batched(g->stream, cm, g->dsnap.as<DevSnap>(), (int)g->h.size(), cells, 0, c0->table_bf16,
+                   g->de.as<double>(), g->stats.as<unsigned long long>(), g->queue.as<int>(), g->wg_per_cu,
+                   g->threshold, g->spi);
+    n += 3;
+    RC_HIP(hipEventRecord(g->evm1, g->stream));
+    /* each session's tallies, means and deviations into its own buffers */
+    size_t off = 0;
+    for (rc_ctx *c : g->cs) {
+        const size_t nde = (size_t)c->ncell * nVar;
+        for (size_t s = 0; s < c->snaps.size(); ++s) {
+            RC_HIP(hipMemcpyAsync(c->de.as<double>() + s * nde, g->de.as<double>() + off, nde * 8,
+                                  hipMemcpyDeviceToDevice, g->stream));
+            launch_finalize(g->stream, c->de.as<double>() + s * nde, c->ncell, nVar,
+                            c->mean.as<float>() + s * c->ncell, c->sdev.as<float>() + s * c->ncell);
+            ++n;
+            off += nde;
+        }
+    }
+    RC_HIP(hipEventRecord(g->ev1, g->stream));
+    RC_HIP(hipGetLastError());
+    g->last = rc_stats{};
+    g->last.launches = n;
+    g->last.variant = RC_VARIANT_BATCHED;
+    g->launched = true;
+    return 0;
+}
+
+int rc_group_sync(rc_group *g, rc_stats *st) {
+    if (!g) { set_error("null group"); return -1; }
+    if (g->launched) {
+        RC_HIP(hipStreamSynchronize(g->stream));
+        float ms = 0.f, mms = 0.f;
+        RC_HIP(hipEventElapsedTime(&ms, g->ev0, g->ev1));
+        RC_HIP(hipEventElapsedTime(&mms, g->evm0, g->evm1));
+        unsigned long long h[8];
+        RC_HIP(hipMemcpy(h, g->stats.p, sizeof h, hipMemcpyDeviceToHost));
+        g->last.kernel_ms = ms;
+        g->last.march_ms = mms;
+        g->last.rays = (int64_t)h[0];
+        g->last.steps = (int64_t)h[1];
+        g->last.capped = (int64_t)h[2];
+        g->launched = false;
+    }
+    if (st) *st = g->last;
+    return 0;
+}
 
 static std::mutex g_mu;
 static bool g_params_set = false;

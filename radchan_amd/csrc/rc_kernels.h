@@ -58,6 +58,13 @@ struct DevSnap {
     int gM, pad2;
     const float *T0;               /* ghost-padded input field            */
     const float *x0, *y0, *z0;     /* level-0 cell centres                */
+    /* the snapshot's grid (snapshots of one batched launch may differ in
+     * it): global sizes, first z plane and counts of emitting cells and
+     * wall cells per wall; ubase / sbase: the launch's emitting (wall)
+     * cells before this snapshot, so its (cell, run) tallies start at
+     * ubase nVar (sbase nVar) */
+    int imax, jmax, kmax, zstart;
+    int ncell, nsurf, ubase, sbase;
     DevLevel lev[RC_MAXLEV];
 };
 

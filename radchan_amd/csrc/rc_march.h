@@ -578,7 +578,8 @@ __device__ __forceinline__ void energy(const DevCommon &cm, const DevSnap &sn, u
 }
 
 /* emitting fluid cell n -> (i, j, k global) */
-__device__ __forceinline__ void cell_ijk(const DevCommon &cm, int n, int &i, int &j, int &k) {
+template <class D>
+__device__ __forceinline__ void cell_ijk(const D &cm, int n, int &i, int &j, int &k) {
     i = n % cm.imax + 1;
     j = (n / cm.imax) % cm.jmax + 1;
     k = n / (cm.imax * cm.jmax) + cm.zstart;
@@ -698,10 +699,13 @@ __device__ __forceinline__ float surface_ray(const DevCommon &cm, const DevSnap 
     return de;
 }
 
-__device__ __forceinline__ float cell_T(const DevCommon &cm, const DevSnap &sn, int i, int j, int k) {
+/* grid sizes from D: the solve-wide DevCommon, or the snapshot's own */
+template <class D>
+__device__ __forceinline__ float cell_T(const D &cm, const DevSnap &sn, int i, int j, int k) {
     return gld(sn.T0, (size_t)i + (size_t)(cm.imax + 2) * ((size_t)j + (size_t)(cm.jmax + 2) * (size_t)k));
 }
-__device__ __forceinline__ float wall_T(const DevCommon &cm, const DevSnap &sn, int bound, int j, int k) {
+template <class D>
+__device__ __forceinline__ float wall_T(const D &cm, const DevSnap &sn, int bound, int j, int k) {
     const size_t row = (size_t)(cm.imax + 2) * ((size_t)j + (size_t)(cm.jmax + 2) * (size_t)k);
     /* texel coordinate 1.0 (imax+1.0): halfway between ghost and first cell */
     return bound ? lerp(gld(sn.T0, row + cm.imax), gld(sn.T0, row + cm.imax + 1), 0.5f)

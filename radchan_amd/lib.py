@@ -135,6 +135,11 @@ def load() -> C.CDLL:
         "rc_set_graph": (C.c_int, [vp, C.c_int]),
         "rc_set_table_precision": (C.c_int, [vp, C.c_int]),
         "rc_autotune": (C.c_int, [vp, P(C.c_int32), P(C.c_int32), P(C.c_int32), P(C.c_float)]),
+        "rc_group_create": (vp, [P(vp), C.c_int]),
+        "rc_group_destroy": (None, [vp]),
+        "rc_group_set_schedule": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
+        "rc_group_run": (C.c_int, [vp]),
+        "rc_group_sync": (C.c_int, [vp, P(Stats)]),
         "rc_device_synchronize": (C.c_int, []),
     }
     for name, (res, args) in sig.items():

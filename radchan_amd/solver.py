@@ -143,3 +143,43 @@ def get_results(res: np.ndarray, var: np.ndarray, rank: int = 0) -> None:
     """GET_RESULTS(qr, vr, nrank): wait and scatter into (imax+2)(jmax+2)(kloc+2) arrays."""
     r = C.c_int(rank)
     L.load().get_results_(L.fptr(res), L.fptr(var), C.byref(r))
+
+
+class Group:
+    """Snapshots of several sessions (grids of different sizes) solved by
+    one batched launch (rc_group_*); the results land in the sessions."""
+
+    def __init__(self, solvers: Sequence[Solver]):
+        self.lib = L.load()
+        self.solvers = list(solvers)
+        arr = (C.c_void_p * len(self.solvers))(*[s.ctx for s in self.solvers])
+        self.g = self.lib.rc_group_create(arr, len(self.solvers))
+        if not self.g:
+            raise RuntimeError("rc_group_create failed: " + L.last_error())
+
+    def close(self) -> None:
+        if self.g:
+            self.lib.rc_group_destroy(self.g)
+            self.g = None
+
+    def __del__(self):
+        try:
+            self.close()
+        except Exception:
+            pass
+
+    def set_schedule(self, wg_per_cu: int = 0, threshold: int = 0, steps_per_iter: int = 0) -> None:
+        L.check(self.lib.rc_group_set_schedule(self.g, wg_per_cu, threshold, steps_per_iter), "rc_group_set_schedule")
+
+    def run(self) -> None:
+        L.check(self.lib.rc_group_run(self.g), "rc_group_run")
+
+    def sync(self) -> dict:
+        st = L.Stats()
+        L.check(self.lib.rc_group_sync(self.g, C.byref(st)), "rc_group_sync")
+        return {k: getattr(st, k) for k, _ in L.Stats._fields_}
+
+    def solve(self) -> dict:
+        self.run()
+        return self.sync()
+

@@ -159,6 +159,48 @@ def test_snapshot_batch_one_launch(nb_tables):
     s.close()
 
 
+def test_group_of_grids_one_launch(nb_tables):
+    """Snapshots of two sessions with different grids (12^3 with two
+    snapshots, 16x8x20 with one) in one batched launch: every snapshot's
+    tallies equal the oracle's on its own grid and Philox streams, and the
+    group's step count is the sum of the oracle's."""
+    from radchan_amd import fields as F
+    from radchan_amd import params as P
+    from radchan_amd.solver import Group
+    p1, tab, _ = cases.nb_case(nb_tables, n=12, photmax=48, nVar=2)
+    T1 = [F.channel_field(p1, 300.0, 1500.0 - 300.0 * q, noise=0.03, seed=60 + q) for q in range(2)]
+    p2 = P.base_params(16, 8, 20, photmax=48, nVar=2, fact_mesh=1.2, seed=p1.seed, srt=1)
+    T2 = [F.channel_field(p2, 300.0, 1400.0, noise=0.03, seed=70)]
+    print("group: 2 x 12^3 + 1 x 16x8x20", flush=True)
+    s1, s2 = Solver(p1, tab), Solver(p2, tab)
+    s1.set_fields(T1)
+    s2.set_fields(T2)
+    g = Group([s1, s2])
+    g.set_schedule(4, 8, 2)
+    st = g.solve()
+    total = 0
+    for s, p, Ts in ((s1, p1, T1), (s2, p2, T2)):
+        for q, T in enumerate(Ts):
+            o = O.Oracle(p, tab, T, snapshot=q)
+            de32, de32d, de64, steps = o.solve(O.all_cells(p))
+            got = s.tallies(q)
+            assert np.abs(got - de32d).max() <= 1e-12 * np.abs(de32d).max(), (p.imax, q)
+            total += int(steps.sum())
+            m, _ = s.results(q)
+            assert np.allclose(m.reshape(-1), got.mean(axis=1), rtol=1e-6, atol=0)
+    assert st["steps"] == total
+    assert st["rays"] == (p1.imax * p1.jmax * p1.kmax * 2 + p2.imax * p2.jmax * p2.kmax) * 48 * 2
+    g.close()
+    # sessions that differ in more than the grid are refused
+    p3 = P.base_params(8, 8, 8, photmax=32, nVar=2, fact_mesh=1.5, seed=p1.seed, srt=1)
+    s3 = Solver(p3, tab)
+    s3.set_fields([F.channel_field(p3, 300.0, 1500.0, noise=0.02, seed=1)])
+    with pytest.raises(RuntimeError):
+        Group([s1, s3])
+    for s in (s1, s2, s3):
+        s.close()
+
+
 def test_results_are_mean_and_reference_sdev(nb_tables):
     """mean over runs, and sqrt of sum (x-m)^2 / ((n-1/2)(n-1)) as the
     reference reports it (mc_gpu_noint.cu:234-239, :984)."""
