@@ -229,7 +229,10 @@ __device__ __forceinline__ bool geo_core_fx(const DevLevel &L, Ray &r, float fx,
     const float dsx = (fx - r.xp) * r.isx;
     const float dsy = (fy - r.yp) * r.isy;
     const float dsz = (fz - r.zp) * r.isz;
-    cell = min((uint32_t)(r.ic - 1) + (uint32_t)L.im * ((uint32_t)(r.jc - 1) + (uint32_t)L.jm * (uint32_t)(r.kc - 1)),
+    /* 24-bit products (full rate): the factors are below 2^24 for every
+     * grid whose jm km and im are, and the product is the cell index */
+    cell = min((uint32_t)(r.ic - 1) + __umul24((uint32_t)L.im, (uint32_t)(r.jc - 1) +
+                                                                  __umul24((uint32_t)L.jm, (uint32_t)(r.kc - 1))),
                (uint32_t)L.ncell - 1u);
     ds = dsx;
     ds = (ds > dsy) ? dsy : ds;

@@ -121,6 +121,8 @@ int check_params(const rc_params &p) {
         if (p.maxs[g] < 1) return fail("maxs must be positive");
     }
     if (p.photmax < 1 || p.nVar < 1) return fail("photmax and nVar must be positive");
+    if ((int64_t)p.jmax * p.kmax >= (1 << 24) || (int64_t)p.imax * p.jmax * p.kmax >= ((int64_t)1 << 31))
+        return fail("grid too large: jmax kmax must stay below 2^24 and imax jmax kmax below 2^31 (cell indices)");
     if (p.srt == 1 && p.photmax > 16384)
         return fail("srt: photmax above 16384 (the band keys of a cell's rays are sorted in one workgroup's LDS)");
     if (!(p.Lx > 0) || !(p.Ly > 0) || !(p.Lz > 0)) return fail("domain lengths must be positive");
