@@ -578,87 +578,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                 qcount -= npop;
             }
             const bool drained = qdone && nxt < 0 && cur_h >= P && qcount == 0;
-            if (drained && (__ballot(active) == 0ull || xfit)) {
-                /* (f) no ray left to start: the rays still live here are
-                 * the longest ones.  Multigrid: every live lane marches
-                 * its ray to the end with the look-ahead march (the step
-                 * caps of the levels keep the rays short).  Single grid:
-                 * while more than COOP_RAYS lanes are live, every live lane
-                 * marches its ray DRAIN_BLOCK steps with the look-ahead
-                 * march; below that, the wavefront marches the remaining
-                 * rays one at a time, all lanes on one ray.  Then the
-                 * wavefront is done. */
-                if (__ballot(active) != 0ull) {
-                    const SnapView sv = snap_view(snaps + snap);
-                    DevLevel l0;
-                    if (!MULTI) l0 = level_view(&snaps[snap].lev[0]);
-                    /* the x faces of every level to LDS (the ray queue's
-                     * space, empty now), level g from xoff(g) */
-                    float *xul = reinterpret_cast<float *>(rq);
-                    int off = 0;
-                    for (int g = 0; g < (MULTI ? cm.grid_num : 1); ++g) {
-                        const DevLevel Lg = MULTI ? llev[g] : l0;
-                        for (int w = lane; w <= Lg.im; w += 64) xul[off + w] = gld(Lg.xu, w);
-                        off += Lg.im + 1;
-                    }
-                    __syncthreads();
-                    if (MULTI) {
-                        if (active) {
-                            /* the lane's ray may be on a coarser level already */
-                            int xoff = 0;
-                            for (int g = 0; g < lev; ++g) xoff += llev[g].im + 1;
-                            for (;;) {
-                                const DevLevel L = llev[lev];
-                                march_ahead<TM, true>(cm, sv, L, r, e, tol, dacc, counter, nsteps, xul + xoff, L.maxs);
-                                if (!(r.tra > tol) || lev == cm.grid_num - 1) break;
-                                /* coarsen the indices for the next grid level */
-                                const DevLevel Nl = llev[lev + 1];
-                                const int ri = L.im / Nl.im, rj = L.jm / Nl.jm, rk = L.km / Nl.km;
-                                r.ic = (r.ic + ri - 1) / ri;
-                                r.jc = (r.jc + rj - 1) / rj;
-                                r.kc = (r.kc + rk - 1) / rk;
-                                xoff += L.im + 1;
-                                lev += 1;
-                                counter = 0;
-                            }
-                            finish_ray(sl, seq, runit, rtid, dacc, de);
-                            nrays += 1;
-                            active = false;
-                        }
-                    } else {
-                        for (;;) {
-                            const unsigned long long act = __ballot(active);
-                            if (act == 0ull) break;
-                            if (__popcll(act) > COOP_RAYS) {
-                                if (active) {
-                                    march_ahead<TM, true>(cm, sv, l0, r, e, tol, dacc, counter, nsteps, xul,
-                                                          min(l0.maxs, counter + DRAIN_BLOCK));
-                                    if (!(r.tra > tol) || counter >= l0.maxs) {
-                                        finish_ray(sl, seq, runit, rtid, dacc, de);
-                                        nrays += 1;
-                                        active = false;
-                                    }
-                                }
-                            } else {
-                                /* the lowest live lane's ray, by every lane */
-                                const int src = __builtin_ctzll(act);
-                                Ray cr = bcast_ray(r, src);
-                                const En ce = bcast_en(e, src);
-                                float cd = bcast(dacc, src);
-                                int cc = bcast(counter, src), cs = 0;
-                                march_coop<TM>(cm, sv, l0, cr, ce, tol, cd, cc, cs, xul, lane);
-                                if (lane == src) {
-                                    finish_ray(sl, seq, runit, rtid, cd, de);
-                                    nsteps += cs;
-                                    nrays += 1;
-                                    active = false;
-                                }
-                            }
-                        }
-                    }
-                }
-                break;
-            }
+            if (drained && (__ballot(active) == 0ull || xfit)) break;   /* to the drain (f) */
             if (__ballot(active) != 0ull) {
                 /* (e) march steps of every active lane; the snapshot's
                  * fields (and the level, single grid) are wavefront-uniform
@@ -697,6 +617,84 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                         r.kc = (r.kc + rk - 1) / rk;
                         lev += 1;
                         counter = 0;
+                    }
+                }
+            }
+        }
+    }
+    /* (f) no ray left to start: the rays still live here are
+     * the longest ones.  Multigrid: every live lane marches
+     * its ray to the end with the look-ahead march (the step
+     * caps of the levels keep the rays short).  Single grid:
+     * while more than COOP_RAYS lanes are live, every live lane
+     * marches its ray DRAIN_BLOCK steps with the look-ahead
+     * march; below that, the wavefront marches the remaining
+     * rays one at a time, all lanes on one ray.  Then the
+     * wavefront is done. */
+    if (__ballot(active) != 0ull) {
+        const SnapView sv = snap_view(snaps + snap);
+        DevLevel l0;
+        if (!MULTI) l0 = level_view(&snaps[snap].lev[0]);
+        /* the x faces of every level to LDS (the ray queue's
+         * space, empty now), level g from xoff(g) */
+        float *xul = reinterpret_cast<float *>(rq);
+        int off = 0;
+        for (int g = 0; g < (MULTI ? cm.grid_num : 1); ++g) {
+            const DevLevel Lg = MULTI ? llev[g] : l0;
+            for (int w = lane; w <= Lg.im; w += 64) xul[off + w] = gld(Lg.xu, w);
+            off += Lg.im + 1;
+        }
+        __syncthreads();
+        if (MULTI) {
+            if (active) {
+                /* the lane's ray may be on a coarser level already */
+                int xoff = 0;
+                for (int g = 0; g < lev; ++g) xoff += llev[g].im + 1;
+                for (;;) {
+                    const DevLevel L = llev[lev];
+                    march_ahead<TM, true>(cm, sv, L, r, e, tol, dacc, counter, nsteps, xul + xoff, L.maxs);
+                    if (!(r.tra > tol) || lev == cm.grid_num - 1) break;
+                    /* coarsen the indices for the next grid level */
+                    const DevLevel Nl = llev[lev + 1];
+                    const int ri = L.im / Nl.im, rj = L.jm / Nl.jm, rk = L.km / Nl.km;
+                    r.ic = (r.ic + ri - 1) / ri;
+                    r.jc = (r.jc + rj - 1) / rj;
+                    r.kc = (r.kc + rk - 1) / rk;
+                    xoff += L.im + 1;
+                    lev += 1;
+                    counter = 0;
+                }
+                finish_ray(sl, seq, runit, rtid, dacc, de);
+                nrays += 1;
+                active = false;
+            }
+        } else {
+            for (;;) {
+                const unsigned long long act = __ballot(active);
+                if (act == 0ull) break;
+                if (__popcll(act) > COOP_RAYS) {
+                    if (active) {
+                        march_ahead<TM, true>(cm, sv, l0, r, e, tol, dacc, counter, nsteps, xul,
+                                              min(l0.maxs, counter + DRAIN_BLOCK));
+                        if (!(r.tra > tol) || counter >= l0.maxs) {
+                            finish_ray(sl, seq, runit, rtid, dacc, de);
+                            nrays += 1;
+                            active = false;
+                        }
+                    }
+                } else {
+                    /* the lowest live lane's ray, by every lane */
+                    const int src = __builtin_ctzll(act);
+                    Ray cr = bcast_ray(r, src);
+                    const En ce = bcast_en(e, src);
+                    float cd = bcast(dacc, src);
+                    int cc = bcast(counter, src), cs = 0;
+                    march_coop<TM>(cm, sv, l0, cr, ce, tol, cd, cc, cs, xul, lane);
+                    if (lane == src) {
+                        finish_ray(sl, seq, runit, rtid, cd, de);
+                        nsteps += cs;
+                        nrays += 1;
+                        active = false;
                     }
                 }
             }
