@@ -4,22 +4,26 @@
  *
  * Work unit: one emitting cell (or wall cell) of one snapshot and one of its
  * nVar runs, i.e. photmax rays.  A wavefront (one 64-lane workgroup) takes
- * units from a global atomic queue.  For each unit it draws the photmax
- * (band, g) samples and bitonic-sorts them in LDS (the srt enumeration of
- * mc_gpu_noint.cu:1234-1267) once the previous unit's rays are all
- * generated, while lanes still march them.
+ * units in chunks of nVar from per-XCD ranges of a global atomic queue.  For
+ * each unit it draws the photmax (band, g) samples and sorts them (the srt
+ * enumeration of mc_gpu_noint.cu:1234-1267): in registers up to 256 keys,
+ * in LDS above; the previous unit's rays keep marching from the LDS ray
+ * queue meanwhile.
  *
- * The march loop advances every active lane by one step per iteration
- * (march_step, the loop body of march_ray) and refills idle lanes with the
- * next rays of the current unit once enough lanes are idle, so a long ray
- * does not hold 63 idle lanes.  Rays keep their (sorted) order of dispatch,
- * so neighbouring lanes march neighbouring bands.  Each finished ray adds its
- * FP32 tally to its unit's FP64 total in LDS; the unit's last ray stores the
- * total.  Every per-ray result is bit-identical to the plain path and the
- * CPU oracle; only the summation order of the FP64 totals differs.
+ * The main loop generates rays in bulk (all 64 lanes set up one ray each),
+ * refills idle lanes from the queue once enough of them are idle, and
+ * advances every lane by a few steps (march_core_cached, the loop body of
+ * march_ray).  Rays keep their (sorted) order of dispatch, so neighbouring
+ * lanes march neighbouring bands.  Each finished ray adds its FP32 tally to
+ * its unit's FP64 total in an LDS slot; a slot's total goes to HBM when the
+ * slot is reused.  When no unit is left the wavefront drains its last rays
+ * (look-ahead march, then a whole-wavefront march of the longest ones).
+ * Every per-ray result is bit-identical to the plain path and the CPU
+ * oracle; only the summation order of the FP64 totals differs.
  *
- * Units of all snapshots share the queue; a wavefront drains its lanes
- * before crossing to another snapshot.
+ * Units of all snapshots (of one session, or of a group of sessions with
+ * different grids) share the queue; a wavefront drains its lanes before
+ * crossing to another snapshot.
  */
 #include <hip/hip_runtime.h>
 
