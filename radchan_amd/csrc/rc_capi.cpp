@@ -468,6 +468,11 @@ int rc_set_fields(rc_ctx *c, int nsnap, const float *const *T, int zstart) {
     const rc_params &p = c->p;
     if (zstart < 1 || zstart + c->kloc - 1 > p.kmax) { set_error("zstart outside 1..kmax-kmax/p_col+1"); return -2; }
     if (!p.grey && !c->have_tables) { set_error("narrow-band run without tables (rc_load_tables)"); return -3; }
+    /* (snapshot, cell, run) units and their tallies are numbered in 32 bits */
+    if ((int64_t)nsnap * std::max<int64_t>(c->ncell, 2 * (int64_t)c->nsurf) * p.nVar >= ((int64_t)1 << 31)) {
+        set_error("too many (snapshot, cell, run) units for one launch (2^31): fewer snapshots or runs");
+        return -5;
+    }
     if (c->launched) { (void)hipStreamSynchronize(c->stream); c->launched = false; }
     c->drop_graph();
     c->zstart = zstart;
@@ -887,6 +892,12 @@ int rc_group_run(rc_group *g) {
     rc_ctx *c0 = g->cs[0];
     const int nVar = c0->p.nVar;
     g->h.clear();
+    int64_t units = 0;
+    for (rc_ctx *c : g->cs) units += (int64_t)c->hsnap.size() * c->ncell * nVar;
+    if (units >= ((int64_t)1 << 31)) {
+        set_error("rc_group_run: too many (snapshot, cell, run) units for one launch (2^31)");
+        return -5;
+    }
     int cells = 0;
     for (rc_ctx *c : g->cs)
         for (const DevSnap &d0 : c->hsnap) {
