@@ -1035,6 +1035,33 @@ int orc_solve_fluid(const orc_ctx *c, int precision, const int32_t *cells, int n
     return 0;
 }
 
+/* orc_solve_fluid split by band group (test infrastructure: the precision
+ * study of the mixed-precision solver against full FP64) */
+int orc_solve_fluid_groups(const orc_ctx *c, int precision, const int32_t *cells, int ncells,
+                           const int32_t *group_of_rank, int ngroups, double *out, int nthreads) {
+    const int nVar = c->p.nVar, P = c->p.photmax;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+    for (int n = 0; n < ncells; ++n) {
+        const int i = cells[3 * n], j = cells[3 * n + 1], k = cells[3 * n + 2];
+        std::vector<int32_t> nb, g;
+        for (int v = 0; v < nVar; ++v) {
+            band_list(*c, false, fluid_cell_id(c->p, i, j, k), TAG_FLUID, v, nb, g);
+            double *o = out + ((size_t)n * nVar + v) * ngroups;
+            for (int q = 0; q < ngroups; ++q) o[q] = 0.0;
+            for (int h = 0; h < P; ++h) {
+                Tally t = tally0();
+                if (precision == 0) fluid_ray<float>(*c, i, j, k, v, h, nb[h], g[h], nullptr, t);
+                else fluid_ray<double>(*c, i, j, k, v, h, nb[h], g[h], nullptr, t);
+                o[group_of_rank[nb[h]]] += (precision == 0) ? (double)t.de : t.de64;
+            }
+        }
+    }
+    return 0;
+}
+
 int orc_solve_surface(const orc_ctx *c, int precision, const int32_t *cells, int ncells,
                       float *de32, double *de32d, double *de64, int64_t *steps, int nthreads) {
     const int nVar = c->p.nVar, P = c->p.photmax;

@@ -134,6 +134,13 @@ void     orc_trace_surface(const orc_ctx *c, int bound, int j, int k, int v, int
 int      orc_solve_fluid(const orc_ctx *c, int precision, const int32_t *cells,
                          int ncells, float *de32, double *de32d, double *de64,
                          int64_t *steps, int nthreads);
+/* orc_solve_fluid with every ray's tally added to its band group: group
+ * of a ray = group_of_rank[sorted band rank]; out receives, per cell, run
+ * and group, the FP64 sum of the per-ray tallies ([cell][run][group]) --
+ * the FP32 tallies (precision 0) or the full FP64 ones (precision 1) */
+int      orc_solve_fluid_groups(const orc_ctx *c, int precision, const int32_t *cells,
+                                int ncells, const int32_t *group_of_rank, int ngroups,
+                                double *out, int nthreads);
 /* walls: cells are (bound, j, k) triples */
 int      orc_solve_surface(const orc_ctx *c, int precision, const int32_t *cells,
                            int ncells, float *de32, double *de32d, double *de64,

@@ -55,6 +55,8 @@ def load():
                                       P(C.c_int64), C.c_int]),
         "orc_solve_surface": (C.c_int, [vp, C.c_int, P(C.c_int32), C.c_int, P(f), P(C.c_double), P(C.c_double),
                                         P(C.c_int64), C.c_int]),
+        "orc_solve_fluid_groups": (C.c_int, [vp, C.c_int, P(C.c_int32), C.c_int, P(C.c_int32), C.c_int,
+                                             P(C.c_double), C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -235,6 +237,20 @@ class Oracle:
                 steps.ctypes.data_as(C.POINTER(C.c_int64)), nthreads)
         assert rc == 0
         return de32.reshape(n, nv), de32d.reshape(n, nv), de64.reshape(n, nv), steps
+
+
+    def solve_groups(self, cells, group_of_rank, precision: int = 0, nthreads: int = 0) -> np.ndarray:
+        """Per (cell, run, band group) FP64 sums of the per-ray tallies
+        (FP32 tallies at precision 0, full FP64 at precision 1)."""
+        c = np.ascontiguousarray(np.asarray(cells, dtype=np.int32).reshape(-1, 3))
+        gr = np.ascontiguousarray(np.asarray(group_of_rank, dtype=np.int32))
+        ng = int(gr.max()) + 1
+        out = np.zeros(len(c) * self.p.nVar * ng, np.float64)
+        rc = self.lib.orc_solve_fluid_groups(self.ctx, precision, c.ctypes.data_as(C.POINTER(C.c_int32)), len(c),
+                                             gr.ctypes.data_as(C.POINTER(C.c_int32)), ng,
+                                             out.ctypes.data_as(C.POINTER(C.c_double)), nthreads)
+        assert rc == 0
+        return out.reshape(len(c), self.p.nVar, ng)
 
 
 def all_cells(p, zstart: int = 1) -> np.ndarray:

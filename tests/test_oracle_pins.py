@@ -431,3 +431,23 @@ def test_step_cap_drops_energy():
     rays = np.array([[8, 4, 4, 0, h] for h in range(64)], np.int32)
     r = o.trace(rays)
     assert (r["status"] == 3).any() and np.all(r["steps"] <= 3)
+
+
+def test_band_group_split_sums_to_the_cell_tallies(nb_tables):
+    """The band-group split of the precision study: per (cell, run), the
+    groups' FP32 tallies add up to the cell's FP64 sum of the same per-ray
+    tallies, and the full-FP64 groups to the full-FP64 solve (to FP64
+    reordering); a group that holds no band receives nothing."""
+    p = P.base_params(10, 8, 12, photmax=48, nVar=2, fact_mesh=1.5, seed=9)
+    T = F.channel_field(p, 300.0, 1500.0, noise=0.03, seed=9)
+    o = O.Oracle(p, nb_tables, T)
+    cells = O.all_cells(p)[::7]
+    gor = np.minimum(4 * np.arange(p.nB) // p.nB, 3)
+    gor[gor == 2] = 3                     # group 2 left empty
+    g0 = o.solve_groups(cells, gor, precision=0)
+    _, de32d, _, _ = o.solve(cells)
+    assert np.abs(g0.sum(axis=2) - de32d).max() <= 1e-12 * np.abs(de32d).max()
+    assert np.all(g0[:, :, 2] == 0.0)
+    g1 = o.solve_groups(cells, gor, precision=1)
+    _, _, de64, _ = o.solve(cells, precision=1)
+    assert np.abs(g1.sum(axis=2) - de64).max() <= 1e-12 * np.abs(de64).max()
