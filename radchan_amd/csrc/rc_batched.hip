@@ -774,7 +774,7 @@ void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, i
     }
     if (wg_per_cu <= 0) wg_per_cu = 32;
     if (threshold <= 0) threshold = 8;
-    if (spi <= 0) spi = 1;
+    if (spi <= 0) spi = 3;
     int n2 = 1;
     while (n2 < cm.photmax) n2 <<= 1;
     const size_t lds = (size_t)batched_lds_words(n2, cm.srt, cm.grid_num > 1) * 4;
