@@ -147,6 +147,18 @@ int rc_write_tables(const char *dir, const rc_tables *t, double chanHeight);
 int rc_grey_tables(const rc_params *p, float Tmax, rc_tables *t);
 /* sort_idx (mc_gpu_noint.cu:1278): idx[rank] = band, ascending kavg, ties by index */
 int rc_sort_idx(const float *kavg, int nB, int32_t *idx);
+/* correlated-k distributions of line-by-line spectra (correlatedKT,
+ * nbcorrk/correlatedK.cpp:144-225) and their band means k_avg (calcProbT,
+ * nbcorrk/calc_prob.cpp:52-56), on the GPU.  Spectrum s holds the points
+ * off[s]..off[s+1]-1 of wvn/kvn (1..8192 points, wavenumbers ascending) and
+ * the band edges wvl[s] < wvr[s]; n_pwrk bins of k^pwr (20000 and 0.1 in the
+ * generator, param.h:25 and correlatedK.cpp:147); gq[nq] nondecreasing in
+ * [0, 1].  Outputs: kq[nspec][nq], kavg, kmin, kmax per spectrum (host
+ * arrays; the call is synchronous).  kernel_ms (may be NULL) receives the
+ * kernel's device time. */
+int rc_ckdist(int nspec, const int64_t *off, const double *wvn, const double *kvn, const double *wvl,
+              const double *wvr, int n_pwrk, double pwr, int nq, const double *gq, double *kq,
+              double *kavg, double *kmin, double *kmax, double *kernel_ms);
 /* Gridn::mk_grid (definitions.h:68-136) of one level */
 int rc_grid(const rc_params *p, int level, float *x, float *xu, float *y,
             float *yu, float *z, float *zu);

@@ -1095,4 +1095,59 @@ int orc_solve_surface(const orc_ctx *c, int precision, const int32_t *cells, int
     return 0;
 }
 
+/* correlatedKT (nbcorrk/correlatedK.cpp:144-225): per spectrum, the
+ * k^pwr histogram ff, the interval centres kk, the cumulative gg and the
+ * quadrature values kq; kavg as calcProbT sums it (calc_prob.cpp:52-56).
+ * Deviations: bins are clamped to [0, n_pwrk] and a band of constant k
+ * (zero bin width) puts every point in bin 1. */
+int orc_ckdist(int nspec, const int64_t *off, const double *wvn, const double *kvn, const double *wvl,
+               const double *wvr, int n_pwrk, double pwr, int nq, const double *gq, double *kq, double *kavg,
+               double *kmin, double *kmax, int nthreads) {
+    if (nspec < 0 || n_pwrk < 2 || nq < 1 || !(pwr > 0.0)) return -1;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+    for (int s = 0; s < nspec; ++s) {
+        const int64_t b = off[s], e = off[s + 1];
+        double lo = 1e10, hi = -1000.0;
+        for (int64_t h = b; h < e; ++h) {
+            lo = std::min(lo, kvn[h]);
+            hi = std::max(hi, kvn[h]);
+        }
+        kmin[s] = lo;
+        kmax[s] = hi;
+        const double pmin = std::pow(lo, pwr), pmax = std::pow(hi, pwr);
+        const double step = (pmax - pmin) / (n_pwrk - 1);
+        std::vector<double> ff(n_pwrk + 1, 0.0), gg(n_pwrk + 1, 0.0), kk(n_pwrk + 1);
+        for (int h = 0; h < n_pwrk + 1; ++h) kk[h] = std::pow(h * step + pmin, 1.0 / pwr);
+        const double width = wvr[s] - wvl[s];
+        for (int64_t h = b + 1; h < e; ++h) {
+            const double ki = (kvn[h] + kvn[h - 1]) / 2;
+            int bin = 1;
+            if (step > 0.0) {
+                const double x = (std::pow(ki, pwr) - pmin) / step + 1;
+                bin = std::max(0, (int)x);
+                bin = std::min(bin, n_pwrk);
+            }
+            ff[bin] = ff[bin] + (wvn[h] - wvn[h - 1]) / width;
+        }
+        const int ng = n_pwrk - 1;
+        for (int h = 0; h < n_pwrk; ++h) kk[h] = 0.5 * (kk[h] + kk[h + 1]);
+        gg[ng] = 1.0;
+        for (int h = ng; h > 0; --h) gg[h - 1] = gg[h] - ff[h];
+        int t = 0;
+        for (int r = 0; r < nq; ++r) {
+            while (t + 1 < ng && !(gg[t + 1] >= gq[r])) ++t;
+            double v = kk[t] + (kk[t + 1] - kk[t]) * (gq[r] - gg[t]) / (gg[t + 1] - gg[t]);
+            if (gg[t + 1] == gg[t]) v = kk[t + 1];
+            kq[(size_t)s * nq + r] = v;
+        }
+        double a = 0.0;
+        for (int h = 1; h < n_pwrk; ++h) a += kk[h] * ff[h];
+        kavg[s] = a;
+    }
+    return 0;
+}
+
 }  // extern "C"

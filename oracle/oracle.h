@@ -146,6 +146,22 @@ int      orc_solve_surface(const orc_ctx *c, int precision, const int32_t *cells
                            int ncells, float *de32, double *de32d, double *de64,
                            int64_t *steps, int nthreads);
 
+/* Correlated-k distributions of narrow bands from line-by-line spectra
+ * (nbcorrk/correlatedK.cpp:144-225 and the k_avg sum of calc_prob.cpp:52-56),
+ * one band of one temperature per spectrum s: points off[s] .. off[s+1]-1
+ * of (wvn [cm^-1], kvn), band edges wvl[s], wvr[s].  The k range of the
+ * band is cut into n_pwrk intervals of equal width in k^pwr; each pair of
+ * neighbouring points adds its wavenumber width (over the band width) to
+ * the interval of its mean k; the cumulative g is summed down from 1 at
+ * the top interval; k at the quadrature points gq is interpolated
+ * linearly in g between interval centres.  Outputs per spectrum:
+ * kq[s][nq], kavg[s] (sum of interval centre k times its fraction),
+ * kmin[s], kmax[s].  nthreads OpenMP threads (0 = runtime default). */
+int      orc_ckdist(int nspec, const int64_t *off, const double *wvn, const double *kvn,
+                    const double *wvl, const double *wvr, int n_pwrk, double pwr, int nq,
+                    const double *gq, double *kq, double *kavg, double *kmin, double *kmax,
+                    int nthreads);
+
 #ifdef __cplusplus
 }
 #endif

@@ -57,6 +57,9 @@ def load():
                                         P(C.c_int64), C.c_int]),
         "orc_solve_fluid_groups": (C.c_int, [vp, C.c_int, P(C.c_int32), C.c_int, P(C.c_int32), C.c_int,
                                              P(C.c_double), C.c_int]),
+        "orc_ckdist": (C.c_int, [C.c_int, P(C.c_int64), P(C.c_double), P(C.c_double), P(C.c_double),
+                                 P(C.c_double), C.c_int, C.c_double, C.c_int, P(C.c_double), P(C.c_double),
+                                 P(C.c_double), P(C.c_double), P(C.c_double), C.c_int]),
     }
     for name, (res, args) in sig.items():
         fn = getattr(lib, name)
@@ -251,6 +254,22 @@ class Oracle:
                                              out.ctypes.data_as(C.POINTER(C.c_double)), nthreads)
         assert rc == 0
         return out.reshape(len(c), self.p.nVar, ng)
+
+
+def ckdist(off, wvn, kvn, wvl, wvr, gq, n_pwrk: int = 20000, pwr: float = 0.1, nthreads: int = 0):
+    """Correlated-k distributions of line-by-line spectra (the restatement of
+    correlatedKT): returns (kq [nspec][nq], kavg, kmin, kmax)."""
+    d = lambda a: np.ascontiguousarray(a, np.float64)
+    off = np.ascontiguousarray(off, np.int64)
+    wvn, kvn, wvl, wvr, gq = d(wvn), d(kvn), d(wvl), d(wvr), d(gq)
+    ns, nq = len(off) - 1, len(gq)
+    kq = np.zeros((ns, nq))
+    kavg, kmin, kmax = np.zeros(ns), np.zeros(ns), np.zeros(ns)
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    rc = load().orc_ckdist(ns, off.ctypes.data_as(C.POINTER(C.c_int64)), dp(wvn), dp(kvn), dp(wvl), dp(wvr),
+                           n_pwrk, pwr, nq, dp(gq), dp(kq), dp(kavg), dp(kmin), dp(kmax), nthreads)
+    assert rc == 0
+    return kq, kavg, kmin, kmax
 
 
 def all_cells(p, zstart: int = 1) -> np.ndarray:

@@ -22,7 +22,7 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libradchan.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 
-SOURCES = ["rc_tables.cpp", "rc_capi.cpp", "rc_solver.hip", "rc_batched.hip"]
+SOURCES = ["rc_tables.cpp", "rc_capi.cpp", "rc_solver.hip", "rc_batched.hip", "rc_ckdist.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",

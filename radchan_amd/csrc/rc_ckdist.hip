@@ -1,0 +1,402 @@
+/*
+ * rc_ckdist.hip -- correlated-k distributions of line-by-line spectra
+ * (nbcorrk/correlatedK.cpp:144-225, correlatedKT) and the band means k_avg
+ * of calcProbT (nbcorrk/calc_prob.cpp:52-56), on the GPU.
+ *
+ * Per spectrum (one narrow band at one temperature, points (wvn, kvn)):
+ *   kmin, kmax over the points; p = k^pwr spaced evenly in n_pwrk steps,
+ *   kk[h] = (h*step + pmin)^(1/pwr), then shifted half a bin upwards;
+ *   ff[bin] += dwv_h / (wv_right - wv_left) for the pair average
+ *   ki = (k_h + k_{h-1}) / 2, bin = max(0, int((ki^pwr - pmin)/step + 1));
+ *   gg[ng] = 1, gg[h-1] = gg[h] - ff[h] for h = ng..1 (ng = n_pwrk - 1);
+ *   kq[r] interpolates kk linearly in gg at each quadrature point gq[r];
+ *   k_avg = sum_{h=1}^{ng} kk[h] ff[h].
+ *
+ * The reference walks all n_pwrk + 1 bins per spectrum.  A narrow band holds
+ * far fewer points (about 1000 at dwv = 0.01 cm^-1 and 10 cm^-1 bands) than
+ * bins (20 000), so the histogram is sparse: this kernel sorts the point
+ * pairs by bin and works on the occupied bins only.  gg is a step function
+ * that changes at occupied bins, so the quadrature walk reduces to a
+ * binary search over them, and kk is evaluated only where it is used.
+ *
+ * One workgroup of 256 threads per spectrum, everything in LDS:
+ *   (1) kmin/kmax: block min/max (exact in any order);
+ *   (2) per pair h = 1..n-1 the key bin << 13 | h; bitonic sort in LDS, so
+ *       that pairs of one bin come in ascending h;
+ *   (3) run starts (occupied bins) compacted by a block scan;
+ *   (4) per occupied bin, ff summed in ascending h by one thread -- the
+ *       reference's order, so ff is bit for bit the reference's;
+ *   (5) the gg chain (one lane, descending bins) and the k_avg sum (one lane
+ *       of another wave, ascending bins): the reference's sequences of
+ *       operations with the zero terms of empty bins left out (x - 0 and
+ *       x + 0 are exact), so both are bit for bit given the same kk;
+ *   (6) one thread per quadrature point: binary searches for the first
+ *       occupied bin whose gg reaches gq[r].
+ * kk and the bin index use the device's FP64 pow, which may differ from the
+ * host libm's in the last bit: results agree with the CPU restatement to FP64
+ * rounding (and a pair exactly on a bin edge could land one bin over).
+ *
+ * LDS per spectrum, np2 = pow2 >= n - 1 (at least 256): keys u32[np2], run
+ * starts u32[np2], ff f64[np2], run bins u16[np2] = 18 B per slot; the
+ * products kk*ff (f64[np2]) overlay keys and run starts once those are dead.
+ * Up to 8192 points per spectrum (13-bit pair index), n_pwrk <= 65535.
+ */
+#include <hip/hip_runtime.h>
+
+#include <algorithm>
+#include <cmath>
+#include <cstdint>
+#include <string>
+#include <vector>
+
+#include "rc_internal.h"
+
+namespace rc {
+namespace {
+
+constexpr int CK_THREADS = 256;
+constexpr int CK_WAVES = CK_THREADS / 64;
+constexpr int CK_NMAX = 8192;        /* points per spectrum (13-bit pair index) */
+constexpr int CK_IDX_BITS = 13;
+
+struct CkArgs {
+    const int64_t *off;
+    const double *wvn, *kvn, *wvl, *wvr, *gq;
+    double *kq, *kavg, *kmin, *kmax;
+    double pwr, ipwr;
+    int nq, n_pwrk, np2;
+};
+
+__device__ __forceinline__ double kk_at(int h, double step, double pmin, double ipwr) {
+    /* kk[h] after the half-bin shift: 0.5 * (K(h) + K(h+1)), K(h) = (h*step + pmin)^(1/pwr) */
+    const double a = pow((double)h * step + pmin, ipwr);
+    const double b = pow((double)(h + 1) * step + pmin, ipwr);
+    return 0.5 * (a + b);
+}
+
+/* gg(j) = G_lo of the first occupied bin above j, 1 above the last */
+__device__ __forceinline__ double gg_at(int j, const uint16_t *RB, const double *G, int r0, int r1) {
+    int lo = r0, hi = r1;
+    while (lo < hi) {
+        const int mid = (lo + hi) >> 1;
+        if ((int)RB[mid] > j) hi = mid; else lo = mid + 1;
+    }
+    return lo < r1 ? G[lo] : 1.0;
+}
+
+__global__ __launch_bounds__(CK_THREADS) void k_ckdist(CkArgs a) {
+    extern __shared__ __align__(16) unsigned char lds[];
+    __shared__ double red_lo[CK_WAVES], red_hi[CK_WAVES];
+    __shared__ int scan[CK_WAVES];
+    __shared__ int s_m, s_r0, s_r1;
+
+    const int s = blockIdx.x;
+    const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
+    const int np2 = a.np2;
+    uint32_t *K = (uint32_t *)lds;                      /* [np2] sort keys          */
+    uint32_t *RS = K + np2;                             /* [np2] run starts         */
+    double *P = (double *)lds;                          /* [np2] kk*ff (over K, RS) */
+    double *F = (double *)(lds + 8 * (size_t)np2);      /* [np2] ff, then gg below  */
+    uint16_t *RB = (uint16_t *)(lds + 16 * (size_t)np2); /* [np2] run bins          */
+
+    const int64_t base = a.off[s];
+    const int n = (int)(a.off[s + 1] - base);
+    const double *wvs = a.wvn + base, *kvs = a.kvn + base;
+
+    /* (1) kmin, kmax (the reference's initial values 1e10 and -1000) */
+    double lo = 1e10, hi = -1000.0;
+    for (int h = tid; h < n; h += CK_THREADS) {
+        const double k = kvs[h];
+        lo = k < lo ? k : lo;
+        hi = hi < k ? k : hi;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        const double l2 = __shfl_xor(lo, o), h2 = __shfl_xor(hi, o);
+        lo = l2 < lo ? l2 : lo;
+        hi = hi < h2 ? h2 : hi;
+    }
+    if (lane == 0) { red_lo[wv] = lo; red_hi[wv] = hi; }
+    __syncthreads();
+    lo = red_lo[0];
+    hi = red_hi[0];
+    for (int w = 1; w < CK_WAVES; ++w) {
+        lo = red_lo[w] < lo ? red_lo[w] : lo;
+        hi = hi < red_hi[w] ? red_hi[w] : hi;
+    }
+    const int n_pwrk = a.n_pwrk, ng = n_pwrk - 1;
+    const double pmin = pow(lo, a.pwr), pmax = pow(hi, a.pwr);
+    const double step = (pmax - pmin) / (double)(n_pwrk - 1);
+    const double width = a.wvr[s] - a.wvl[s];
+    const int npair = n > 1 ? n - 1 : 0;
+
+    /* (2) keys: bin << 13 | h for the pair (h-1, h); padding sorts last */
+    for (int i = tid; i < np2; i += CK_THREADS) {
+        uint32_t key = 0xFFFFFFFFu;
+        if (i < npair) {
+            const int h = i + 1;
+            const double ki = (kvs[h] + kvs[h - 1]) / 2;
+            int bin = 1;                                 /* a flat spectrum (step 0) */
+            if (step > 0.0) {
+                const double x = (pow(ki, a.pwr) - pmin) / step + 1;
+                bin = max(0, (int)x);
+                bin = min(bin, n_pwrk);
+            }
+            key = ((uint32_t)bin << CK_IDX_BITS) | (uint32_t)h;
+        }
+        K[i] = key;
+    }
+    for (int k = 2; k <= np2; k <<= 1) {
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            __syncthreads();
+            for (int i = tid; i < np2; i += CK_THREADS) {
+                const int p = i ^ j;
+                if (p > i) {
+                    const uint32_t x = K[i], y = K[p];
+                    if ((x > y) == ((i & k) == 0)) { K[i] = y; K[p] = x; }
+                }
+            }
+        }
+    }
+    __syncthreads();
+
+    /* (3) run starts, compacted in order: each thread scans a contiguous chunk */
+    const int chunk = np2 / CK_THREADS;
+    const int c0 = tid * chunk;
+    int cnt = 0;
+    for (int i = c0; i < c0 + chunk; ++i)
+        cnt += (i < npair && (i == 0 || (K[i] >> CK_IDX_BITS) != (K[i - 1] >> CK_IDX_BITS))) ? 1 : 0;
+    int inc = cnt;
+    for (int o = 1; o < 64; o <<= 1) {
+        const int v = __shfl_up(inc, o);
+        if (lane >= o) inc += v;
+    }
+    if (lane == 63) scan[wv] = inc;
+    __syncthreads();
+    int pos = inc - cnt;
+    for (int w = 0; w < wv; ++w) pos += scan[w];
+    if (tid == CK_THREADS - 1) s_m = pos + cnt;
+    for (int i = c0; i < c0 + chunk; ++i) {
+        if (i < npair && (i == 0 || (K[i] >> CK_IDX_BITS) != (K[i - 1] >> CK_IDX_BITS))) {
+            RS[pos] = (uint32_t)i;
+            RB[pos] = (uint16_t)(K[i] >> CK_IDX_BITS);
+            ++pos;
+        }
+    }
+    __syncthreads();
+    const int m = s_m;
+
+    /* (4) ff per occupied bin, summed in ascending h */
+    for (int r = tid; r < m; r += CK_THREADS) {
+        const int b = (int)RS[r], e = r + 1 < m ? (int)RS[r + 1] : npair;
+        double f = 0.0;
+        for (int q = b; q < e; ++q) {
+            const int h = (int)(K[q] & ((1u << CK_IDX_BITS) - 1));
+            f = f + (wvs[h] - wvs[h - 1]) / width;
+        }
+        F[r] = f;
+    }
+    __syncthreads();
+    /* kk*ff of the bins the k_avg sum covers (1..ng), over the dead keys */
+    for (int r = tid; r < m; r += CK_THREADS) {
+        const int bin = RB[r];
+        P[r] = (bin >= 1 && bin <= ng) ? kk_at(bin, step, pmin, a.ipwr) * F[r] : 0.0;
+    }
+    if (tid == 0) {
+        int r0 = (m > 0 && RB[0] == 0) ? 1 : 0;
+        int r1 = (m > r0 && (int)RB[m - 1] > ng) ? m - 1 : m;
+        s_r0 = r0;
+        s_r1 = r1;
+    }
+    __syncthreads();
+    const int r0 = s_r0, r1 = s_r1;
+
+    /* (5) gg chain from the top (wave 0) and the k_avg sum (wave 1) */
+    if (tid == 0) {
+        double g = 1.0;
+        int r = r1 - 1;
+        for (; r - 7 >= r0; r -= 8) {
+            double f[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) f[u] = F[r - u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) { g = g - f[u]; F[r - u] = g; }
+        }
+        for (; r >= r0; --r) { g = g - F[r]; F[r] = g; }
+    } else if (tid == 64) {
+        double acc = 0.0;
+        int r = r0;
+        for (; r + 7 < r1; r += 8) {
+            double v[8];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) v[u] = P[r + u];
+#pragma unroll
+            for (int u = 0; u < 8; ++u) acc = acc + v[u];
+        }
+        for (; r < r1; ++r) acc = acc + P[r];
+        a.kavg[s] = acc;
+        a.kmin[s] = lo;
+        a.kmax[s] = hi;
+    }
+    __syncthreads();
+
+    /* (6) quadrature: t + 1 = the first j >= 1 with gg(j) >= gq (at most ng) */
+    for (int q = tid; q < a.nq; q += CK_THREADS) {
+        const double g = a.gq[q];
+        int jstar;
+        if (gg_at(1, RB, F, r0, r1) >= g) {
+            jstar = 1;
+        } else {
+            int lo2 = r0;                                /* first occupied bin >= 2 */
+            int hi2 = r1;
+            while (lo2 < hi2) {
+                const int mid = (lo2 + hi2) >> 1;
+                if ((int)RB[mid] >= 2) hi2 = mid; else lo2 = mid + 1;
+            }
+            hi2 = r1;                                    /* first whose upper gg >= g */
+            while (lo2 < hi2) {
+                const int mid = (lo2 + hi2) >> 1;
+                const double up = mid + 1 < r1 ? F[mid + 1] : 1.0;
+                if (up >= g) hi2 = mid; else lo2 = mid + 1;
+            }
+            jstar = lo2 < r1 ? (int)RB[lo2] : ng;
+        }
+        const int t = jstar - 1;
+        const double g0 = gg_at(t, RB, F, r0, r1), g1 = gg_at(t + 1, RB, F, r0, r1);
+        const double k0 = kk_at(t, step, pmin, a.ipwr), k1 = kk_at(t + 1, step, pmin, a.ipwr);
+        double v = k0 + (k1 - k0) * (g - g0) / (g1 - g0);
+        if (g1 == g0) v = k1;
+        a.kq[(size_t)s * a.nq + q] = v;
+    }
+}
+
+struct DevBuf {
+    void *p = nullptr;
+    ~DevBuf() { if (p) (void)hipFree(p); }
+};
+
+}  // namespace
+}  // namespace rc
+
+using namespace rc;
+
+extern "C" int rc_ckdist(int nspec, const int64_t *off, const double *wvn, const double *kvn, const double *wvl,
+                         const double *wvr, int n_pwrk, double pwr, int nq, const double *gq, double *kq,
+                         double *kavg, double *kmin, double *kmax, double *kernel_ms) {
+    if (nspec < 1 || !off || !wvn || !kvn || !wvl || !wvr || !gq || !kq || !kavg || !kmin || !kmax) {
+        set_error("rc_ckdist: null array or no spectra");
+        return -1;
+    }
+    if (n_pwrk < 2 || n_pwrk > 65535 || !(pwr > 0.0) || !std::isfinite(pwr)) {
+        set_error("rc_ckdist: n_pwrk in 2..65535 and pwr > 0");
+        return -2;
+    }
+    if (nq < 1 || nq > 4096) { set_error("rc_ckdist: nq in 1..4096"); return -3; }
+    for (int q = 0; q < nq; ++q) {
+        if (!(gq[q] >= 0.0 && gq[q] <= 1.0) || (q > 0 && gq[q] < gq[q - 1])) {
+            set_error("rc_ckdist: gq must be nondecreasing in [0, 1]");
+            return -3;
+        }
+    }
+    if (off[0] < 0) { set_error("rc_ckdist: negative offset"); return -4; }
+    int nmax = 0;
+    for (int s = 0; s < nspec; ++s) {
+        const int64_t n = off[s + 1] - off[s];
+        if (n < 1 || n > CK_NMAX) {
+            set_error("rc_ckdist: spectrum " + std::to_string(s) + " has " + std::to_string(n) +
+                      " points (1.." + std::to_string(CK_NMAX) + " per spectrum)");
+            return -4;
+        }
+        if (!(wvr[s] > wvl[s])) {
+            set_error("rc_ckdist: spectrum " + std::to_string(s) + " has wv_right <= wv_left");
+            return -5;
+        }
+        nmax = std::max(nmax, (int)n);
+    }
+    int np2 = CK_THREADS;
+    while (np2 < nmax - 1) np2 <<= 1;
+    const size_t lds = (size_t)np2 * 18;
+    int dev = 0, lds_max = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
+        set_error("rc_ckdist: no HIP device");
+        return -6;
+    }
+    if (lds + 256 > (size_t)lds_max) {
+        set_error("rc_ckdist: spectra of " + std::to_string(nmax) + " points need " + std::to_string(lds) +
+                  " B of LDS; the device allows " + std::to_string(lds_max));
+        return -6;
+    }
+
+    const int64_t p0 = off[0], np = off[nspec] - p0;
+    std::vector<int64_t> hoff(off, off + nspec + 1);
+    for (auto &o : hoff) o -= p0;
+    DevBuf d_off, d_wvn, d_kvn, d_wvl, d_wvr, d_gq, d_kq, d_kavg, d_kmin, d_kmax;
+    hipStream_t st = nullptr;
+    hipEvent_t e0 = nullptr, e1 = nullptr;
+    int rc = 0;
+    auto fail = [&](const char *what, hipError_t e) {
+        set_error(std::string("rc_ckdist: ") + what + ": " + hipGetErrorString(e));
+        rc = -7;
+    };
+    hipError_t e;
+    if ((e = hipStreamCreate(&st)) != hipSuccess) { fail("stream", e); return rc; }
+#define CK_TRY(call)                                   \
+    if ((e = (call)) != hipSuccess) { fail(#call, e); break; }
+    const size_t nsb = (size_t)nspec * sizeof(double);
+    do {
+        CK_TRY(hipEventCreate(&e0));
+        CK_TRY(hipEventCreate(&e1));
+        CK_TRY(hipMalloc(&d_off.p, (nspec + 1) * sizeof(int64_t)));
+        CK_TRY(hipMalloc(&d_wvn.p, np * sizeof(double)));
+        CK_TRY(hipMalloc(&d_kvn.p, np * sizeof(double)));
+        CK_TRY(hipMalloc(&d_wvl.p, nsb));
+        CK_TRY(hipMalloc(&d_wvr.p, nsb));
+        CK_TRY(hipMalloc(&d_gq.p, nq * sizeof(double)));
+        CK_TRY(hipMalloc(&d_kq.p, nsb * nq));
+        CK_TRY(hipMalloc(&d_kavg.p, nsb));
+        CK_TRY(hipMalloc(&d_kmin.p, nsb));
+        CK_TRY(hipMalloc(&d_kmax.p, nsb));
+        CK_TRY(hipMemcpyAsync(d_off.p, hoff.data(), (nspec + 1) * sizeof(int64_t), hipMemcpyHostToDevice, st));
+        CK_TRY(hipMemcpyAsync(d_wvn.p, wvn + p0, np * sizeof(double), hipMemcpyHostToDevice, st));
+        CK_TRY(hipMemcpyAsync(d_kvn.p, kvn + p0, np * sizeof(double), hipMemcpyHostToDevice, st));
+        CK_TRY(hipMemcpyAsync(d_wvl.p, wvl, nsb, hipMemcpyHostToDevice, st));
+        CK_TRY(hipMemcpyAsync(d_wvr.p, wvr, nsb, hipMemcpyHostToDevice, st));
+        CK_TRY(hipMemcpyAsync(d_gq.p, gq, nq * sizeof(double), hipMemcpyHostToDevice, st));
+        CkArgs a;
+        a.off = (const int64_t *)d_off.p;
+        a.wvn = (const double *)d_wvn.p;
+        a.kvn = (const double *)d_kvn.p;
+        a.wvl = (const double *)d_wvl.p;
+        a.wvr = (const double *)d_wvr.p;
+        a.gq = (const double *)d_gq.p;
+        a.kq = (double *)d_kq.p;
+        a.kavg = (double *)d_kavg.p;
+        a.kmin = (double *)d_kmin.p;
+        a.kmax = (double *)d_kmax.p;
+        a.pwr = pwr;
+        a.ipwr = 1.0 / pwr;
+        a.nq = nq;
+        a.n_pwrk = n_pwrk;
+        a.np2 = np2;
+        CK_TRY(hipEventRecord(e0, st));
+        hipLaunchKernelGGL(k_ckdist, dim3(nspec), dim3(CK_THREADS), lds, st, a);
+        CK_TRY(hipGetLastError());
+        CK_TRY(hipEventRecord(e1, st));
+        CK_TRY(hipMemcpyAsync(kq, d_kq.p, nsb * nq, hipMemcpyDeviceToHost, st));
+        CK_TRY(hipMemcpyAsync(kavg, d_kavg.p, nsb, hipMemcpyDeviceToHost, st));
+        CK_TRY(hipMemcpyAsync(kmin, d_kmin.p, nsb, hipMemcpyDeviceToHost, st));
+        CK_TRY(hipMemcpyAsync(kmax, d_kmax.p, nsb, hipMemcpyDeviceToHost, st));
+        CK_TRY(hipStreamSynchronize(st));
+        if (kernel_ms) {
+            float ms = 0.0f;
+            CK_TRY(hipEventElapsedTime(&ms, e0, e1));
+            *kernel_ms = ms;
+        }
+    } while (0);
+#undef CK_TRY
+    (void)hipStreamSynchronize(st);
+    if (e0) (void)hipEventDestroy(e0);
+    if (e1) (void)hipEventDestroy(e1);
+    (void)hipStreamDestroy(st);
+    return rc;
+}

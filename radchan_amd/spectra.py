@@ -1,0 +1,70 @@
+"""Synthetic line-by-line absorption spectra for the correlated-k
+generator (rc_ckdist): per temperature and narrow band, k(nu) on a uniform
+wavenumber grid of spacing dwv (nbcorrk/param.h:20) made of a continuum and
+Lorentz lines whose positions, widths and strengths come from a seeded
+generator; each line's strength has its own power law in temperature, so
+the k-distribution changes shape from one temperature to the next.  This is
+the shape of an H2O spectrum, not its physics.
+
+Band layout follows the reference generator (nbcorrk/ckdist_hitran_part.cpp
+:120-160): every point belongs to exactly one narrow band, wv_left and
+wv_right are the band's first and last wavenumber, and the default band
+width is Deltawv = 10 cm^-1 (param.h:21-23).
+"""
+from __future__ import annotations
+
+import numpy as np
+
+
+def synthetic_lbl(wv0: float = 150.0, wv1: float = 9300.0, deltawv: float = 10.0, temps=None,
+                  dwv: float = 0.01, lines_per_band: int = 12, seed: int = 0):
+    """Spectra ordered temperature-major (spectrum t*nB + j).  Returns a dict
+    with off[nspec+1] (int64 point offsets), wvn, kvn (FP64 points, k in
+    m^-1), wvl, wvr (band edges), T (per spectrum), nB and temps."""
+    if temps is None:
+        temps = np.arange(52) * 50.0 + 300.0
+    temps = np.asarray(temps, np.float64)
+    rng = np.random.default_rng(seed)
+    npt_all = int(round((wv1 - wv0) / dwv))
+    per = int(round(deltawv / dwv))
+    nB = max(1, npt_all // per)
+    nu_all = wv0 + dwv * np.arange(nB * per)
+    nu = nu_all.reshape(nB, per)                                     # [nB][npt]
+    x = (nu - nu[:, :1]) / (nu[:, -1:] - nu[:, :1])
+    pos = rng.uniform(0.0, 1.0, (nB, lines_per_band))
+    width = 10.0 ** rng.uniform(-3.0, -1.0, (nB, lines_per_band))     # half width / band width
+    s0 = 10.0 ** rng.uniform(-2.0, 3.0, (nB, lines_per_band))        # peak strength at 1000 K, m^-1
+    expo = rng.uniform(-2.0, 1.0, (nB, lines_per_band))              # strength ~ (T/1000)^expo
+    cont = 10.0 ** rng.uniform(-3.0, -1.0, nB)
+    shape = width[:, :, None] ** 2 / ((x[:, None, :] - pos[:, :, None]) ** 2 + width[:, :, None] ** 2)
+    ks = []
+    for T in temps:
+        s = s0 * (T / 1000.0) ** expo                                  # [nB][lines]
+        ks.append(cont[:, None] * (T / 1000.0) ** -0.5 + np.einsum("jq,jqx->jx", s, shape))
+    kvn = np.concatenate([k.reshape(-1) for k in ks])
+    wvn = np.tile(nu.reshape(-1), len(temps))
+    nspec = nB * len(temps)
+    off = np.arange(nspec + 1, dtype=np.int64) * per
+    wvl = np.tile(nu[:, 0], len(temps))
+    wvr = np.tile(nu[:, -1], len(temps))
+    return dict(off=off, wvn=wvn, kvn=kvn, wvl=wvl, wvr=wvr, T=np.repeat(temps, nB), nB=nB, temps=temps)
+
+
+def ckdist(off, wvn, kvn, wvl, wvr, gq, n_pwrk: int = 20000, pwr: float = 0.1):
+    """Correlated-k distributions on the GPU (rc_ckdist): returns a dict
+    with kq [nspec][nq], kavg, kmin, kmax and kernel_ms."""
+    import ctypes as C
+
+    from . import lib as L
+
+    d = lambda a: np.ascontiguousarray(a, np.float64)
+    off = np.ascontiguousarray(off, np.int64)
+    wvn, kvn, wvl, wvr, gq = d(wvn), d(kvn), d(wvl), d(wvr), d(gq)
+    ns, nq = len(off) - 1, len(gq)
+    kq = np.zeros((ns, nq))
+    kavg, kmin, kmax = np.zeros(ns), np.zeros(ns), np.zeros(ns)
+    ms = C.c_double()
+    L.check(L.load().rc_ckdist(ns, off.ctypes.data_as(C.POINTER(C.c_int64)), L.dptr(wvn), L.dptr(kvn),
+                               L.dptr(wvl), L.dptr(wvr), n_pwrk, pwr, nq, L.dptr(gq), L.dptr(kq), L.dptr(kavg),
+                               L.dptr(kmin), L.dptr(kmax), C.byref(ms)), "rc_ckdist")
+    return dict(kq=kq, kavg=kavg, kmin=kmin, kmax=kmax, kernel_ms=ms.value)

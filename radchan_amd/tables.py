@@ -26,13 +26,16 @@ PI = 3.14159265358979323846
 C1 = 3.741771790075259e-16
 C2 = 0.014387741858429
 
-# NarrowBand.h:21-27 (nQ = 16)
-GQ16 = np.array([0.04830766, 0.14447196, 0.23928736, 0.33186860, 0.42135127, 0.50689990, 0.58771575,
-                 0.66304426, 0.73218211, 0.79448379, 0.84936761, 0.89632115, 0.93490607, 0.96476225,
-                 0.98561151, 0.99726386], dtype=np.float32)
-WQ16 = np.array([0.09654009, 0.09563872, 0.09384440, 0.09117387, 0.08765209, 0.08331193, 0.07819389,
-                 0.07234580, 0.06582222, 0.05868409, 0.05099806, 0.04283589, 0.03427387, 0.02539207,
-                 0.01627439, 0.00701862], dtype=np.float32)
+# NarrowBand.h:21-27 (nQ = 16); the generator holds the same digits as
+# doubles (nbcorrk/param.h:99-105)
+_GQ16 = [0.04830766, 0.14447196, 0.23928736, 0.33186860, 0.42135127, 0.50689990, 0.58771575, 0.66304426,
+         0.73218211, 0.79448379, 0.84936761, 0.89632115, 0.93490607, 0.96476225, 0.98561151, 0.99726386]
+_WQ16 = [0.09654009, 0.09563872, 0.09384440, 0.09117387, 0.08765209, 0.08331193, 0.07819389, 0.07234580,
+         0.06582222, 0.05868409, 0.05099806, 0.04283589, 0.03427387, 0.02539207, 0.01627439, 0.00701862]
+GQ16 = np.array(_GQ16, dtype=np.float32)
+WQ16 = np.array(_WQ16, dtype=np.float32)
+GQ16_F64 = np.array(_GQ16, dtype=np.float64)
+WQ16_F64 = np.array(_WQ16, dtype=np.float64)
 
 
 def planck(T, nu):

@@ -1,0 +1,173 @@
+"""Correlated-k generator (correlatedKT, nbcorrk/correlatedK.cpp:144-225, and
+the k_avg of calcProbT, calc_prob.cpp:52-56).
+
+CPU: the oracle's restatement pinned by what the mathematics fixes -- the
+inverse-CDF property of kq on a linear spectrum (closed form) and on random
+spectra (brute-force CDF of the point pairs), k_avg as the band mean of the
+pair averages, exact scaling by 2^10 (k^0.1 scales by 2 exactly), the flat
+and one-point spectra.  GPU: rc_ckdist against the oracle on the same
+spectra, ragged sizes and the edge cases.  The reference's own tests hold no
+vectors for this path (nbcorrk/tests holds plots only)."""
+import numpy as np
+import pytest
+
+from oracle import oracle as O
+from radchan_amd.spectra import synthetic_lbl
+from radchan_amd.tables import GQ16_F64
+
+N_PWRK, PWR = 20000, 0.1
+
+
+def _one(wv, kv, gq=GQ16_F64, wl=None, wr=None, **kw):
+    wv, kv = np.asarray(wv, float), np.asarray(kv, float)
+    wl = wv[0] if wl is None else wl
+    wr = wv[-1] if wr is None else wr
+    kq, ka, lo, hi = O.ckdist([0, len(wv)], wv, kv, [wl], [wr], gq, **kw)
+    return kq[0], ka[0], lo[0], hi[0]
+
+
+def _rel_bin(kmin, kmax, k):
+    """Relative width of the k bin holding k: (p + step)^10 / p^10 - 1."""
+    step = (kmax ** PWR - kmin ** PWR) / (N_PWRK - 1)
+    return (1.0 + step / k ** PWR) ** (1.0 / PWR) - 1.0
+
+
+def test_linear_spectrum_inverse_cdf():
+    """k rising linearly across the band has a uniform k-distribution:
+    k(g) = k0 + g (k1 - k0), up to a bin width (the half-bin shift) and the
+    point spacing."""
+    n, k0, k1 = 20001, 1.0, 100.0
+    wv = 1000.0 + 0.01 * np.arange(n)
+    kv = np.linspace(k0, k1, n)
+    kq, ka, lo, hi = _one(wv, kv)
+    assert lo == k0 and hi == k1
+    exact = k0 + GQ16_F64 * (k1 - k0)
+    tol = 2.0 * _rel_bin(k0, k1, exact) * exact + 2.0 * (k1 - k0) / n
+    assert np.all(np.abs(kq - exact) <= tol)
+    assert abs(ka - 0.5 * (k0 + k1)) <= 2e-4 * 0.5 * (k0 + k1)
+    # a planted mistake (no half-bin shift, i.e. kk = K(h)) would sit half a
+    # bin low: the shift is visible above the point-spacing error
+    assert np.all(kq > exact - 2.0 * (k1 - k0) / n)
+
+
+def test_random_spectra_inverse_cdf_and_band_mean():
+    """On seeded line spectra: the weighted fraction of pair averages below
+    kq[r] brackets gq[r] within the weight of the bins next to kq, and k_avg
+    is the band mean of the pair averages to a bin width."""
+    d = synthetic_lbl(temps=[300.0, 1500.0])
+    sel = np.arange(0, len(d["off"]) - 1, 37)
+    for s in sel:
+        b, e = d["off"][s], d["off"][s + 1]
+        wv, kv = d["wvn"][b:e], d["kvn"][b:e]
+        kq, ka, lo, hi = _one(wv, kv, wl=d["wvl"][s], wr=d["wvr"][s])
+        ki = 0.5 * (kv[1:] + kv[:-1])
+        w = np.diff(wv) / (d["wvr"][s] - d["wvl"][s])
+        rb = _rel_bin(lo, hi, np.maximum(kq, lo))
+        for r, g in enumerate(GQ16_F64):
+            below = w[ki < kq[r] * (1 - 2 * rb[r])].sum()
+            upto = w[ki <= kq[r] * (1 + 2 * rb[r])].sum()
+            assert below <= g + 1e-12 and upto >= g - 1e-12, (s, r, below, g, upto)
+        assert np.all(np.diff(kq) >= 0.0) and lo <= kq[0] and kq[-1] <= hi
+        mean = (ki * w).sum()
+        assert abs(ka - mean) <= _rel_bin(lo, hi, lo) * mean
+
+
+def test_single_pair_lands_in_its_bin_centre():
+    """Two points give one pair, a delta at ki = (k0 + k1)/2.  The bins are
+    [K(b-1), K(b)) with K(h) = (pmin + h step)^10 and the generator shifts
+    every k value half a bin up, so the k-distribution rises from the centre
+    of ki's bin (g -> 0) to the centre of the next one (g -> 1): kq(0+) is
+    within half a bin of ki and kq(1-) - kq(0+) is one bin.  A missing
+    shift, a bin index off by one or a missing pair average all fail this."""
+    rng = np.random.default_rng(11)
+    gq = np.array([1e-9, 1.0 - 1e-9])
+    for _ in range(300):
+        k0, k1 = np.sort(10.0 ** rng.uniform(-3.0, 3.0, 2))
+        kq, ka, lo, hi = _one([10.0, 10.01], [k0, k1], gq=gq)
+        ki = 0.5 * (k0 + k1)
+        step = (k1 ** PWR - k0 ** PWR) / (N_PWRK - 1)
+        w = step / PWR * ki ** (1.0 - PWR)                 # dK/dh at ki: one bin width
+        assert abs(kq[0] - ki) <= 0.5 * w * (1 + 1e-3), (k0, k1)
+        assert abs((kq[1] - kq[0]) - w) <= 1e-3 * w
+        assert abs(ka - kq[1]) <= 1e-6 * w                 # k_avg = the shifted value of ki's bin
+
+
+def test_scaling_by_two_to_the_tenth_is_exact():
+    """k -> 1024 k scales k^0.1 by exactly 2, so every bin stays the same and
+    kq, k_avg, kmin, kmax scale by 1024 (to the libm's pow rounding)."""
+    d = synthetic_lbl(temps=[800.0])
+    for s in (0, 100, 400):
+        b, e = d["off"][s], d["off"][s + 1]
+        a = _one(d["wvn"][b:e], d["kvn"][b:e])
+        c = _one(d["wvn"][b:e], 1024.0 * d["kvn"][b:e])
+        np.testing.assert_allclose(c[0], 1024.0 * a[0], rtol=1e-13)
+        assert abs(c[1] - 1024.0 * a[1]) <= 1e-13 * c[1]
+        assert c[2] == 1024.0 * a[2] and c[3] == 1024.0 * a[3]
+
+
+def test_flat_and_one_point_spectra():
+    """A flat spectrum (kmin == kmax; the reference divides 0 by 0 there) and
+    a single point both give kq = k; k_avg is k for the flat band and 0 for
+    the single point (no pair)."""
+    kq, ka, lo, hi = _one([5.0, 5.01, 5.02, 5.03], [2.5] * 4)
+    np.testing.assert_allclose(kq, 2.5, rtol=1e-14)
+    assert abs(ka - 2.5) <= 1e-14 * 2.5
+    kq, ka, lo, hi = _one([5.0], [7.0], wl=5.0, wr=5.01)
+    np.testing.assert_allclose(kq, 7.0, rtol=1e-14)
+    assert ka == 0.0 and lo == hi == 7.0
+
+
+def _gpu_vs_oracle(off, wvn, kvn, wvl, wvr, gq=GQ16_F64):
+    from radchan_amd.spectra import ckdist
+    g = ckdist(off, wvn, kvn, wvl, wvr, gq)
+    kq, ka, lo, hi = O.ckdist(off, wvn, kvn, wvl, wvr, gq)
+    assert np.array_equal(g["kmin"], lo) and np.array_equal(g["kmax"], hi)
+    # FP64 pow of the device and of the host libm may differ in the last bit
+    np.testing.assert_allclose(g["kq"], kq, rtol=1e-12, atol=0)
+    np.testing.assert_allclose(g["kavg"], ka, rtol=1e-12, atol=1e-300)
+    return g, kq
+
+
+@pytest.mark.gpu
+def test_gpu_ckdist_matches_oracle_synthetic():
+    d = synthetic_lbl(temps=[300.0, 1000.0, 2000.0, 2850.0])
+    print(f"rc_ckdist: {len(d['off']) - 1} spectra of {d['off'][1]} points", flush=True)
+    g, kq = _gpu_vs_oracle(d["off"], d["wvn"], d["kvn"], d["wvl"], d["wvr"])
+    assert g["kernel_ms"] > 0.0
+
+
+@pytest.mark.gpu
+def test_gpu_ckdist_ragged_and_edge_cases():
+    """Spectra of 1, 2, 3, 257, 1000, 4097 and 8192 points in one launch, a
+    flat one, one with a plateau at kmax (pairs there fall in bin n_pwrk and
+    leave gg and k_avg, as in the reference), 17 quadrature points."""
+    rng = np.random.default_rng(5)
+    sizes = [1, 2, 3, 257, 1000, 4097, 8192, 50, 64]
+    wv, kv, off, wl, wr = [], [], [0], [], []
+    for i, n in enumerate(sizes):
+        nu = 2000.0 + 0.01 * np.arange(n)
+        if i == 7:
+            k = np.full(n, 3.25)
+        elif i == 8:
+            k = np.concatenate([rng.uniform(0.1, 1.0, 40), np.full(24, 2.0)])
+        else:
+            k = 10.0 ** rng.uniform(-2.0, 3.0, n)
+        wv.append(nu)
+        kv.append(k)
+        off.append(off[-1] + n)
+        wl.append(nu[0])
+        wr.append(nu[-1] if n > 1 else nu[0] + 0.01)
+    gq = np.sort(np.concatenate([GQ16_F64, [0.5]]))
+    print(f"rc_ckdist: ragged sizes {sizes}", flush=True)
+    g, kq = _gpu_vs_oracle(np.array(off), np.concatenate(wv), np.concatenate(kv), np.array(wl), np.array(wr), gq)
+    np.testing.assert_allclose(g["kq"][7], 3.25, rtol=1e-14)
+
+
+@pytest.mark.gpu
+def test_gpu_ckdist_rejects_bad_input():
+    from radchan_amd.spectra import ckdist
+    nu = 100.0 + 0.01 * np.arange(8193)
+    with pytest.raises(RuntimeError, match="8193 points"):
+        ckdist([0, 8193], nu, np.ones(8193), [nu[0]], [nu[-1]], GQ16_F64)
+    with pytest.raises(RuntimeError, match="nondecreasing"):
+        ckdist([0, 10], nu[:10], np.ones(10), [nu[0]], [nu[9]], GQ16_F64[::-1].copy())
