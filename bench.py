@@ -13,7 +13,8 @@ Legs: an untimed autotune of the batched schedule; the batched solver with
 FP32 and with BF16 k-tables, each replayed as a hipGraph; the batched march
 kernel alone (HIP events, no graph) for the roofline; the plain solver; one
 solve at 4x photmax (checks eps ~ rays^-1/2, which the time-to-1%
-projection assumes); the 256x128x256 field, batched against plain.
+projection assumes); the 256x128x256 field, batched against plain; the
+correlated-k table generator (rc_ckdist) on 47 580 line-by-line spectra.
 
 Printed (one JSON line on stdout): rays/s of the batched solver (FP32
 k-tables unless --tables bf16), the plain solver's rays/s beside it, the
@@ -159,6 +160,46 @@ def target_leg(lib, tab):
             "time_to_1pct_speedup_vs_plain": tp / t1}
 
 
+def ckdist_leg(cpu: bool, reps: int = 5):
+    """The correlated-k generator (rc_ckdist) on line-by-line spectra of the
+    reference generator's layout: 10 cm^-1 bands over 150-9300 cm^-1 at
+    dwv 0.01 (915 bands x 1000 points) at 52 temperatures, 20000 k^0.1 bins,
+    16 g-points.  Kernel time from HIP events on the call's stream; the call
+    time adds the host-to-device copy of the spectra.  CPU baseline: the
+    oracle's restatement on a 2000-spectrum sample."""
+    from radchan_amd.spectra import ckdist, synthetic_lbl
+    from radchan_amd.tables import GQ16_F64
+    d = synthetic_lbl()
+    ns, npts, nq = len(d["off"]) - 1, len(d["wvn"]), len(GQ16_F64)
+    args = (d["off"], d["wvn"], d["kvn"], d["wvl"], d["wvr"], GQ16_F64)
+    ckdist(*args)                                        # warm-up
+    ks, calls = [], []
+    for _ in range(reps):
+        t0 = time.perf_counter()
+        g = ckdist(*args)
+        calls.append((time.perf_counter() - t0) * 1e3)
+        ks.append(g["kernel_ms"])
+    kms = float(np.mean(ks))
+    alg = npts * 16 + ns * (8 * (nq + 3) + 8 * 3)       # points in; kq, kavg, kmin, kmax out; edges, offsets
+    out = {"spectra": ns, "points": npts, "n_pwrk": 20000, "nq": nq, "kernel_ms": round(kms, 4),
+           "call_ms_with_h2d": round(float(np.mean(calls)), 2), "spectra_per_s": ns / (kms * 1e-3),
+           "points_per_s": npts / (kms * 1e-3), "algorithmic_bytes": int(alg),
+           "hbm_frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "cpu_baseline": None}
+    if cpu:
+        from oracle import oracle as O
+        sub = 2000
+        nth = min(os.cpu_count() or 1, 16)
+        t0 = time.perf_counter()
+        O.ckdist(d["off"][:sub + 1], d["wvn"], d["kvn"], d["wvl"][:sub], d["wvr"][:sub], GQ16_F64, nthreads=nth)
+        dt = time.perf_counter() - t0
+        out["cpu_baseline"] = {"value": sub / dt, "unit": "spectra/s", "cores": nth, "kind": "port",
+                               "sample": f"first {sub} of the {ns} spectra"}
+        out["speedup_vs_cpu"] = out["spectra_per_s"] / (sub / dt)
+    log(f"ckdist: {ns} spectra x {npts // ns} points in {kms:.3f} ms (kernel), "
+        f"{out['call_ms_with_h2d']:.1f} ms with the copy in")
+    return out
+
+
 def bench_c4(args) -> int:
     """The snapshot ensemble: 8 snapshots of 64x96x64 and 4 of 256x128x256
     (laminar 300-1800 K profile plus 5% smoothed noise, its own noise seed
@@ -251,6 +292,7 @@ def main() -> int:
     ap.add_argument("--no-plain", action="store_true")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
+    ap.add_argument("--no-ckdist", action="store_true", help="skip the correlated-k generator leg")
     ap.add_argument("--no-target", action="store_true",
                     help="skip the 256x128x256 batched-vs-plain leg")
     ap.add_argument("--probe", action="store_true",
@@ -364,6 +406,8 @@ def main() -> int:
     if not args.no_target and args.workload != "big256" and not p.grey:
         target = target_leg(lib, tab)
 
+    ck = None if args.no_ckdist else ckdist_leg(not args.no_cpu)
+
     cpu = None
     if not args.no_cpu:
         try:
@@ -401,6 +445,7 @@ def main() -> int:
         "time_to_1pct_speedup_vs_plain": (plain["time_to_1pct_s"] / head["time_to_1pct_s"]) if plain else None,
         "eps_scaling": scal,
         "target_256x128x256": target,
+        "ckdist": ck,
         "roofline": roof,
         "cpu_baseline": cpu,
     }

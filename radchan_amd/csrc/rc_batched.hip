@@ -55,20 +55,6 @@ __device__ __forceinline__ void lds_sort(K *lds, int n2) {
         }
 }
 
-/* the value of lane (lane ^ m) of a wavefront: quad permutes for m = 1, 2,
- * LDS-crossbar swizzles within 32 lanes for m = 4, 8, 16, a permute for 32
- * (m is a constant once the caller's loops are unrolled) */
-__device__ __forceinline__ uint32_t lane_xor(uint32_t v, int m) {
-    switch (m) {
-        case 1: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
-        case 2: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);
-        case 4: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x101F);
-        case 8: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x201F);
-        case 16: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F);
-        default: return (uint32_t)__shfl_xor((int)v, m);
-    }
-}
-
 /* bitonic sort of 64 R keys held in registers, element R lane + r in x[r]:
  * compare-exchanges at distances below R stay inside a lane; the stages at
  * distance R..32 R exchange with lane (lane ^ distance/R) (21 of them).  No

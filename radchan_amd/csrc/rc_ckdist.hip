@@ -21,22 +21,27 @@
  *
  * One workgroup of 256 threads per spectrum, everything in LDS:
  *   (1) kmin/kmax: block min/max (exact in any order);
- *   (2) per pair h = 1..n-1 the key bin << 13 | h; bitonic sort in LDS, so
- *       that pairs of one bin come in ascending h;
+ *   (2) per pair h = 1..n-1 the key bin << 13 | h, sorted (so that pairs of
+ *       one bin come in ascending h) by a bitonic network held in registers:
+ *       only 3 of its stages cross wavefronts through LDS;
  *   (3) run starts (occupied bins) compacted by a block scan;
- *   (4) per occupied bin, ff summed in ascending h by one thread -- the
- *       reference's order, so ff is bit for bit the reference's;
+ *   (4) per occupied bin, ff summed in ascending h by one thread from the
+ *       pair weights staged in LDS -- the reference's order, so ff is bit
+ *       for bit the reference's;
  *   (5) the gg chain (one lane, descending bins) and the k_avg sum (one lane
  *       of another wave, ascending bins): the reference's sequences of
  *       operations with the zero terms of empty bins left out (x - 0 and
  *       x + 0 are exact), so both are bit for bit given the same kk;
  *   (6) one thread per quadrature point: binary searches for the first
  *       occupied bin whose gg reaches gq[r].
- * kk and the bin index use the device's FP64 pow, which may differ from the
- * host libm's in the last bit: results agree with the CPU restatement to FP64
- * rounding (and a pair exactly on a bin edge could land one bin over).
+ * The bin index uses the device's FP64 pow, which may differ from the host
+ * libm's in the last bit, and kk = (h step + pmin)^(1/pwr) is formed by
+ * squaring when 1/pwr is an integer (10 for pwr = 0.1), within 5 ulp of
+ * pow: results agree with the CPU restatement to FP64 rounding (and a pair
+ * exactly on a bin edge could land one bin over).
  *
- * LDS per spectrum, np2 = pow2 >= n - 1 (at least 256): keys u32[np2], run
+ * LDS per spectrum, np2 = 256 R >= n - 1 (R = 1, 2, .., 32 keys per thread,
+ * one kernel instance each, chosen by the launch's longest spectrum): keys u32[np2], run
  * starts u32[np2], ff f64[np2], run bins u16[np2] = 18 B per slot; the
  * products kk*ff (f64[np2]) overlay keys and run starts once those are dead.
  * Up to 8192 points per spectrum (13-bit pair index), n_pwrk <= 65535.
@@ -49,7 +54,10 @@
 #include <string>
 #include <vector>
 
+#include "rc_device.h"
 #include "rc_internal.h"
+
+using namespace rcd;
 
 namespace rc {
 namespace {
@@ -64,14 +72,88 @@ struct CkArgs {
     const double *wvn, *kvn, *wvl, *wvr, *gq;
     double *kq, *kavg, *kmin, *kmax;
     double pwr, ipwr;
-    int nq, n_pwrk, np2;
+    int nq, n_pwrk, ie;
 };
 
-__device__ __forceinline__ double kk_at(int h, double step, double pmin, double ipwr) {
+/* bitonic sort of 256 R keys held in registers, element R tid + r in x[r]:
+ * stages at distance below R stay inside a thread, distances R..32 R
+ * exchange with lane (lane ^ distance/R) of the same wavefront, and only
+ * the 64 R and 128 R distances (3 of the log2(256R)(log2(256R)+1)/2
+ * stages) go through LDS, with a barrier on each side. */
+template <int R>
+__device__ __forceinline__ void block_sort(uint32_t (&x)[R], uint32_t *K, int tid) {
+    int base = tid * R;
+    /* opaque, so the per-stage masks are formed here rather than hoisted */
+    asm volatile("" : "+v"(base));
+#pragma unroll
+    for (int k = 2; k <= 256 * R; k <<= 1) {
+#pragma unroll
+        for (int j = k >> 1; j > 0; j >>= 1) {
+            if (j < R) {
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const int p = r ^ j;
+                    if (p > r) {
+                        const bool up = ((base + r) & k) == 0;
+                        const uint32_t lo = min(x[r], x[p]), hi = max(x[r], x[p]);
+                        x[r] = up ? lo : hi;
+                        x[p] = up ? hi : lo;
+                    }
+                }
+            } else if (j < 64 * R) {
+                const bool keep_min = ((base & j) == 0) == ((base & k) == 0);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint32_t y = lane_xor(x[r], j / R);
+                    x[r] = keep_min ? min(x[r], y) : max(x[r], y);
+                }
+            } else {
+                __syncthreads();
+#pragma unroll
+                for (int r = 0; r < R; ++r) K[base + r] = x[r];
+                __syncthreads();
+                const bool keep_min = ((base & j) == 0) == ((base & k) == 0);
+#pragma unroll
+                for (int r = 0; r < R; ++r) {
+                    const uint32_t y = K[(base + r) ^ j];
+                    x[r] = keep_min ? min(x[r], y) : max(x[r], y);
+                }
+            }
+        }
+    }
+    __syncthreads();
+#pragma unroll
+    for (int r = 0; r < R; ++r) K[base + r] = x[r];
+    __syncthreads();
+}
+
+/* x^(1/pwr): by squaring when 1/pwr is a small integer (ie > 0; 10 for the
+ * generator's pwr = 0.1: four products, within 5 ulp of the rounded power),
+ * else pow */
+__device__ __forceinline__ double kpow(double x, double ipwr, int ie) {
+    if (ie > 0) {
+        double r = 1.0, b = x;
+        for (int e = ie; e; e >>= 1) {
+            if (e & 1) r = r * b;
+            if (e > 1) b = b * b;
+        }
+        return r;
+    }
+    return pow(x, ipwr);
+}
+
+__device__ __forceinline__ double kk_at(int h, double step, double pmin, double ipwr, int ie) {
     /* kk[h] after the half-bin shift: 0.5 * (K(h) + K(h+1)), K(h) = (h*step + pmin)^(1/pwr) */
-    const double a = pow((double)h * step + pmin, ipwr);
-    const double b = pow((double)(h + 1) * step + pmin, ipwr);
+    const double a = kpow((double)h * step + pmin, ipwr, ie);
+    const double b = kpow((double)(h + 1) * step + pmin, ipwr, ie);
     return 0.5 * (a + b);
+}
+
+/* the same, out of line: the quadrature phase calls it four times, and
+ * inlined there it raises the kernel's register peak (and lowers its
+ * occupancy) */
+__device__ __noinline__ double kk_at_q(int h, double step, double pmin, double ipwr, int ie) {
+    return kk_at(h, step, pmin, ipwr, ie);
 }
 
 /* gg(j) = G_lo of the first occupied bin above j, 1 above the last */
@@ -84,6 +166,7 @@ __device__ __forceinline__ double gg_at(int j, const uint16_t *RB, const double 
     return lo < r1 ? G[lo] : 1.0;
 }
 
+template <int R>
 __global__ __launch_bounds__(CK_THREADS) void k_ckdist(CkArgs a) {
     extern __shared__ __align__(16) unsigned char lds[];
     __shared__ double red_lo[CK_WAVES], red_hi[CK_WAVES];
@@ -92,7 +175,7 @@ __global__ __launch_bounds__(CK_THREADS) void k_ckdist(CkArgs a) {
 
     const int s = blockIdx.x;
     const int tid = threadIdx.x, lane = tid & 63, wv = tid >> 6;
-    const int np2 = a.np2;
+    constexpr int np2 = CK_THREADS * R;
     uint32_t *K = (uint32_t *)lds;                      /* [np2] sort keys          */
     uint32_t *RS = K + np2;                             /* [np2] run starts         */
     double *P = (double *)lds;                          /* [np2] kk*ff (over K, RS) */
@@ -124,43 +207,38 @@ __global__ __launch_bounds__(CK_THREADS) void k_ckdist(CkArgs a) {
         hi = hi < red_hi[w] ? red_hi[w] : hi;
     }
     const int n_pwrk = a.n_pwrk, ng = n_pwrk - 1;
-    const double pmin = pow(lo, a.pwr), pmax = pow(hi, a.pwr);
+    /* k^pwr of kmin (even lanes) and kmax (odd lanes): one pow per wave */
+    const double pv = pow((lane & 1) ? hi : lo, a.pwr);
+    const double pmin = __shfl(pv, 0), pmax = __shfl(pv, 1);
     const double step = (pmax - pmin) / (double)(n_pwrk - 1);
     const double width = a.wvr[s] - a.wvl[s];
     const int npair = n > 1 ? n - 1 : 0;
 
-    /* (2) keys: bin << 13 | h for the pair (h-1, h); padding sorts last */
-    for (int i = tid; i < np2; i += CK_THREADS) {
+    /* (2) keys: bin << 13 | h for the pair (h-1, h), pair i = R tid + r in
+     * x[r]; padding sorts last */
+    uint32_t x[R];
+#pragma unroll
+    for (int r = 0; r < R; ++r) {
+        const int i = tid * R + r;
         uint32_t key = 0xFFFFFFFFu;
         if (i < npair) {
             const int h = i + 1;
             const double ki = (kvs[h] + kvs[h - 1]) / 2;
             int bin = 1;                                 /* a flat spectrum (step 0) */
             if (step > 0.0) {
-                const double x = (pow(ki, a.pwr) - pmin) / step + 1;
-                bin = max(0, (int)x);
+                const double xb = (pow(ki, a.pwr) - pmin) / step + 1;
+                bin = max(0, (int)xb);
                 bin = min(bin, n_pwrk);
             }
             key = ((uint32_t)bin << CK_IDX_BITS) | (uint32_t)h;
+            F[i] = (wvs[h] - wvs[h - 1]) / width;        /* the pair's weight, by pair */
         }
-        K[i] = key;
+        x[r] = key;
     }
-    for (int k = 2; k <= np2; k <<= 1) {
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            __syncthreads();
-            for (int i = tid; i < np2; i += CK_THREADS) {
-                const int p = i ^ j;
-                if (p > i) {
-                    const uint32_t x = K[i], y = K[p];
-                    if ((x > y) == ((i & k) == 0)) { K[i] = y; K[p] = x; }
-                }
-            }
-        }
-    }
-    __syncthreads();
+    block_sort<R>(x, K, tid);
 
-    /* (3) run starts, compacted in order: each thread scans a contiguous chunk */
-    const int chunk = np2 / CK_THREADS;
+    /* (3) run starts, compacted in order: each thread scans its R keys */
+    constexpr int chunk = R;
     const int c0 = tid * chunk;
     int cnt = 0;
     for (int i = c0; i < c0 + chunk; ++i)
@@ -185,21 +263,33 @@ __global__ __launch_bounds__(CK_THREADS) void k_ckdist(CkArgs a) {
     __syncthreads();
     const int m = s_m;
 
-    /* (4) ff per occupied bin, summed in ascending h */
-    for (int r = tid; r < m; r += CK_THREADS) {
-        const int b = (int)RS[r], e = r + 1 < m ? (int)RS[r + 1] : npair;
+    /* (4) ff per occupied bin, summed in ascending h from the pair weights
+     * (held in F by pair); the sums replace them after a barrier */
+    double fr[R];
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        const int r = tid + u * CK_THREADS;
         double f = 0.0;
-        for (int q = b; q < e; ++q) {
-            const int h = (int)(K[q] & ((1u << CK_IDX_BITS) - 1));
-            f = f + (wvs[h] - wvs[h - 1]) / width;
+        if (r < m) {
+            const int b = (int)RS[r], e = r + 1 < m ? (int)RS[r + 1] : npair;
+            for (int q = b; q < e; ++q) {
+                const int h = (int)(K[q] & ((1u << CK_IDX_BITS) - 1));
+                f = f + F[h - 1];
+            }
         }
-        F[r] = f;
+        fr[u] = f;
+    }
+    __syncthreads();
+#pragma unroll
+    for (int u = 0; u < R; ++u) {
+        const int r = tid + u * CK_THREADS;
+        if (r < m) F[r] = fr[u];
     }
     __syncthreads();
     /* kk*ff of the bins the k_avg sum covers (1..ng), over the dead keys */
     for (int r = tid; r < m; r += CK_THREADS) {
         const int bin = RB[r];
-        P[r] = (bin >= 1 && bin <= ng) ? kk_at(bin, step, pmin, a.ipwr) * F[r] : 0.0;
+        P[r] = (bin >= 1 && bin <= ng) ? kk_at(bin, step, pmin, a.ipwr, a.ie) * F[r] : 0.0;
     }
     if (tid == 0) {
         int r0 = (m > 0 && RB[0] == 0) ? 1 : 0;
@@ -262,7 +352,7 @@ __global__ __launch_bounds__(CK_THREADS) void k_ckdist(CkArgs a) {
         }
         const int t = jstar - 1;
         const double g0 = gg_at(t, RB, F, r0, r1), g1 = gg_at(t + 1, RB, F, r0, r1);
-        const double k0 = kk_at(t, step, pmin, a.ipwr), k1 = kk_at(t + 1, step, pmin, a.ipwr);
+        const double k0 = kk_at_q(t, step, pmin, a.ipwr, a.ie), k1 = kk_at_q(t + 1, step, pmin, a.ipwr, a.ie);
         double v = k0 + (k1 - k0) * (g - g0) / (g1 - g0);
         if (g1 == g0) v = k1;
         a.kq[(size_t)s * a.nq + q] = v;
@@ -375,11 +465,18 @@ extern "C" int rc_ckdist(int nspec, const int64_t *off, const double *wvn, const
         a.kmax = (double *)d_kmax.p;
         a.pwr = pwr;
         a.ipwr = 1.0 / pwr;
+        a.ie = (a.ipwr == std::rint(a.ipwr) && a.ipwr >= 1.0 && a.ipwr <= 64.0) ? (int)a.ipwr : 0;
         a.nq = nq;
         a.n_pwrk = n_pwrk;
-        a.np2 = np2;
         CK_TRY(hipEventRecord(e0, st));
-        hipLaunchKernelGGL(k_ckdist, dim3(nspec), dim3(CK_THREADS), lds, st, a);
+        switch (np2 / CK_THREADS) {
+            case 1: hipLaunchKernelGGL(k_ckdist<1>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
+            case 2: hipLaunchKernelGGL(k_ckdist<2>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
+            case 4: hipLaunchKernelGGL(k_ckdist<4>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
+            case 8: hipLaunchKernelGGL(k_ckdist<8>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
+            case 16: hipLaunchKernelGGL(k_ckdist<16>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
+            default: hipLaunchKernelGGL(k_ckdist<32>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
+        }
         CK_TRY(hipGetLastError());
         CK_TRY(hipEventRecord(e1, st));
         CK_TRY(hipMemcpyAsync(kq, d_kq.p, nsb * nq, hipMemcpyDeviceToHost, st));

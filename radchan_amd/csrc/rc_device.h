@@ -52,6 +52,20 @@ __device__ __forceinline__ float4 gld32(const float4 *p, uint32_t i) {
     return make_float4(v.x, v.y, v.z, v.w);
 }
 
+/* the value of lane (lane ^ m) of a wavefront: quad permutes for m = 1, 2,
+ * LDS-crossbar swizzles within 32 lanes for m = 4, 8, 16, a permute for 32
+ * (m is a constant once the caller's loops are unrolled) */
+__device__ __forceinline__ uint32_t lane_xor(uint32_t v, int m) {
+    switch (m) {
+        case 1: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0xB1, 0xF, 0xF, false);
+        case 2: return (uint32_t)__builtin_amdgcn_mov_dpp((int)v, 0x4E, 0xF, 0xF, false);
+        case 4: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x101F);
+        case 8: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x201F);
+        case 16: return (uint32_t)__builtin_amdgcn_ds_swizzle((int)v, 0x401F);
+        default: return (uint32_t)__shfl_xor((int)v, m);
+    }
+}
+
 /* ---- Philox4x32-10 (Salmon, Moraes, Dror, Shaw, "Parallel random numbers:
  * as easy as 1, 2, 3", SC'11): 10 rounds of two 32x32->64 multiplies with
  * a Weyl-sequence key schedule. ---------------------------------------- */

@@ -117,10 +117,10 @@ def test_flat_and_one_point_spectra():
     assert ka == 0.0 and lo == hi == 7.0
 
 
-def _gpu_vs_oracle(off, wvn, kvn, wvl, wvr, gq=GQ16_F64):
+def _gpu_vs_oracle(off, wvn, kvn, wvl, wvr, gq=GQ16_F64, **kw):
     from radchan_amd.spectra import ckdist
-    g = ckdist(off, wvn, kvn, wvl, wvr, gq)
-    kq, ka, lo, hi = O.ckdist(off, wvn, kvn, wvl, wvr, gq)
+    g = ckdist(off, wvn, kvn, wvl, wvr, gq, **kw)
+    kq, ka, lo, hi = O.ckdist(off, wvn, kvn, wvl, wvr, gq, **kw)
     assert np.array_equal(g["kmin"], lo) and np.array_equal(g["kmax"], hi)
     # FP64 pow of the device and of the host libm may differ in the last bit
     np.testing.assert_allclose(g["kq"], kq, rtol=1e-12, atol=0)
@@ -137,18 +137,30 @@ def test_gpu_ckdist_matches_oracle_synthetic():
 
 
 @pytest.mark.gpu
-def test_gpu_ckdist_ragged_and_edge_cases():
-    """Spectra of 1, 2, 3, 257, 1000, 4097 and 8192 points in one launch, a
-    flat one, one with a plateau at kmax (pairs there fall in bin n_pwrk and
+@pytest.mark.parametrize("pwr,n_pwrk", [(0.3, 5000), (0.25, 65535)])
+def test_gpu_ckdist_other_bin_laws(pwr, n_pwrk):
+    """Bin laws other than the generator's k^0.1 in 20000 bins: 1/pwr not an
+    integer (the kk values through pow) and an integer one, at the largest
+    bin count."""
+    d = synthetic_lbl(temps=[700.0], deltawv=25.0)
+    _gpu_vs_oracle(d["off"], d["wvn"], d["kvn"], d["wvl"], d["wvr"], n_pwrk=n_pwrk, pwr=pwr)
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("longest", [257, 513, 1025, 2049, 4097, 8192])
+def test_gpu_ckdist_ragged_and_edge_cases(longest):
+    """Spectra of 1, 2, 3, 257 and up to `longest` points in one launch (the
+    longest picks the kernel instance: 256, 512, .., 8192 sort slots), a flat
+    one, one with a plateau at kmax (pairs there fall in bin n_pwrk and
     leave gg and k_avg, as in the reference), 17 quadrature points."""
-    rng = np.random.default_rng(5)
-    sizes = [1, 2, 3, 257, 1000, 4097, 8192, 50, 64]
+    rng = np.random.default_rng(longest)
+    sizes = [1, 2, 3, min(257, longest), longest // 2 + 1, longest, 50, 64]
     wv, kv, off, wl, wr = [], [], [0], [], []
     for i, n in enumerate(sizes):
         nu = 2000.0 + 0.01 * np.arange(n)
-        if i == 7:
+        if i == 6:
             k = np.full(n, 3.25)
-        elif i == 8:
+        elif i == 7:
             k = np.concatenate([rng.uniform(0.1, 1.0, 40), np.full(24, 2.0)])
         else:
             k = 10.0 ** rng.uniform(-2.0, 3.0, n)
@@ -160,7 +172,7 @@ def test_gpu_ckdist_ragged_and_edge_cases():
     gq = np.sort(np.concatenate([GQ16_F64, [0.5]]))
     print(f"rc_ckdist: ragged sizes {sizes}", flush=True)
     g, kq = _gpu_vs_oracle(np.array(off), np.concatenate(wv), np.concatenate(kv), np.array(wl), np.array(wr), gq)
-    np.testing.assert_allclose(g["kq"][7], 3.25, rtol=1e-14)
+    np.testing.assert_allclose(g["kq"][6], 3.25, rtol=1e-14)
 
 
 @pytest.mark.gpu
