@@ -159,6 +159,16 @@ int rc_sort_idx(const float *kavg, int nB, int32_t *idx);
 int rc_ckdist(int nspec, const int64_t *off, const double *wvn, const double *kvn, const double *wvl,
               const double *wvr, int n_pwrk, double pwr, int nq, const double *gq, double *kq,
               double *kavg, double *kmin, double *kmax, double *kernel_ms);
+/* the generator's table step, calcProbT (nbcorrk/calc_prob.cpp:17-106),
+ * on rc_ckdist's output: kq[nT][nB][nQ] and kavg[nT][nB] (FP64, row-major
+ * by temperature T[nT], increasing), band edges wvl/wvr[nB], weights
+ * wq[nQ].  Fills t (sizes nB, nT, nQ; caller-owned arrays) as
+ * rc_read_tables leaves them: FP32, k values times chanHeight, kavg of the
+ * last row, prob = band CDF, probg = g CDF, kP = the correlated-k Planck
+ * mean (also in place of the line-by-line one the reference writes there). */
+int rc_ck_tables(int nB, int nT, int nQ, const double *T, const double *wvl, const double *wvr,
+                 const double *kq, const double *kavg, const double *wq, double chanHeight,
+                 rc_tables *t);
 /* Gridn::mk_grid (definitions.h:68-136) of one level */
 int rc_grid(const rc_params *p, int level, float *x, float *xu, float *y,
             float *yu, float *z, float *zu);

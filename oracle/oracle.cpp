@@ -1095,11 +1095,55 @@ int orc_solve_surface(const orc_ctx *c, int precision, const int32_t *cells, int
     return 0;
 }
 
-/* correlatedKT (nbcorrk/correlatedK.cpp:144-225): per spectrum, the
- * k^pwr histogram ff, the interval centres kk, the cumulative gg and the
- * quadrature values kq; kavg as calcProbT sums it (calc_prob.cpp:52-56).
+/* correlatedKT (nbcorrk/correlatedK.cpp:144-225) of one spectrum, points
+ * b .. e-1: the k^pwr histogram ff, the shifted interval values kk (both
+ * n_pwrk + 1 long, left for the caller), the quadrature values kq[nq] and
+ * kavg as calcProbT sums it (calc_prob.cpp:52-56).
  * Deviations: bins are clamped to [0, n_pwrk] and a band of constant k
  * (zero bin width) puts every point in bin 1. */
+static void ck_one(int64_t b, int64_t e, const double *wvn, const double *kvn, double wl, double wr, int n_pwrk,
+                   double pwr, int nq, const double *gq, std::vector<double> &kk, std::vector<double> &ff,
+                   double *kq, double *kavg, double *kmin, double *kmax) {
+    double lo = 1e10, hi = -1000.0;
+    for (int64_t h = b; h < e; ++h) {
+        lo = std::min(lo, kvn[h]);
+        hi = std::max(hi, kvn[h]);
+    }
+    *kmin = lo;
+    *kmax = hi;
+    const double pmin = std::pow(lo, pwr), pmax = std::pow(hi, pwr);
+    const double step = (pmax - pmin) / (n_pwrk - 1);
+    ff.assign(n_pwrk + 1, 0.0);
+    kk.resize(n_pwrk + 1);
+    std::vector<double> gg(n_pwrk + 1, 0.0);
+    for (int h = 0; h < n_pwrk + 1; ++h) kk[h] = std::pow(h * step + pmin, 1.0 / pwr);
+    const double width = wr - wl;
+    for (int64_t h = b + 1; h < e; ++h) {
+        const double ki = (kvn[h] + kvn[h - 1]) / 2;
+        int bin = 1;
+        if (step > 0.0) {
+            const double x = (std::pow(ki, pwr) - pmin) / step + 1;
+            bin = std::max(0, (int)x);
+            bin = std::min(bin, n_pwrk);
+        }
+        ff[bin] = ff[bin] + (wvn[h] - wvn[h - 1]) / width;
+    }
+    const int ng = n_pwrk - 1;
+    for (int h = 0; h < n_pwrk; ++h) kk[h] = 0.5 * (kk[h] + kk[h + 1]);
+    gg[ng] = 1.0;
+    for (int h = ng; h > 0; --h) gg[h - 1] = gg[h] - ff[h];
+    int t = 0;
+    for (int r = 0; r < nq; ++r) {
+        while (t + 1 < ng && !(gg[t + 1] >= gq[r])) ++t;
+        double v = kk[t] + (kk[t + 1] - kk[t]) * (gq[r] - gg[t]) / (gg[t + 1] - gg[t]);
+        if (gg[t + 1] == gg[t]) v = kk[t + 1];
+        kq[r] = v;
+    }
+    double a = 0.0;
+    for (int h = 1; h < n_pwrk; ++h) a += kk[h] * ff[h];
+    *kavg = a;
+}
+
 int orc_ckdist(int nspec, const int64_t *off, const double *wvn, const double *kvn, const double *wvl,
                const double *wvr, int n_pwrk, double pwr, int nq, const double *gq, double *kq, double *kavg,
                double *kmin, double *kmax, int nthreads) {
@@ -1109,43 +1153,68 @@ int orc_ckdist(int nspec, const int64_t *off, const double *wvn, const double *k
 #pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
 #endif
     for (int s = 0; s < nspec; ++s) {
-        const int64_t b = off[s], e = off[s + 1];
-        double lo = 1e10, hi = -1000.0;
-        for (int64_t h = b; h < e; ++h) {
-            lo = std::min(lo, kvn[h]);
-            hi = std::max(hi, kvn[h]);
+        std::vector<double> kk, ff;
+        ck_one(off[s], off[s + 1], wvn, kvn, wvl[s], wvr[s], n_pwrk, pwr, nq, gq, kk, ff, kq + (size_t)s * nq,
+               kavg + s, kmin + s, kmax + s);
+    }
+    return 0;
+}
+
+/* I_black of the generator (nbcorrk/ckdist_hitran_part.cpp:229-242): the
+ * exponential in single precision, as there */
+static double ck_iblack(double T, double nu) {
+    const double C1 = 3.741771790075259e-16, C2 = 0.014387741858429;
+    return 1.0 / 3.14159265358979323846 * C1 * std::pow((nu * 100), 3) / (expf((float)(C2 * nu * 100 / T)) - 1);
+}
+
+/* calcProbT (nbcorrk/calc_prob.cpp:17-106) after correlatedKT, on spectra
+ * ordered row-major (spectrum i*nB + j is band j at temperature T[i]):
+ * per row the correlated-k Planck mean kP (its per-bin sum over every band),
+ * the band CDF cuprob (per-bin sums over kP) and the g CDF cuprob_g of
+ * every band (kq Ib dnu 100 wq, cumulated and divided by the last); plus
+ * kq and kavg of every spectrum.  Sums run in the reference's order. */
+int orc_ck_tables(int nB, int nT, const int64_t *off, const double *wvn, const double *kvn, const double *wvl,
+                  const double *wvr, const double *T, int n_pwrk, double pwr, int nq, const double *gq,
+                  const double *wq, double *kq, double *kavg, double *kP, double *cuprob, double *cuprob_g,
+                  int nthreads) {
+    if (nB < 1 || nT < 1 || n_pwrk < 2 || nq < 1 || !(pwr > 0.0)) return -1;
+    const double pi = 3.14159265358979323846, stefan = 5.670373e-08;
+#ifdef _OPENMP
+    if (nthreads <= 0) nthreads = omp_get_max_threads();
+#pragma omp parallel for schedule(dynamic, 1) num_threads(nthreads)
+#endif
+    for (int i = 0; i < nT; ++i) {
+        std::vector<double> kk, ff;
+        double lo, hi;
+        double kPi = 0.0;
+        for (int j = 0; j < nB; ++j) {
+            const int s = i * nB + j;
+            ck_one(off[s], off[s + 1], wvn, kvn, wvl[s], wvr[s], n_pwrk, pwr, nq, gq, kk, ff,
+                   kq + (size_t)s * nq, kavg + s, &lo, &hi);
+            const double wvc = (wvl[s] + wvr[s]) / 2;
+            for (int h = 1; h < n_pwrk; ++h)
+                kPi += kk[h] * ck_iblack(T[i], wvc) * (ff[h]) * (wvr[s] - wvl[s]) * 100 * pi /
+                       (stefan * std::pow(T[i], 4));
         }
-        kmin[s] = lo;
-        kmax[s] = hi;
-        const double pmin = std::pow(lo, pwr), pmax = std::pow(hi, pwr);
-        const double step = (pmax - pmin) / (n_pwrk - 1);
-        std::vector<double> ff(n_pwrk + 1, 0.0), gg(n_pwrk + 1, 0.0), kk(n_pwrk + 1);
-        for (int h = 0; h < n_pwrk + 1; ++h) kk[h] = std::pow(h * step + pmin, 1.0 / pwr);
-        const double width = wvr[s] - wvl[s];
-        for (int64_t h = b + 1; h < e; ++h) {
-            const double ki = (kvn[h] + kvn[h - 1]) / 2;
-            int bin = 1;
-            if (step > 0.0) {
-                const double x = (std::pow(ki, pwr) - pmin) / step + 1;
-                bin = std::max(0, (int)x);
-                bin = std::min(bin, n_pwrk);
+        kP[i] = kPi;
+        for (int j = 0; j < nB; ++j) {
+            const int s = i * nB + j;
+            double dummy;
+            ck_one(off[s], off[s + 1], wvn, kvn, wvl[s], wvr[s], n_pwrk, pwr, nq, gq, kk, ff,
+                   kq + (size_t)s * nq, &dummy, &lo, &hi);
+            const double wvc = (wvl[s] + wvr[s]) / 2;
+            double prob = 0.0;
+            for (int h = 1; h < n_pwrk; ++h)
+                prob += kk[h] * ck_iblack(T[i], wvc) * (ff[h]) * (wvr[s] - wvl[s]) * 100 * pi /
+                        (kPi * stefan * std::pow(T[i], 4));
+            cuprob[s] = j > 0 ? cuprob[s - 1] + prob : prob;
+            double *cg = cuprob_g + (size_t)s * nq;
+            for (int g = 0; g < nq; ++g) {
+                const double p2 = kq[(size_t)s * nq + g] * ck_iblack(T[i], wvc) * (wvr[s] - wvl[s]) * 100 * wq[g];
+                cg[g] = g == 0 ? p2 : cg[g - 1] + p2;
             }
-            ff[bin] = ff[bin] + (wvn[h] - wvn[h - 1]) / width;
+            for (int g = 0; g < nq; ++g) cg[g] /= cg[nq - 1];
         }
-        const int ng = n_pwrk - 1;
-        for (int h = 0; h < n_pwrk; ++h) kk[h] = 0.5 * (kk[h] + kk[h + 1]);
-        gg[ng] = 1.0;
-        for (int h = ng; h > 0; --h) gg[h - 1] = gg[h] - ff[h];
-        int t = 0;
-        for (int r = 0; r < nq; ++r) {
-            while (t + 1 < ng && !(gg[t + 1] >= gq[r])) ++t;
-            double v = kk[t] + (kk[t + 1] - kk[t]) * (gq[r] - gg[t]) / (gg[t + 1] - gg[t]);
-            if (gg[t + 1] == gg[t]) v = kk[t + 1];
-            kq[(size_t)s * nq + r] = v;
-        }
-        double a = 0.0;
-        for (int h = 1; h < n_pwrk; ++h) a += kk[h] * ff[h];
-        kavg[s] = a;
     }
     return 0;
 }

@@ -161,6 +161,15 @@ int      orc_ckdist(int nspec, const int64_t *off, const double *wvn, const doub
                     const double *wvl, const double *wvr, int n_pwrk, double pwr, int nq,
                     const double *gq, double *kq, double *kavg, double *kmin, double *kmax,
                     int nthreads);
+/* The generator's table step, calcProbT (nbcorrk/calc_prob.cpp:17-106),
+ * after orc_ckdist's distributions: spectra row-major (s = i*nB + j, band j
+ * at temperature T[i]).  Outputs: kq[nT][nB][nq], kavg[nT][nB], kP[nT] (the
+ * correlated-k Planck mean), cuprob[nT][nB] (band CDF), cuprob_g[nT][nB][nq]
+ * (g CDF per band).  Sums in the reference's order, per bin. */
+int      orc_ck_tables(int nB, int nT, const int64_t *off, const double *wvn, const double *kvn,
+                       const double *wvl, const double *wvr, const double *T, int n_pwrk,
+                       double pwr, int nq, const double *gq, const double *wq, double *kq,
+                       double *kavg, double *kP, double *cuprob, double *cuprob_g, int nthreads);
 
 #ifdef __cplusplus
 }

@@ -57,6 +57,8 @@ def load():
                                         P(C.c_int64), C.c_int]),
         "orc_solve_fluid_groups": (C.c_int, [vp, C.c_int, P(C.c_int32), C.c_int, P(C.c_int32), C.c_int,
                                              P(C.c_double), C.c_int]),
+        "orc_ck_tables": (C.c_int, [C.c_int, C.c_int, P(C.c_int64)] + [P(C.c_double)] * 5 + [C.c_int, C.c_double, C.c_int]
+                          + [P(C.c_double)] * 7 + [C.c_int]),
         "orc_ckdist": (C.c_int, [C.c_int, P(C.c_int64), P(C.c_double), P(C.c_double), P(C.c_double),
                                  P(C.c_double), C.c_int, C.c_double, C.c_int, P(C.c_double), P(C.c_double),
                                  P(C.c_double), P(C.c_double), P(C.c_double), C.c_int]),
@@ -270,6 +272,25 @@ def ckdist(off, wvn, kvn, wvl, wvr, gq, n_pwrk: int = 20000, pwr: float = 0.1, n
                            n_pwrk, pwr, nq, dp(gq), dp(kq), dp(kavg), dp(kmin), dp(kmax), nthreads)
     assert rc == 0
     return kq, kavg, kmin, kmax
+
+
+def ck_tables(spec: dict, gq, wq, n_pwrk: int = 20000, pwr: float = 0.1, nthreads: int = 0):
+    """calcProbT after correlatedKT on spectra in synthetic_lbl's layout:
+    returns dict kq [nT][nB][nq], kavg [nT][nB], kP [nT], cuprob [nT][nB],
+    cuprob_g [nT][nB][nq] (FP64)."""
+    d = lambda a: np.ascontiguousarray(a, np.float64)
+    nB, nT, nq = spec["nB"], len(spec["temps"]), len(gq)
+    off = np.ascontiguousarray(spec["off"], np.int64)
+    wvn, kvn, wvl, wvr, T, gq, wq = (d(spec["wvn"]), d(spec["kvn"]), d(spec["wvl"]), d(spec["wvr"]),
+                                     d(spec["temps"]), d(gq), d(wq))
+    out = dict(kq=np.zeros((nT, nB, nq)), kavg=np.zeros((nT, nB)), kP=np.zeros(nT), cuprob=np.zeros((nT, nB)),
+               cuprob_g=np.zeros((nT, nB, nq)))
+    dp = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    rc = load().orc_ck_tables(nB, nT, off.ctypes.data_as(C.POINTER(C.c_int64)), dp(wvn), dp(kvn), dp(wvl), dp(wvr),
+                              dp(T), n_pwrk, pwr, nq, dp(gq), dp(wq), dp(out["kq"]), dp(out["kavg"]), dp(out["kP"]),
+                              dp(out["cuprob"]), dp(out["cuprob_g"]), nthreads)
+    assert rc == 0
+    return out
 
 
 def all_cells(p, zstart: int = 1) -> np.ndarray:

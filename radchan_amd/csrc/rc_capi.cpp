@@ -361,6 +361,28 @@ int rc_write_tables(const char *dir, const rc_tables *t, double chanHeight) {
     return write_tables(dir, tt, chanHeight);
 }
 
+int rc_ck_tables(int nB, int nT, int nQ, const double *T, const double *wvl, const double *wvr, const double *kq,
+                 const double *kavg, const double *wq, double chanHeight, rc_tables *t) {
+    if (!t || !T || !wvl || !wvr || !kq || !kavg || !wq || nB < 1 || nT < 1 || nQ < 1 || nQ > 1024 ||
+        t->nB != nB || t->nT != nT || t->nQ != nQ) {
+        set_error("rc_ck_tables: null array, sizes out of range or tables of other sizes");
+        return -1;
+    }
+    Tables tt;
+    if (int e = ck_tables(nB, nT, nQ, T, wvl, wvr, kq, kavg, wq, chanHeight, tt)) return e;
+    std::memcpy(t->Tnb, tt.Tnb.data(), tt.Tnb.size() * 4);
+    std::memcpy(t->kP, tt.kP.data(), tt.kP.size() * 4);
+    std::memcpy(t->wvl, tt.wvl.data(), tt.wvl.size() * 4);
+    std::memcpy(t->wvr, tt.wvr.data(), tt.wvr.size() * 4);
+    std::memcpy(t->wvc, tt.wvc.data(), tt.wvc.size() * 4);
+    std::memcpy(t->kavg, tt.kavg.data(), tt.kavg.size() * 4);
+    std::memcpy(t->kq, tt.kq.data(), tt.kq.size() * 4);
+    std::memcpy(t->prob, tt.prob.data(), tt.prob.size() * 4);
+    std::memcpy(t->probg, tt.probg.data(), tt.probg.size() * 4);
+    if (t->wq) std::memcpy(t->wq, tt.wq.data(), tt.wq.size() * 4);
+    return 0;
+}
+
 int rc_grey_tables(const rc_params *p, float Tmax, rc_tables *t) {
     Tables tt;
     tt.resize(t->nB, t->nT, t->nQ);

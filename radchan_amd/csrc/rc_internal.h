@@ -43,6 +43,8 @@ float field_tmax(const rc_params &p, const float *T);
 int read_tables(const std::string &dir, double chanHeight, Tables &t);
 int grey_tables(const rc_params &p, float Tmax, Tables &t);
 int write_tables(const std::string &dir, const Tables &t, double chanHeight);
+int ck_tables(int nB, int nT, int nQ, const double *T, const double *wvl, const double *wvr, const double *kq,
+              const double *kavg, const double *wq, double chanHeight, Tables &t);
 void sort_idx(const float *kavg, int nB, int32_t *idx);
 int check_params(const rc_params &p);
 void default_params(rc_params &p);

@@ -316,6 +316,68 @@ int write_tables(const std::string &dir, const Tables &t, double chanHeight) {
     return 0;
 }
 
+/* calcProbT (nbcorrk/calc_prob.cpp:17-106) from the distributions of every
+ * (temperature row i, band j): kq[i][j][nQ] and kavg[i][j] in FP64.  Per
+ * row, the correlated-k Planck mean kP and the band CDF come from the band
+ * means k_avg (the reference sums kk ff over the bins of each band inside
+ * the same products; the sum over bins is k_avg), the g CDF of each band
+ * from kq Ib dnu 100 wq cumulated and divided by its last value.  The
+ * tables are filled as the reader leaves them: FP32, k values times
+ * chanHeight, kavg from the last row.  kP also stands where the solver
+ * reads the line-by-line Planck mean (planck-mean.txt column 2), so the
+ * bound kappa_max and the band CDF share one normalisation. */
+int ck_tables(int nB, int nT, int nQ, const double *T, const double *wvl, const double *wvr, const double *kq,
+              const double *kavg, const double *wq, double chanHeight, Tables &t) {
+    const double pi = kPi, stefan = kStefan;
+    const double C1 = kC1, C2 = kC2;
+    auto iblack = [&](double Tr, double nu) {     /* nbcorrk/ckdist_hitran_part.cpp:229-242 */
+        return 1.0 / pi * C1 * std::pow((nu * 100), 3) / (std::exp((float)(C2 * nu * 100 / Tr)) - 1);
+    };
+    t.resize(nB, nT, nQ);
+    for (int j = 0; j < nB; ++j) {
+        if (!(wvr[j] > wvl[j])) { set_error("band " + std::to_string(j) + " has wv_right <= wv_left"); return -2; }
+        t.wvl[j] = (float)wvl[j];
+        t.wvr[j] = (float)wvr[j];
+        t.wvc[j] = (float)((wvl[j] + wvr[j]) / 2);
+        t.kavg[j] = (float)kavg[(size_t)(nT - 1) * nB + j];
+    }
+    const float sc = (float)chanHeight;
+    std::vector<double> ib(nB);
+    for (int i = 0; i < nT; ++i) {
+        if (!(T[i] > 0.0) || (i > 0 && !(T[i] > T[i - 1]))) {
+            set_error("temperatures must be positive and increasing");
+            return -3;
+        }
+        const double T4 = stefan * std::pow(T[i], 4);
+        double kP = 0.0;
+        for (int j = 0; j < nB; ++j) {
+            ib[j] = iblack(T[i], (wvl[j] + wvr[j]) / 2);
+            kP += kavg[(size_t)i * nB + j] * ib[j] * (wvr[j] - wvl[j]) * 100 * pi / T4;
+        }
+        if (!(kP > 0.0)) { set_error("row " + std::to_string(i) + " has no absorption (kP = 0)"); return -4; }
+        t.Tnb[i] = (float)T[i];
+        t.kP[i] = (float)kP * sc;
+        double cu = 0.0;
+        for (int j = 0; j < nB; ++j) {
+            cu += kavg[(size_t)i * nB + j] * ib[j] * (wvr[j] - wvl[j]) * 100 * pi / (kP * stefan * std::pow(T[i], 4));
+            t.prob[(size_t)i * nB + j] = (float)cu;
+            const double *k = kq + ((size_t)i * nB + j) * nQ;
+            double cg[1024];
+            for (int g = 0; g < nQ; ++g) {
+                const double p2 = k[g] * ib[j] * (wvr[j] - wvl[j]) * 100 * wq[g];
+                cg[g] = g == 0 ? p2 : cg[g - 1] + p2;
+            }
+            for (int g = 0; g < nQ; ++g) {
+                t.probg[((size_t)i * nB + j) * nQ + g] = (float)(cg[g] / cg[nQ - 1]);
+                t.kq[((size_t)j * nT + i) * nQ + g] = (float)k[g] * sc;
+            }
+        }
+    }
+    t.wq.resize(nQ);
+    for (int g = 0; g < nQ; ++g) t.wq[g] = (float)wq[g];
+    return 0;
+}
+
 /* readT, grey tables (read_tables.cpp:98-175) */
 int grey_tables(const rc_params &p, float Tmax, Tables &t) {
     if (t.nQ != 1) { set_error("grey tables have nQ=1"); return -1; }

@@ -68,3 +68,34 @@ def ckdist(off, wvn, kvn, wvl, wvr, gq, n_pwrk: int = 20000, pwr: float = 0.1):
                                L.dptr(wvl), L.dptr(wvr), n_pwrk, pwr, nq, L.dptr(gq), L.dptr(kq), L.dptr(kavg),
                                L.dptr(kmin), L.dptr(kmax), C.byref(ms)), "rc_ckdist")
     return dict(kq=kq, kavg=kavg, kmin=kmin, kmax=kmax, kernel_ms=ms.value)
+
+
+def tables_from_distributions(T, wvl, wvr, kq, kavg, wq, chanHeight: float = 1.0):
+    """The table step (rc_ck_tables, calcProbT): kq [nT][nB][nQ] and kavg
+    [nT][nB] in FP64 with the band edges wvl/wvr [nB] -> Tables as the
+    reader leaves them (FP32, k values times chanHeight)."""
+    import ctypes as C
+
+    from . import lib as L
+    from .tables import Tables
+
+    kq = np.ascontiguousarray(kq, np.float64)
+    nT, nB, nQ = kq.shape
+    d = lambda a: np.ascontiguousarray(a, np.float64)
+    T, wvl, wvr, kavg, wq = d(T), d(wvl), d(wvr), d(kavg), d(wq)
+    assert T.shape == (nT,) and wvl.shape == (nB,) and wvr.shape == (nB,) and kavg.shape == (nT, nB)
+    t = Tables(nB, nT, nQ)
+    L.check(L.load().rc_ck_tables(nB, nT, nQ, L.dptr(T), L.dptr(wvl), L.dptr(wvr), L.dptr(kq), L.dptr(kavg),
+                                  L.dptr(wq), chanHeight, C.byref(t.cview())), "rc_ck_tables")
+    return t
+
+
+def generate_tables(spec: dict, gq, wq, chanHeight: float = 1.0, n_pwrk: int = 20000, pwr: float = 0.1):
+    """The generator on the GPU: line-by-line spectra (synthetic_lbl's
+    layout) -> correlated-k distributions (rc_ckdist) -> tables
+    (rc_ck_tables).  Returns (Tables, the rc_ckdist outputs)."""
+    g = ckdist(spec["off"], spec["wvn"], spec["kvn"], spec["wvl"], spec["wvr"], gq, n_pwrk=n_pwrk, pwr=pwr)
+    nB, nT, nQ = spec["nB"], len(spec["temps"]), len(gq)
+    t = tables_from_distributions(spec["temps"], spec["wvl"][:nB], spec["wvr"][:nB], g["kq"].reshape(nT, nB, nQ),
+                                  g["kavg"].reshape(nT, nB), wq, chanHeight)
+    return t, g

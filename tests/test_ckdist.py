@@ -183,3 +183,90 @@ def test_gpu_ckdist_rejects_bad_input():
         ckdist([0, 8193], nu, np.ones(8193), [nu[0]], [nu[-1]], GQ16_F64)
     with pytest.raises(RuntimeError, match="nondecreasing"):
         ckdist([0, 10], nu[:10], np.ones(10), [nu[0]], [nu[9]], GQ16_F64[::-1].copy())
+
+
+# ---- the table step (calcProbT) ---------------------------------------------
+
+def _planck_exact(T, nu):
+    """Blackbody intensity per cm^-1 (times 100: per m^-1), exact exponential."""
+    C1, C2 = 3.741771790075259e-16, 0.014387741858429
+    w = 100.0 * nu
+    return C1 / np.pi * w ** 3 / np.expm1(C2 * w / T)
+
+
+def test_ck_tables_grey_closed_form():
+    """A band-independent, almost constant k = kappa (1 + 1e-7 sin): the
+    correlated-k Planck mean is kappa times the fraction of sigma T^4 / pi
+    that the bands' range holds (midpoint rule over 25 cm^-1 bands against
+    the integral; the generator's single-precision exponential), the band
+    CDF is the cumulated blackbody intensity of the bands, and every g CDF
+    is the cumulated quadrature weights."""
+    from scipy.integrate import quad
+    from radchan_amd.tables import WQ16_F64
+    kappa = 3.0
+    sp = synthetic_lbl(wv0=100.0, wv1=12100.0, deltawv=25.0, temps=[500.0, 1000.0, 2000.0])
+    x = np.arange(len(sp["kvn"]))
+    sp["kvn"] = kappa * (1.0 + 1e-7 * np.sin(0.37 * x))
+    o = O.ck_tables(sp, GQ16_F64, WQ16_F64)
+    nB = sp["nB"]
+    wl, wr = sp["wvl"][:nB], sp["wvr"][:nB]
+    sigma = 5.670373e-08
+    for i, T in enumerate(sp["temps"]):
+        frac = sum(quad(lambda nu: _planck_exact(T, nu) * 100.0, a, b)[0] for a, b in zip(wl, wr)) * np.pi / (sigma * T ** 4)
+        assert abs(o["kP"][i] - kappa * frac) <= 2e-4 * kappa, (T, o["kP"][i], kappa * frac)
+        band = _planck_exact(T, 0.5 * (wl + wr)) * (wr - wl)
+        np.testing.assert_allclose(o["cuprob"][i], np.cumsum(band) / band.sum(), atol=1e-6)
+    cw = np.cumsum(WQ16_F64) / WQ16_F64.sum()
+    np.testing.assert_allclose(o["cuprob_g"], np.broadcast_to(cw, o["cuprob_g"].shape), atol=1e-6)
+    np.testing.assert_allclose(o["kq"], kappa, rtol=1e-6)
+
+
+def test_ck_tables_library_matches_oracle():
+    """rc_ck_tables (host code of the library, no GPU) on the oracle's
+    distributions against the oracle's calcProbT: equal to FP32 rounding (the
+    library sums k_avg per band, the reference kk ff per bin)."""
+    from radchan_amd.spectra import tables_from_distributions
+    from radchan_amd.tables import WQ16_F64
+    sp = synthetic_lbl(wv0=150.0, wv1=2150.0, deltawv=25.0, temps=[400.0, 800.0, 1600.0])
+    o = O.ck_tables(sp, GQ16_F64, WQ16_F64)
+    nB = sp["nB"]
+    ch = 0.5
+    t = tables_from_distributions(sp["temps"], sp["wvl"][:nB], sp["wvr"][:nB], o["kq"], o["kavg"], WQ16_F64, ch)
+    np.testing.assert_allclose(t.kP, o["kP"] * ch, rtol=3e-7)
+    np.testing.assert_allclose(t.prob, o["cuprob"], atol=6e-8)
+    np.testing.assert_allclose(t.probg, o["cuprob_g"], atol=6e-8)
+    np.testing.assert_allclose(t.kq, o["kq"].transpose(1, 0, 2) * ch, rtol=3e-7)
+    np.testing.assert_allclose(t.kavg, o["kavg"][-1], rtol=1e-7)
+    assert np.array_equal(t.Tnb, sp["temps"].astype(np.float32))
+    np.testing.assert_allclose(t.wvc, 0.5 * (sp["wvl"][:nB] + sp["wvr"][:nB]), rtol=1e-7)
+
+
+@pytest.mark.gpu
+def test_gpu_generator_end_to_end():
+    """Spectra -> rc_ckdist -> rc_ck_tables -> a batched solve with those
+    tables: the tables equal the oracle's calcProbT to FP32 rounding, and
+    the solve matches the oracle's on the same tables (FP64 tallies to
+    1e-12 of the FP32 per-ray sums)."""
+    from radchan_amd import VARIANT_BATCHED, Solver
+    from radchan_amd import fields as Fd
+    from radchan_amd import params as Pm
+    from radchan_amd.spectra import generate_tables
+    from radchan_amd.tables import WQ16_F64
+    sp = synthetic_lbl(wv0=150.0, wv1=2150.0, deltawv=25.0, temps=np.linspace(300.0, 2400.0, 8))
+    p = Pm.base_params(12, 10, 8, photmax=48, nVar=2, fact_mesh=1.5, seed=3)
+    p.nB, p.nT, p.nQ = sp["nB"], len(sp["temps"]), 16
+    print(f"generator end to end: {p.nB} bands x {p.nT} T, field {p.imax}x{p.jmax}x{p.kmax}", flush=True)
+    tab, g = generate_tables(sp, GQ16_F64, WQ16_F64, chanHeight=p.chanHeight)
+    o = O.ck_tables(sp, GQ16_F64, WQ16_F64)
+    np.testing.assert_allclose(tab.kP, o["kP"] * p.chanHeight, rtol=3e-7)
+    np.testing.assert_allclose(tab.prob, o["cuprob"], atol=6e-8)
+    np.testing.assert_allclose(tab.probg, o["cuprob_g"], atol=6e-8)
+    np.testing.assert_allclose(tab.kq, o["kq"].transpose(1, 0, 2) * p.chanHeight, rtol=3e-7)
+    T = Fd.channel_field(p, 500.0, 1500.0, noise=0.02, seed=3)
+    s = Solver(p, tab)
+    s.set_fields([T])
+    s.solve(VARIANT_BATCHED)
+    got = s.tallies()
+    s.close()
+    _, de32d, _, _ = O.Oracle(p, tab, T).solve(O.all_cells(p))
+    assert np.abs(got - de32d).max() <= 1e-12 * np.abs(de32d).max()
