@@ -282,7 +282,8 @@ def main() -> int:
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=10)
     ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="nb64", choices=["nb64", "nb128", "big256", "grey32", "mg192", "c4"])
+    ap.add_argument("--workload", default="nb64", choices=["nb64", "nb128", "big256", "grey32", "mg192", "c4",
+                                                               "ckdist"])
     ap.add_argument("--nvar", type=int, default=8)
     ap.add_argument("--tables", default="fp32", choices=["bf16", "fp32"],
                     help="k-table storage of the headline (batched) leg; both are measured")
@@ -306,6 +307,16 @@ def main() -> int:
 
     if args.workload == "c4":
         return bench_c4(args)
+    if args.workload == "ckdist":
+        ck = ckdist_leg(not args.no_cpu, reps=args.steps)
+        print(json.dumps({"metric": "correlated-k distributions per second (rc_ckdist), 1 MI355X",
+                          "value": ck["spectra_per_s"], "unit": "spectra/s", "n_gpus": 1, "steps": args.steps,
+                          "warmup": 1, "ms_per_step": ck["kernel_ms"], "higher_is_better": True,
+                          "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic line spectra",
+                          "config": {"workload": f"{ck['spectra']} spectra x {ck['points'] // ck['spectra']} points, "
+                                                 f"n_pwrk {ck['n_pwrk']}, nq {ck['nq']}"},
+                          "ckdist": ck, "cpu_baseline": ck["cpu_baseline"]}), flush=True)
+        return 0
 
     from radchan_amd import VARIANT_BATCHED, VARIANT_PLAIN, Solver, config
     from radchan_amd import fields as F
