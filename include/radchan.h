@@ -169,6 +169,14 @@ int rc_ckdist(int nspec, const int64_t *off, const double *wvn, const double *kv
 int rc_ck_tables(int nB, int nT, int nQ, const double *T, const double *wvl, const double *wvr,
                  const double *kq, const double *kavg, const double *wq, double chanHeight,
                  rc_tables *t);
+/* interp3D (src/montecarlo/spline.cpp:36-177; not called by the
+ * reference, mc_gpu_noint.cu:694): the field T ((imax+2)(jmax+2)(kmax+2)
+ * host floats) restricted to grid level `level` (1..grid_num-1) by natural
+ * cubic splines along x, y, z; Tc [km][jm][im] the cell temperatures, Tw
+ * (may be NULL) [2][km][jm] the west/east x-face temperatures (the spline
+ * of the face means; interp3D's coarse ghost is 2 Tw - Tc).  Uses the GPU;
+ * synchronous. */
+int rc_spline_restrict(const rc_params *p, int level, const float *T, float *Tc, float *Tw);
 /* Gridn::mk_grid (definitions.h:68-136) of one level */
 int rc_grid(const rc_params *p, int level, float *x, float *xu, float *y,
             float *yu, float *z, float *zu);
@@ -216,6 +224,13 @@ int rc_set_table_precision(rc_ctx *c, int bf16);
  * finalize) once as a hipGraph and replay it on later rc_run calls; the
  * graph is dropped when fields or the schedule change */
 int rc_set_graph(rc_ctx *c, int on);
+/* temperature of the coarse grid levels: 0 = the texture mean of the fine
+ * field at the mapped cell centre, as the reference's kernels read it
+ * (mc_gpu_noint.cu:334-338; default); 1 = interp3D's cubic-spline
+ * restriction (spline.cpp:36-177, see rc_spline_restrict) for the cells and
+ * the adiabatic x walls (bd=4 z walls keep the texture mean).  Applies to
+ * fields loaded by later rc_set_fields calls. */
+int rc_set_restriction(rc_ctx *c, int mode);
 /* time the batched solve of the loaded snapshots over schedules (steps per
  * iteration, then workgroups per CU, then refill threshold, each at the
  * best of the previous) and keep the fastest */

@@ -186,6 +186,7 @@ struct Level {
     int im, jm, km, maxs;
     int rx, ry, rz;                     /* imax/im etc. (int division, :334-336) */
     std::vector<float> x, xu, y, yu, z, zu;
+    std::vector<float> Ts, Tws;         /* cubic-spline restriction (empty: texture mean) */
 };
 
 }  // namespace
@@ -373,7 +374,9 @@ int march_level(const orc_ctx &c, int lev, Ray<G> &r, const Energy &e, float tol
         const G dsy = (fy - r.yp) * r.isy;
         const G dsz = (fz - r.zp) * r.isz;
         /* temperature at the mapped cell centre (:334-338) */
-        const float Tc = tex3_T(c, tmap(r.ic, L.rx), tmap(r.jc, L.ry), tmap(r.kc, L.rz));
+        const float Tc = L.Ts.empty()
+            ? tex3_T(c, tmap(r.ic, L.rx), tmap(r.jc, L.ry), tmap(r.kc, L.rz))
+            : L.Ts[(size_t)(r.ic - 1) + (size_t)L.im * ((size_t)(r.jc - 1) + (size_t)L.jm * (r.kc - 1))];
         /* ds = min, position snap to the crossed face (:346-356) */
         G ds = dsx;
         if (ds > dsy) ds = dsy;
@@ -465,8 +468,16 @@ int march_level(const orc_ctx &c, int lev, Ray<G> &r, const Energy &e, float tol
             } else if (bd == 2) {                             /* black */
                 bool fixed = true;
                 float Tw_ = 0.f;
-                if (face == FW && p.adiabw == 1) { fixed = false; Tw_ = tex3_T(c, 1.0f, tmap(r.jc, L.ry), tmap(r.kc, L.rz)); }
-                if (face == FE && p.adiabe == 1) { fixed = false; Tw_ = tex3_T(c, (float)p.imax + 1.0f, tmap(r.jc, L.ry), tmap(r.kc, L.rz)); }
+                const size_t wq = (size_t)(r.jc - 1) + (size_t)L.jm * (r.kc - 1);
+                if (face == FW && p.adiabw == 1) {
+                    fixed = false;
+                    Tw_ = L.Tws.empty() ? tex3_T(c, 1.0f, tmap(r.jc, L.ry), tmap(r.kc, L.rz)) : L.Tws[wq];
+                }
+                if (face == FE && p.adiabe == 1) {
+                    fixed = false;
+                    Tw_ = L.Tws.empty() ? tex3_T(c, (float)p.imax + 1.0f, tmap(r.jc, L.ry), tmap(r.kc, L.rz))
+                                        : L.Tws[(size_t)L.jm * L.km + wq];
+                }
                 wall_tally<G>(c, e, r, Tw_, t, fixed, fixed ? c.bfix[face][e.rank] : 0.f);
                 r.tra = 0; r.tra64 = 0.0;
                 *status = ORC_WALL;
@@ -1156,6 +1167,124 @@ int orc_ckdist(int nspec, const int64_t *off, const double *wvn, const double *k
         std::vector<double> kk, ff;
         ck_one(off[s], off[s + 1], wvn, kvn, wvl[s], wvr[s], n_pwrk, pwr, nq, gq, kk, ff, kq + (size_t)s * nq,
                kavg + s, kmin + s, kmax + s);
+    }
+    return 0;
+}
+
+/* spline (src/montecarlo/spline.cpp:180-207), natural ends (yp1 = ypn =
+ * 1e30): 1-based x[1..n], y[1..n] -> y2[1..n]; float data, the double
+ * constants of the source kept (those products round to float when
+ * assigned) */
+static void nr_spline(const float *x, const float *y, int n, float *y2) {
+    std::vector<float> u(n + 1);
+    y2[1] = u[1] = 0.0f;
+    for (int i = 2; i <= n - 1; i++) {
+        const float sig = (x[i] - x[i - 1]) / (x[i + 1] - x[i - 1]);
+        const float p = sig * y2[i - 1] + 2.0;
+        y2[i] = (sig - 1.0) / p;
+        u[i] = (y[i + 1] - y[i]) / (x[i + 1] - x[i]) - (y[i] - y[i - 1]) / (x[i] - x[i - 1]);
+        u[i] = (6.0 * u[i] / (x[i + 1] - x[i - 1]) - sig * u[i - 1]) / p;
+    }
+    const float qn = 0.0f, un = 0.0f;
+    y2[n] = (un - qn * u[n - 1]) / (qn * y2[n - 1] + 1.0);
+    for (int k = n - 1; k >= 1; k--) y2[k] = y2[k] * y2[k + 1] + u[k];
+}
+
+/* splint (spline.cpp:209-226) */
+static float nr_splint(const float *xa, const float *ya, const float *y2a, int n, float x) {
+    int klo = 1, khi = n;
+    while (khi - klo > 1) {
+        const int k = (khi + klo) >> 1;
+        if (xa[k] > x) khi = k; else klo = k;
+    }
+    const float h = xa[khi] - xa[klo];
+    const float a = (xa[khi] - x) / h, b = (x - xa[klo]) / h;
+    return a * ya[klo] + b * ya[khi] + ((a * a * a - a) * y2a[klo] + (b * b * b - b) * y2a[khi]) * (h * h) / 6.0;
+}
+
+/* interp3D (spline.cpp:36-177) for one coarse level: splines along x over
+ * every fine (j, k), then y over every (coarse i, fine k), then z over
+ * every coarse (i, j); the x walls: face means (ghost + first)/2 splined
+ * along y then z.  Tc [km][jm][im]; Tw [2][km][jm] (the coarse face mean,
+ * since interp3D sets the ghost to 2 face - interior) or NULL.  The source
+ * also fills coarse column im+1 before overwriting it with the ghost; that
+ * column is not formed here. */
+int orc_interp3d(int imax, int jmax, int kmax, const float *x0, const float *y0, const float *z0, const float *T0,
+                 int im, int jm, int km, const float *xc, const float *yc, const float *zc, float *Tc, float *Tw) {
+    if (imax < 2 || jmax < 2 || kmax < 2 || im < 1 || jm < 1 || km < 1) return -1;
+    auto T0at = [&](int i, int j, int k) {
+        return T0[(size_t)i + (size_t)(imax + 2) * ((size_t)j + (size_t)(jmax + 2) * (size_t)k)];
+    };
+    std::vector<float> tx((size_t)(im + 1) * (jmax + 1) * (kmax + 1)), txy((size_t)(im + 1) * (jm + 1) * (kmax + 1));
+    auto TX = [&](int i, int j, int k) -> float & { return tx[(size_t)i + (size_t)(im + 1) * ((size_t)j + (size_t)(jmax + 1) * k)]; };
+    auto TXY = [&](int i, int j, int k) -> float & { return txy[(size_t)i + (size_t)(im + 1) * ((size_t)j + (size_t)(jm + 1) * k)]; };
+    const int nmax = std::max(imax, std::max(jmax, kmax)) + 2;
+    std::vector<float> tmp(nmax), tmp2(nmax), tmp3(nmax), tmp4(nmax);
+    for (int j = 1; j <= jmax; j++)
+        for (int k = 1; k <= kmax; k++) {
+            for (int i = 1; i <= imax; i++) tmp[i] = T0at(i, j, k);
+            nr_spline(x0, tmp.data(), imax, tmp2.data());
+            for (int i = 1; i <= im; i++) TX(i, j, k) = nr_splint(x0, tmp.data(), tmp2.data(), imax, xc[i]);
+        }
+    for (int i = 1; i <= im; i++)
+        for (int k = 1; k <= kmax; k++) {
+            for (int j = 1; j <= jmax; j++) tmp[j] = TX(i, j, k);
+            nr_spline(y0, tmp.data(), jmax, tmp2.data());
+            for (int j = 1; j <= jm; j++) TXY(i, j, k) = nr_splint(y0, tmp.data(), tmp2.data(), jmax, yc[j]);
+        }
+    for (int i = 1; i <= im; i++)
+        for (int j = 1; j <= jm; j++) {
+            for (int k = 1; k <= kmax; k++) tmp[k] = TXY(i, j, k);
+            nr_spline(z0, tmp.data(), kmax, tmp2.data());
+            for (int k = 1; k <= km; k++)
+                Tc[(size_t)(i - 1) + (size_t)im * ((size_t)(j - 1) + (size_t)jm * (k - 1))] =
+                    nr_splint(z0, tmp.data(), tmp2.data(), kmax, zc[k]);
+        }
+    if (!Tw) return 0;
+    std::vector<float> by((size_t)2 * (jm + 1) * (kmax + 1));
+    auto BY = [&](int side, int j, int k) -> float & { return by[(size_t)side + 2 * ((size_t)j + (size_t)(jm + 1) * k)]; };
+    for (int k = 1; k <= kmax; k++) {
+        for (int j = 1; j <= jmax; j++) {
+            tmp[j] = 0.5 * (T0at(0, j, k) + T0at(1, j, k));
+            tmp3[j] = 0.5 * (T0at(imax + 1, j, k) + T0at(imax, j, k));
+        }
+        nr_spline(y0, tmp.data(), jmax, tmp2.data());
+        nr_spline(y0, tmp3.data(), jmax, tmp4.data());
+        for (int j = 1; j <= jm; j++) {
+            BY(0, j, k) = nr_splint(y0, tmp.data(), tmp2.data(), jmax, yc[j]);
+            BY(1, j, k) = nr_splint(y0, tmp3.data(), tmp4.data(), jmax, yc[j]);
+        }
+    }
+    for (int j = 1; j <= jm; j++) {
+        for (int k = 1; k <= kmax; k++) {
+            tmp[k] = BY(0, j, k);
+            tmp3[k] = BY(1, j, k);
+        }
+        nr_spline(z0, tmp.data(), kmax, tmp2.data());
+        nr_spline(z0, tmp3.data(), kmax, tmp4.data());
+        for (int k = 1; k <= km; k++) {
+            Tw[(size_t)(j - 1) + (size_t)jm * (k - 1)] = nr_splint(z0, tmp.data(), tmp2.data(), kmax, zc[k]);
+            Tw[(size_t)jm * km + (size_t)(j - 1) + (size_t)jm * (k - 1)] =
+                nr_splint(z0, tmp3.data(), tmp4.data(), kmax, zc[k]);
+        }
+    }
+    return 0;
+}
+
+/* coarse levels of this field by interp3D (mode 1) or by the texture mean
+ * (mode 0, the default) */
+int orc_set_restriction(orc_ctx *c, int mode) {
+    if (!c || (mode != 0 && mode != 1)) return -1;
+    const orc_params &p = c->p;
+    for (size_t g = 1; g < c->lev.size(); ++g) {
+        Level &L = c->lev[g];
+        if (mode == 0) { L.Ts.clear(); L.Tws.clear(); continue; }
+        const Level &F = c->lev[0];
+        L.Ts.assign((size_t)L.im * L.jm * L.km, 0.f);
+        L.Tws.assign((size_t)2 * L.jm * L.km, 0.f);
+        if (int e = orc_interp3d(p.imax, p.jmax, p.kmax, F.x.data(), F.y.data(), F.z.data(), c->T.data(), L.im, L.jm,
+                                 L.km, L.x.data(), L.y.data(), L.z.data(), L.Ts.data(), L.Tws.data()))
+            return e;
     }
     return 0;
 }

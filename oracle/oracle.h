@@ -161,6 +161,18 @@ int      orc_ckdist(int nspec, const int64_t *off, const double *wvn, const doub
                     const double *wvl, const double *wvr, int n_pwrk, double pwr, int nq,
                     const double *gq, double *kq, double *kavg, double *kmin, double *kmax,
                     int nthreads);
+/* interp3D (src/montecarlo/spline.cpp:36-177) of one coarse level: the
+ * field T0 ((imax+2)(jmax+2)(kmax+2), idx_T layout) at the coarse cell
+ * centres by natural cubic splines along x, then y, then z; the x walls
+ * from the splines of the face means.  x0/y0/z0 and xc/yc/zc are the fine
+ * and coarse cell centres (1-based, Gridn::mk_grid's x, y, z).  Outputs:
+ * Tc [km][jm][im], Tw [2][km][jm] (may be NULL). */
+int      orc_interp3d(int imax, int jmax, int kmax, const float *x0, const float *y0,
+                      const float *z0, const float *T0, int im, int jm, int km,
+                      const float *xc, const float *yc, const float *zc, float *Tc, float *Tw);
+/* coarse-level temperatures of a context: 0 = texture mean (default),
+ * 1 = interp3D for cells and adiabatic x walls */
+int      orc_set_restriction(orc_ctx *c, int mode);
 /* The generator's table step, calcProbT (nbcorrk/calc_prob.cpp:17-106),
  * after orc_ckdist's distributions: spectra row-major (s = i*nB + j, band j
  * at temperature T[i]).  Outputs: kq[nT][nB][nq], kavg[nT][nB], kP[nT] (the

@@ -57,6 +57,8 @@ def load():
                                         P(C.c_int64), C.c_int]),
         "orc_solve_fluid_groups": (C.c_int, [vp, C.c_int, P(C.c_int32), C.c_int, P(C.c_int32), C.c_int,
                                              P(C.c_double), C.c_int]),
+        "orc_interp3d": (C.c_int, [C.c_int] * 3 + [P(f)] * 4 + [C.c_int] * 3 + [P(f)] * 5),
+        "orc_set_restriction": (C.c_int, [vp, C.c_int]),
         "orc_ck_tables": (C.c_int, [C.c_int, C.c_int, P(C.c_int64)] + [P(C.c_double)] * 5 + [C.c_int, C.c_double, C.c_int]
                           + [P(C.c_double)] * 7 + [C.c_int]),
         "orc_ckdist": (C.c_int, [C.c_int, P(C.c_int64), P(C.c_double), P(C.c_double), P(C.c_double),
@@ -144,6 +146,20 @@ def grid(p, level: int):
     return dict(zip(("x", "xu", "y", "yu", "z", "zu"), a))
 
 
+def interp3d(p, level: int, T: np.ndarray, walls: bool = True):
+    """interp3D of the field T to grid level `level`: (Tc [km][jm][im],
+    Tw [2][km][jm] or None)."""
+    g0, gc = grid(p, 0), grid(p, level)
+    im, jm, km = p.maxi[level], p.maxj[level], p.maxk[level]
+    T = np.ascontiguousarray(T, np.float32).reshape(-1)
+    tc = np.zeros(im * jm * km, np.float32)
+    tw = np.zeros(2 * jm * km, np.float32) if walls else None
+    rc = load().orc_interp3d(p.imax, p.jmax, p.kmax, _f(g0["x"]), _f(g0["y"]), _f(g0["z"]), _f(T), im, jm, km,
+                             _f(gc["x"]), _f(gc["y"]), _f(gc["z"]), _f(tc), _f(tw) if walls else None)
+    assert rc == 0
+    return tc.reshape(km, jm, im), (tw.reshape(2, km, jm) if walls else None)
+
+
 def tmax(p, T: np.ndarray) -> float:
     T = np.ascontiguousarray(T, np.float32)
     return load().orc_tmax(C.byref(p), _f(T))
@@ -173,7 +189,7 @@ def grey_tables(p, Tmax: float):
 class Oracle:
     """CPU restatement of the solver for one temperature field."""
 
-    def __init__(self, p, tables, T: np.ndarray, zstart: int = 1, snapshot: int = 0):
+    def __init__(self, p, tables, T: np.ndarray, zstart: int = 1, snapshot: int = 0, restriction: int = 0):
         from radchan_amd import lib as L
         self.L = L
         self.lib = load()
@@ -190,6 +206,8 @@ class Oracle:
             raise RuntimeError("orc_create: " + err.value.decode())
         if snapshot:
             self.lib.orc_set_snapshot(self.ctx, snapshot)
+        if restriction:
+            assert self.lib.orc_set_restriction(self.ctx, restriction) == 0
 
     def close(self):
         if self.ctx:

@@ -22,7 +22,8 @@ CSRC = os.path.join(HERE, "csrc")
 LIB = os.path.join(HERE, "libradchan.so")
 ORACLE_DIR = os.path.join(ROOT, "oracle")
 
-SOURCES = ["rc_tables.cpp", "rc_capi.cpp", "rc_solver.hip", "rc_batched.hip", "rc_ckdist.hip"]
+SOURCES = ["rc_tables.cpp", "rc_capi.cpp", "rc_solver.hip", "rc_batched.hip", "rc_ckdist.hip",
+           "rc_spline.hip"]
 HIP_FLAGS = [
     "--offload-arch=gfx950",
     "-O3",

@@ -113,6 +113,9 @@ struct rc_ctx {
     TableSet nbset;
     std::vector<Grid> grids;
     DevBuf xu[RC_MAXLEV], x0, y0, z0;
+    DevBuf cx[RC_MAXLEV], cy[RC_MAXLEV], cz[RC_MAXLEV];   /* cell centres per level */
+    DevBuf spl;                             /* spline restriction scratch */
+    int restriction = 0;                    /* coarse-level T: 0 texture mean, 1 cubic spline */
     std::vector<std::unique_ptr<SnapState>> snaps;
     std::vector<DevSnap> hsnap;             /* host copy of the descriptors */
     DevBuf dsnap;
@@ -447,7 +450,9 @@ rc_ctx *rc_create(const rc_params *pp) {
     c->grids.resize(p.grid_num);
     for (int g = 0; g < p.grid_num; ++g) {
         build_grid(p, g, c->grids[g]);
-        if (upload(c->xu[g], c->grids[g].xu, c->stream)) return nullptr;
+        if (upload(c->xu[g], c->grids[g].xu, c->stream) || upload(c->cx[g], c->grids[g].x, c->stream) ||
+            upload(c->cy[g], c->grids[g].y, c->stream) || upload(c->cz[g], c->grids[g].z, c->stream))
+            return nullptr;
     }
     if (upload(c->x0, c->grids[0].x, c->stream) || upload(c->y0, c->grids[0].y, c->stream) ||
         upload(c->z0, c->grids[0].z, c->stream))
@@ -531,6 +536,15 @@ int rc_set_fields(rc_ctx *c, int nsnap, const float *const *T, int zstart) {
             if (wz) { if (int e = S.lvTz[g].ensure((size_t)2 * p.maxi[g] * p.maxj[g] * 4)) return e; }
             launch_level_T(c->stream, S.T0.as<float>(), p, g, S.lvT[g].as<float>(),
                            wx ? S.lvTw[g].as<float>() : nullptr, wz ? S.lvTz[g].as<float>() : nullptr);
+            if (c->restriction == 1 && g > 0) {
+                /* cells and x-wall faces by cubic splines; bd=4 z walls keep the texture mean */
+                const size_t ns = spline_scratch_floats(p.imax, p.jmax, p.kmax, p.maxi[g], p.maxj[g], p.maxk[g]);
+                if (int e = c->spl.ensure(ns * 4)) return e;
+                launch_spline_level(c->stream, S.T0.as<float>(), c->cx[0].as<float>(), c->cy[0].as<float>(),
+                                    c->cz[0].as<float>(), c->cx[g].as<float>(), c->cy[g].as<float>(),
+                                    c->cz[g].as<float>(), p.imax, p.jmax, p.kmax, p.maxi[g], p.maxj[g], p.maxk[g],
+                                    S.lvT[g].as<float>(), wx ? S.lvTw[g].as<float>() : nullptr, c->spl.as<float>());
+            }
         }
         RC_HIP(hipGetLastError());
     }
@@ -661,6 +675,60 @@ int rc_set_table_precision(rc_ctx *c, int bf16) {
         }
     if (!c->snaps.empty()) return push_snaps(c);
     return 0;
+}
+
+int rc_set_restriction(rc_ctx *c, int mode) {
+    if (!c || (mode != 0 && mode != 1)) { set_error("restriction: 0 (texture mean) or 1 (cubic spline)"); return -1; }
+    if (mode == 1 && (c->p.imax < 2 || c->p.jmax < 2 || c->p.kmax < 2)) {
+        set_error("cubic-spline restriction needs imax, jmax, kmax >= 2");
+        return -2;
+    }
+    c->restriction = mode;
+    return 0;
+}
+
+int rc_spline_restrict(const rc_params *p, int level, const float *T, float *Tc, float *Tw) {
+    if (!p || !T || !Tc) { set_error("rc_spline_restrict: null argument"); return -1; }
+    if (int e = check_params(*p)) return e;
+    if (level < 1 || level >= p->grid_num) { set_error("rc_spline_restrict: level in 1..grid_num-1"); return -2; }
+    if (p->imax < 2 || p->jmax < 2 || p->kmax < 2) { set_error("cubic-spline restriction needs imax, jmax, kmax >= 2"); return -2; }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) { set_error("no HIP device available"); return -3; }
+    Grid g0, gc;
+    build_grid(*p, 0, g0);
+    build_grid(*p, level, gc);
+    const int im = p->maxi[level], jm = p->maxj[level], km = p->maxk[level];
+    const size_t nT0 = (size_t)(p->imax + 2) * (p->jmax + 2) * (p->kmax + 2);
+    const size_t nc = (size_t)im * jm * km, nw = (size_t)2 * jm * km;
+    hipStream_t st = nullptr;
+    if (hipStreamCreateWithFlags(&st, hipStreamNonBlocking) != hipSuccess) { set_error("cannot create HIP stream"); return -3; }
+    int rc = 0;
+    {
+        DevBuf dT, dc, dw, x0, y0, z0, xc, yc, zc, scr;
+        const size_t ns = spline_scratch_floats(p->imax, p->jmax, p->kmax, im, jm, km);
+        if ((rc = dT.ensure(nT0 * 4)) || (rc = dc.ensure(nc * 4)) || (rc = dw.ensure(nw * 4)) ||
+            (rc = scr.ensure(ns * 4)) || (rc = upload(x0, g0.x, st)) || (rc = upload(y0, g0.y, st)) ||
+            (rc = upload(z0, g0.z, st)) || (rc = upload(xc, gc.x, st)) || (rc = upload(yc, gc.y, st)) ||
+            (rc = upload(zc, gc.z, st))) {
+            (void)hipStreamSynchronize(st);
+            (void)hipStreamDestroy(st);
+            return rc;
+        }
+        hipError_t e = hipMemcpyAsync(dT.p, T, nT0 * 4, hipMemcpyHostToDevice, st);
+        if (e == hipSuccess) {
+            launch_spline_level(st, dT.as<float>(), x0.as<float>(), y0.as<float>(), z0.as<float>(), xc.as<float>(),
+                                yc.as<float>(), zc.as<float>(), p->imax, p->jmax, p->kmax, im, jm, km, dc.as<float>(),
+                                Tw ? dw.as<float>() : nullptr, scr.as<float>());
+            e = hipGetLastError();
+        }
+        if (e == hipSuccess) e = hipMemcpyAsync(Tc, dc.p, nc * 4, hipMemcpyDeviceToHost, st);
+        if (e == hipSuccess && Tw) e = hipMemcpyAsync(Tw, dw.p, nw * 4, hipMemcpyDeviceToHost, st);
+        const hipError_t es = hipStreamSynchronize(st);
+        if (e == hipSuccess) e = es;
+        if (e != hipSuccess) { set_error(std::string("rc_spline_restrict: ") + hipGetErrorString(e)); rc = -4; }
+    }
+    (void)hipStreamDestroy(st);
+    return rc;
 }
 
 int rc_set_graph(rc_ctx *c, int on) {

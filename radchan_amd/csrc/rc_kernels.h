@@ -69,6 +69,12 @@ struct DevSnap {
 };
 
 void launch_level_T(hipStream_t st, const float *T0, const rc_params &p, int g, float *Tl, float *Tw, float *Tz);
+/* cubic-spline restriction of level g (interp3D): cells Tl [km][jm][im] and,
+ * if Tw, x-wall faces [2][km][jm]; scratch of spline_scratch_floats floats */
+size_t spline_scratch_floats(int imax, int jmax, int kmax, int im, int jm, int km);
+void launch_spline_level(hipStream_t st, const float *T0, const float *x0, const float *y0, const float *z0,
+                         const float *xc, const float *yc, const float *zc, int imax, int jmax, int kmax, int im,
+                         int jm, int km, float *Tl, float *Tw, float *scratch);
 void launch_sort(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v, int surface, int nunit,
                  const int32_t *ulist, uint16_t *keys);
 void launch_fluid_plain(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v, const uint16_t *keys,

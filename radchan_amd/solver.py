@@ -105,6 +105,11 @@ class Solver:
     def set_table_precision(self, bf16: bool) -> None:
         L.check(self.lib.rc_set_table_precision(self.ctx, int(bf16)), "rc_set_table_precision")
 
+    def set_restriction(self, mode: int) -> None:
+        """Coarse-level temperatures: 0 texture mean (default), 1 cubic
+        splines (interp3D); applies to fields set afterwards."""
+        L.check(self.lib.rc_set_restriction(self.ctx, mode), "rc_set_restriction")
+
     def set_graph(self, on: bool = True) -> None:
         L.check(self.lib.rc_set_graph(self.ctx, int(on)), "rc_set_graph")
 
@@ -119,6 +124,18 @@ class Solver:
         L.check(self.lib.rc_info(self.ctx, s, L.fptr(f), L.iptr(i)), "rc_info")
         return dict(Tmax=float(f[0]), kappamax=float(f[1]), Ibmax=float(f[2]), Ibmax_w=float(f[3]),
                     Ibmax_e=float(f[4]), tm=int(i[0]), tstar=int(i[1]))
+
+
+def spline_restrict(p: L.Params, level: int, T: np.ndarray, walls: bool = True):
+    """interp3D on the GPU (rc_spline_restrict): (Tc [km][jm][im], Tw
+    [2][km][jm] or None) of grid level `level` from the field T."""
+    T = np.ascontiguousarray(T, dtype=np.float32).reshape(-1)
+    im, jm, km = p.maxi[level], p.maxj[level], p.maxk[level]
+    tc = np.zeros(im * jm * km, np.float32)
+    tw = np.zeros(2 * jm * km, np.float32) if walls else None
+    L.check(L.load().rc_spline_restrict(C.byref(p), level, L.fptr(T), L.fptr(tc),
+                                        L.fptr(tw) if walls else None), "rc_spline_restrict")
+    return tc.reshape(km, jm, im), (tw.reshape(2, km, jm) if walls else None)
 
 
 # ---- the reference's Fortran-facing entry points ---------------------------

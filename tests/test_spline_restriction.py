@@ -1,0 +1,111 @@
+"""Cubic-spline restriction of the temperature to coarse grid levels
+(interp3D, src/montecarlo/spline.cpp:36-177; an option beside the texture
+mean the reference's kernels use).
+
+CPU: the oracle's restatement against scipy's natural cubic spline
+(an independent implementation) applied along x, y, then z, on a stretched
+grid with coarsening in every direction, and the x-wall face values
+against the same splines of the face means; a linear field comes back
+linear.  GPU: rc_spline_restrict equal to the oracle bit for bit; a
+three-level solve with the spline restriction equal to the oracle's with
+the same restriction."""
+import numpy as np
+import pytest
+from scipy.interpolate import CubicSpline
+
+from oracle import oracle as O
+from radchan_amd import fields as F
+from radchan_amd import params as P
+from tests import cases
+
+
+def _case(seed=4):
+    p = P.base_params(24, 16, 20, photmax=32, nVar=2, fact_mesh=1.5, seed=seed,
+                      levels=[(24, 16, 20), (12, 8, 10), (6, 4, 5)], maxs=[4, 8, 64])
+    T = F.channel_field(p, 300.0, 1500.0, noise=0.05, seed=seed)
+    # wall planes of their own (the face means are then not constant)
+    A = T.reshape(p.kmax + 2, p.jmax + 2, p.imax + 2)
+    rng = np.random.default_rng(seed)
+    A[:, :, 0] = 400.0 + 50.0 * rng.random((p.kmax + 2, p.jmax + 2))
+    A[:, :, -1] = 600.0 + 80.0 * rng.random((p.kmax + 2, p.jmax + 2))
+    return p, T
+
+
+def _scipy_restrict(p, level, T):
+    g0, gc = O.grid(p, 0), O.grid(p, level)
+    im, jm, km = p.maxi[level], p.maxj[level], p.maxk[level]
+    A = np.asarray(T, np.float64).reshape(p.kmax + 2, p.jmax + 2, p.imax + 2)
+    x0, y0, z0 = (g0[c][1:n + 1].astype(np.float64) for c, n in (("x", p.imax), ("y", p.jmax), ("z", p.kmax)))
+    xc, yc, zc = (gc[c][1:n + 1].astype(np.float64) for c, n in (("x", im), ("y", jm), ("z", km)))
+    a = A[1:-1, 1:-1, 1:-1]
+    a = CubicSpline(x0, a, axis=2, bc_type="natural")(xc)
+    a = CubicSpline(y0, a, axis=1, bc_type="natural")(yc)
+    tc = CubicSpline(z0, a, axis=0, bc_type="natural")(zc)
+    tw = []
+    for ig, ii in ((0, 1), (p.imax + 1, p.imax)):
+        f = 0.5 * (A[1:-1, 1:-1, ig] + A[1:-1, 1:-1, ii])          # [k][j]
+        f = CubicSpline(y0, f, axis=1, bc_type="natural")(yc)
+        tw.append(CubicSpline(z0, f, axis=0, bc_type="natural")(zc))
+    return tc, np.stack(tw)
+
+
+@pytest.mark.parametrize("level", [1, 2])
+def test_interp3d_is_the_natural_cubic_spline(level):
+    p, T = _case()
+    tc, tw = O.interp3d(p, level, T)
+    sc, sw = _scipy_restrict(p, level, T)
+    # FP32 against scipy's FP64: a few float roundings
+    assert np.abs(tc - sc).max() <= 2e-6 * np.abs(sc).max()
+    assert np.abs(tw - sw).max() <= 2e-6 * np.abs(sw).max()
+
+
+def test_interp3d_keeps_a_linear_field():
+    p, _ = _case()
+    g = O.grid(p, 0)
+    X, Y, Z = np.meshgrid(g["x"], g["y"], g["z"], indexing="ij")
+    lin = (800.0 + 150.0 * X + 20.0 * Y - 7.0 * Z).transpose(2, 1, 0).astype(np.float32)
+    gc = O.grid(p, 1)
+    Xc, Yc, Zc = np.meshgrid(gc["x"][1:-1], gc["y"][1:-1], gc["z"][1:-1], indexing="ij")
+    want = (800.0 + 150.0 * Xc + 20.0 * Yc - 7.0 * Zc).transpose(2, 1, 0)
+    tc, tw = O.interp3d(p, 1, lin)
+    assert np.abs(tc - want).max() <= 2e-3
+    # the face mean of a linear field is its value on the face (x = 0, x = Lx)
+    face = np.stack([800.0 + 20.0 * Yc[0] - 7.0 * Zc[0], 800.0 + 150.0 * p.Lx + 20.0 * Yc[0] - 7.0 * Zc[0]])
+    assert np.abs(tw - face.transpose(0, 2, 1)).max() <= 2e-3
+
+
+@pytest.mark.gpu
+@pytest.mark.parametrize("level", [1, 2])
+def test_gpu_spline_restrict_matches_oracle(level):
+    from radchan_amd.solver import spline_restrict
+    p, T = _case()
+    print(f"spline restriction: level {level}", flush=True)
+    tc, tw = spline_restrict(p, level, T)
+    oc, ow = O.interp3d(p, level, T)
+    assert np.array_equal(tc, oc) and np.array_equal(tw, ow)
+
+
+@pytest.mark.gpu
+def test_gpu_solve_with_spline_restriction(nb_tables):
+    """The three-level case with adiabatic x walls and wall fluxes, coarse
+    temperatures by splines, both variants against the oracle."""
+    from radchan_amd import VARIANT_BATCHED, VARIANT_PLAIN, Solver
+    p, tab, T = cases.mg_case(nb_tables)
+    o = O.Oracle(p, tab, T, restriction=1)
+    de32, de32d, _, steps = o.solve(O.all_cells(p))
+    for variant in (VARIANT_PLAIN, VARIANT_BATCHED):
+        print(f"spline restriction solve: variant {variant}", flush=True)
+        s = Solver(p, tab)
+        s.set_restriction(1)
+        s.set_fields([T])
+        st = s.solve(variant)
+        got = s.tallies()
+        s.close()
+        if variant == VARIANT_PLAIN:
+            assert np.array_equal(got, de32.astype(np.float64))
+        else:
+            assert np.abs(got - de32d).max() <= 1e-12 * np.abs(de32d).max()
+        assert st["rays"] >= p.imax * p.jmax * p.kmax * p.photmax * p.nVar   # fluid and wall units
+    # the restriction changes the coarse temperatures, so the tallies differ
+    _, ref32d, _, _ = O.Oracle(p, tab, T).solve(O.all_cells(p))
+    assert not np.array_equal(ref32d, de32d)
