@@ -43,6 +43,20 @@ B_RAY_BF16 = 28
 B_CELL = 20                     # T in, FP64 tally pair out
 
 
+RESTRICTION = 0                 # coarse-level T: 0 texture mean, 1 cubic splines
+
+
+def new_session(p, tab, Ts):
+    """A solver session with the fields Ts loaded (coarse levels by the
+    chosen restriction)."""
+    from radchan_amd import Solver
+    s = Solver(p, tab)
+    if RESTRICTION:
+        s.set_restriction(RESTRICTION)
+    s.set_fields(Ts)
+    return s
+
+
 def log(*a):
     print(*a, file=sys.stderr, flush=True)
 
@@ -113,8 +127,7 @@ def eps_scaling(p, tab, T, sched, eps, bf16: bool):
     import copy
     q = copy.deepcopy(p)
     q.photmax = 4 * p.photmax
-    s = Solver(q, tab)
-    s.set_fields([T])
+    s = new_session(q, tab, [T])
     s.set_table_precision(bf16)
     if sched:
         s.set_schedule(sched["wg_per_cu"], sched["threshold"], sched["steps_per_iter"])
@@ -136,8 +149,7 @@ def target_leg(lib, tab):
     p = config("big256")
     p.nVar = 8
     T = F.channel_field(p, 300.0, 1500.0, noise=0.02, seed=0)
-    s = Solver(p, tab)
-    s.set_fields([T])
+    s = new_session(p, tab, [T])
     sched = s.autotune()
     s.set_graph(True)
     wall, st, _ = timed_leg(s, lib, VARIANT_BATCHED, 2, 1)
@@ -219,8 +231,7 @@ def bench_c4(args) -> int:
         p = config(name)
         p.nVar = args.nvar
         Ts = [F.channel_field(p, 300.0, 1800.0, noise=0.05, seed=seed0 + i) for i in range(n)]
-        s = Solver(p, tab)
-        s.set_fields(Ts)
+        s = new_session(p, tab, Ts)
         shapes.append((name, p, n, s))
         log(f"c4: {n} x {p.imax}x{p.jmax}x{p.kmax}")
     from radchan_amd import Group
@@ -294,12 +305,17 @@ def main() -> int:
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-ckdist", action="store_true", help="skip the correlated-k generator leg")
+    ap.add_argument("--restriction", default="texture", choices=["texture", "spline"],
+                    help="coarse-level temperatures of multigrid workloads: texture mean (as the reference) or "
+                         "cubic splines (interp3D)")
     ap.add_argument("--no-target", action="store_true",
                     help="skip the 256x128x256 batched-vs-plain leg")
     ap.add_argument("--probe", action="store_true",
                     help="no timing: one elementwise copy of 512 MiB (counter calibration) and one batched "
                          "solve, for hardware-counter runs")
     args = ap.parse_args()
+    global RESTRICTION
+    RESTRICTION = 1 if args.restriction == "spline" else 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
         print("bench.py: this build runs on one GPU; the multi-GPU path (cell slabs per GPU) is not "
               "part of it, so --gpus must be 1", file=sys.stderr)
@@ -332,8 +348,7 @@ def main() -> int:
         tab = synthetic_nb()
         T = F.channel_field(p, 300.0, 1500.0, noise=0.02, seed=0)
     log(f"bench: workload {args.workload} {p.imax}x{p.jmax}x{p.kmax} photmax {p.photmax} nVar {p.nVar}")
-    s = Solver(p, tab)
-    s.set_fields([T])
+    s = new_session(p, tab, [T])
     lib = L.load()
     bf16 = args.tables == "bf16" and not p.grey
 
@@ -446,7 +461,7 @@ def main() -> int:
                                f"grid_num {p.grid_num}" if not p.grey else
                                f"grey OERM {p.imax}x{p.jmax}x{p.kmax}, photmax {p.photmax}, nVar {p.nVar}",
                    "cells": ncell, "rays_per_step": int(rays), "steps_per_ray": steps / rays,
-                   "schedule": sched, "graph": not args.no_graph},
+                   "schedule": sched, "graph": not args.no_graph, "restriction": args.restriction},
         "eps": head["eps"],
         "eps_p95_cell": head["eps_p95_cell"],
         "time_to_1pct_s": head["time_to_1pct_s"],
