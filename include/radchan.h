@@ -231,6 +231,16 @@ int rc_set_graph(rc_ctx *c, int on);
  * the adiabatic x walls (bd=4 z walls keep the texture mean).  Applies to
  * fields loaded by later rc_set_fields calls. */
 int rc_set_restriction(rc_ctx *c, int mode);
+/* band and g pdf of the estimator: 0 = the reference's (CDFs of the table
+ * row nearest Tmax, kappa_max = the tabulated Planck mean at Tmax,
+ * mc_gpu_noint.cu:1110-1131, read_tables.cpp:35-38; default); 1 = exact:
+ * pdf and normalisation from the tables interpolated at Tmax itself (p(b,g)
+ * proportional to K_bg(Tmax) B_b(Tmax) w_g dnu_b, kappa_max = pi Z /
+ * (sigma Tmax^4); walls: p(b) from B_b(Tmax) dnu_b and the banded emission
+ * eps pi sum B dnu), unbiased at any Tmax and for any table normalisation.
+ * Narrow-band runs only (grey runs are unaffected); applies to fields loaded
+ * by later rc_set_fields calls. */
+int rc_set_estimator(rc_ctx *c, int mode);
 /* time the batched solve of the loaded snapshots over schedules (steps per
  * iteration, then workgroups per CU, then refill threshold, each at the
  * best of the previous) and keep the fastest */

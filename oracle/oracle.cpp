@@ -212,6 +212,7 @@ struct orc_ctx {
     std::vector<float> cdf_bs;          /* surface band CDF row t*               */
     std::vector<float> cdf_gs;          /* [nQ] cumulative wq                    */
     std::vector<float> bfix[6];         /* fixed-T wall Ib per rank              */
+    std::vector<float> wq, wvl, wvr;    /* quadrature weights, band edges        */
     bool grey_true;
 };
 
@@ -957,6 +958,9 @@ orc_ctx *orc_create(const orc_params *pp, const orc_tables *t, const float *T, i
         }
         for (int b = 0; b < nB; ++b) c->cdf_bs[b] = c->cdf_bs[b] / acc;
     }
+    c->wq.assign(t->wq, t->wq + nQ);
+    c->wvl.assign(t->wvl, t->wvl + nB);
+    c->wvr.assign(t->wvr, t->wvr + nB);
     c->cdf_gs.assign(nQ, 0.f);
     {
         float acc = 0.f;
@@ -1268,6 +1272,65 @@ int orc_interp3d(int imax, int jmax, int kmax, const float *x0, const float *y0,
                 nr_splint(z0, tmp3.data(), tmp4.data(), kmax, zc[k]);
         }
     }
+    return 0;
+}
+
+/* the estimator's exact mode: p(b, g) = q_bg / Z with q_bg = K_bg(Tmax)
+ * B_b(Tmax) w_g dnu_b (K, B interpolated linearly between the rows around
+ * Tmax), kappa_max = pi Z / (sigma Tmax^4); walls: p(b) = B_b dnu_b / Zs,
+ * Ibmax_s = eps pi Zs / photmax.  With these, the expected tally of a cell
+ * is its emission 4 pi sum K B w dnu (fluid) or eps pi sum B dnu (walls)
+ * for any Tmax; the reference's nearest-row pdf (mc_gpu_noint.cu:1110-1131)
+ * and tabulated kP (read_tables.cpp:35-38) are exact only at tabulated
+ * temperatures of self-consistent tables.  FP64 sums in band order. */
+int orc_set_estimator(orc_ctx *c, int mode) {
+    if (!c || (mode != 0 && mode != 1)) return -1;
+    const orc_params &p = c->p;
+    if (mode == 0 || p.grey) return 0;
+    const int nB = c->nB, nQ = c->nQ, nT = c->nT, tm = c->tm;
+    const double f = ((double)c->Tmax - (double)c->Tnb[tm]) / ((double)c->Tnb[tm + 1] - (double)c->Tnb[tm]);
+    std::vector<double> qsum(nB), q((size_t)nB * nQ), qs(nB);
+    double Z = 0.0, Zs = 0.0;
+    for (int b = 0; b < nB; ++b) {
+        const int r = c->rank_of[b];
+        const float *bt = &c->btab[(size_t)r * (nT + 1)];
+        const double Bm = (double)bt[tm] + f * ((double)bt[tm + 1] - (double)bt[tm]);
+        const double dnu = ((double)c->wvr[b] - (double)c->wvl[b]) * 100.0;
+        double acc = 0.0;
+        for (int g = 0; g < nQ; ++g) {
+            const float *kt = &c->ktab[((size_t)r * nQ + g) * (nT + 1)];
+            const double v = ((double)kt[tm] + f * ((double)kt[tm + 1] - (double)kt[tm])) * Bm * (double)c->wq[g] * dnu;
+            q[(size_t)b * nQ + g] = v;
+            acc += v;
+        }
+        qsum[b] = acc;
+        Z += acc;
+        qs[b] = Bm * dnu;
+        Zs += qs[b];
+    }
+    double cum = 0.0, cums = 0.0;
+    for (int b = 0; b < nB; ++b) {
+        cum += qsum[b];
+        cums += qs[b];
+        c->cdf_b[b] = (float)(cum / Z);
+        c->cdf_bs[b] = (float)(cums / Zs);
+        double cg = 0.0;
+        for (int g = 0; g < nQ; ++g) {
+            cg += q[(size_t)b * nQ + g];
+            c->cdf_g[(size_t)b * nQ + g] = qsum[b] > 0.0 ? (float)(cg / qsum[b]) : (float)(g + 1) / (float)nQ;
+        }
+    }
+    c->cdf_b[nB - 1] = 1.0f;
+    c->cdf_bs[nB - 1] = 1.0f;
+    const double T4 = (double)c->Tmax * c->Tmax * c->Tmax * c->Tmax;
+    c->kappamax64 = kPi * Z / (kStefan * T4);
+    c->kappamax = (float)c->kappamax64;
+    c->Ibmax64 = 4.0 * kPi * Z / p.photmax;
+    c->Ibmax = (float)c->Ibmax64;
+    c->Ibmax_s64[0] = (double)p.eps[FW] * kPi * Zs / p.photmax;
+    c->Ibmax_s64[1] = (double)p.eps[FE] * kPi * Zs / p.photmax;
+    c->Ibmax_s[0] = (float)c->Ibmax_s64[0];
+    c->Ibmax_s[1] = (float)c->Ibmax_s64[1];
     return 0;
 }
 

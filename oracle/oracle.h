@@ -173,6 +173,10 @@ int      orc_interp3d(int imax, int jmax, int kmax, const float *x0, const float
 /* coarse-level temperatures of a context: 0 = texture mean (default),
  * 1 = interp3D for cells and adiabatic x walls */
 int      orc_set_restriction(orc_ctx *c, int mode);
+/* the estimator's band/g pdf: 0 = the reference's nearest-row CDFs and
+ * tabulated kP (default), 1 = exact: pdf and normalisation from the tables
+ * interpolated at Tmax (narrow band only) */
+int      orc_set_estimator(orc_ctx *c, int mode);
 /* The generator's table step, calcProbT (nbcorrk/calc_prob.cpp:17-106),
  * after orc_ckdist's distributions: spectra row-major (s = i*nB + j, band j
  * at temperature T[i]).  Outputs: kq[nT][nB][nq], kavg[nT][nB], kP[nT] (the

@@ -59,6 +59,7 @@ def load():
                                              P(C.c_double), C.c_int]),
         "orc_interp3d": (C.c_int, [C.c_int] * 3 + [P(f)] * 4 + [C.c_int] * 3 + [P(f)] * 5),
         "orc_set_restriction": (C.c_int, [vp, C.c_int]),
+        "orc_set_estimator": (C.c_int, [vp, C.c_int]),
         "orc_ck_tables": (C.c_int, [C.c_int, C.c_int, P(C.c_int64)] + [P(C.c_double)] * 5 + [C.c_int, C.c_double, C.c_int]
                           + [P(C.c_double)] * 7 + [C.c_int]),
         "orc_ckdist": (C.c_int, [C.c_int, P(C.c_int64), P(C.c_double), P(C.c_double), P(C.c_double),
@@ -189,7 +190,8 @@ def grey_tables(p, Tmax: float):
 class Oracle:
     """CPU restatement of the solver for one temperature field."""
 
-    def __init__(self, p, tables, T: np.ndarray, zstart: int = 1, snapshot: int = 0, restriction: int = 0):
+    def __init__(self, p, tables, T: np.ndarray, zstart: int = 1, snapshot: int = 0, restriction: int = 0,
+                 estimator: int = 0):
         from radchan_amd import lib as L
         self.L = L
         self.lib = load()
@@ -208,6 +210,8 @@ class Oracle:
             self.lib.orc_set_snapshot(self.ctx, snapshot)
         if restriction:
             assert self.lib.orc_set_restriction(self.ctx, restriction) == 0
+        if estimator:
+            assert self.lib.orc_set_estimator(self.ctx, estimator) == 0
 
     def close(self):
         if self.ctx:

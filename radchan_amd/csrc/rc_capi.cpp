@@ -116,6 +116,7 @@ struct rc_ctx {
     DevBuf cx[RC_MAXLEV], cy[RC_MAXLEV], cz[RC_MAXLEV];   /* cell centres per level */
     DevBuf spl;                             /* spline restriction scratch */
     int restriction = 0;                    /* coarse-level T: 0 texture mean, 1 cubic spline */
+    int estimator = 0;                      /* band pdf: 0 nearest row (reference), 1 exact at Tmax */
     std::vector<std::unique_ptr<SnapState>> snaps;
     std::vector<DevSnap> hsnap;             /* host copy of the descriptors */
     DevBuf dsnap;
@@ -232,6 +233,60 @@ int snapshot_scalars(const rc_params &p, const Tables &t, SnapState &s) {
     return 0;
 }
 
+/* the estimator's exact mode: the band/g pdf and its normalisation taken
+ * from the tables interpolated at Tmax itself, instead of the nearest row's
+ * CDFs and the tabulated Planck mean (the reference's choice, whose pdf and
+ * weights disagree away from tabulated temperatures, and whose kappa_max
+ * need not match the CDF normalisation).  With the emission of (band b,
+ * g) at Tmax q_bg = K_bg(Tmax) B_b(Tmax) w_g dnu_b (linear interpolation in
+ * T, as the weights use) and Z = sum q_bg: p(b, g) = q_bg / Z and
+ * kappa_max = pi Z / (sigma Tmax^4), so that photmax rays carry the cell's
+ * emission 4 pi sum K B w dnu exactly.  Walls: p(b) from B_b(Tmax) dnu_b and
+ * Ibmax_s = eps pi Zs / photmax, the banded emission rather than eps sigma
+ * T^4.  Sums in FP64 in band order, rounded once. */
+void snapshot_exact(const rc_params &p, const Tables &t, SnapState &s, std::vector<float> &cb, std::vector<float> &cg,
+                    std::vector<float> &cs) {
+    const int nB = t.nB, nQ = t.nQ, nT = t.nT, tm = s.tm;
+    const double f = ((double)s.Tmax - (double)t.Tnb[tm]) / ((double)t.Tnb[tm + 1] - (double)t.Tnb[tm]);
+    std::vector<double> qb(nB), qbg((size_t)nB * nQ), bs(nB);
+    double Z = 0.0, Zs = 0.0;
+    for (int b = 0; b < nB; ++b) {
+        const double b0 = planck(t.Tnb[tm], t.wvc[b]), b1 = planck(t.Tnb[tm + 1], t.wvc[b]);
+        const double B = b0 + f * (b1 - b0);
+        const double dnu = ((double)t.wvr[b] - (double)t.wvl[b]) * 100.0;
+        double sb = 0.0;
+        for (int g = 0; g < nQ; ++g) {
+            const double k0 = t.kq[((size_t)b * nT + tm) * nQ + g], k1 = t.kq[((size_t)b * nT + tm + 1) * nQ + g];
+            const double q = (k0 + f * (k1 - k0)) * B * (double)t.wq[g] * dnu;
+            qbg[(size_t)b * nQ + g] = q;
+            sb += q;
+        }
+        qb[b] = sb;
+        Z += sb;
+        bs[b] = B * dnu;
+        Zs += bs[b];
+    }
+    double run = 0.0, runs = 0.0;
+    for (int b = 0; b < nB; ++b) {
+        run += qb[b];
+        runs += bs[b];
+        cb[b] = (float)(run / Z);
+        cs[b] = (float)(runs / Zs);
+        double rg = 0.0;
+        for (int g = 0; g < nQ; ++g) {
+            rg += qbg[(size_t)b * nQ + g];
+            cg[(size_t)b * nQ + g] = qb[b] > 0.0 ? (float)(rg / qb[b]) : (float)(g + 1) / (float)nQ;
+        }
+    }
+    cb[nB - 1] = 1.0f;
+    cs[nB - 1] = 1.0f;
+    const double T4 = (double)s.Tmax * s.Tmax * s.Tmax * s.Tmax;
+    s.kappamax = (float)(kPi * Z / (kStefan * T4));
+    s.Ibmax = (float)(4.0 * kPi * Z / p.photmax);
+    s.Ibmax_w = (float)((double)p.eps[FW] * kPi * Zs / p.photmax);
+    s.Ibmax_e = (float)((double)p.eps[FE] * kPi * Zs / p.photmax);
+}
+
 int snapshot_rows(rc_ctx *c, const Tables &t, SnapState &s) {
     const rc_params &p = c->p;
     const int nB = t.nB, nQ = t.nQ, ts = s.tstar;
@@ -248,6 +303,7 @@ int snapshot_rows(rc_ctx *c, const Tables &t, SnapState &s) {
         cs[b] = acc;
     }
     for (int b = 0; b < nB; ++b) cs[b] = cs[b] / acc;
+    if (c->estimator == 1 && !p.grey) snapshot_exact(p, t, s, cb, cg, cs);
     /* guide tables: g[m] = first index with cdf > m/M, m = 0..M */
     auto guide = [&](const std::vector<float> &cdf) {
         const int M = kGuideM;
@@ -684,6 +740,12 @@ int rc_set_restriction(rc_ctx *c, int mode) {
         return -2;
     }
     c->restriction = mode;
+    return 0;
+}
+
+int rc_set_estimator(rc_ctx *c, int mode) {
+    if (!c || (mode != 0 && mode != 1)) { set_error("estimator: 0 (reference) or 1 (exact at Tmax)"); return -1; }
+    c->estimator = mode;
     return 0;
 }
 

@@ -121,6 +121,7 @@ def load() -> C.CDLL:
         "rc_grid": (C.c_int, [P(Params), C.c_int] + [P(C.c_float)] * 6),
         "rc_spline_restrict": (C.c_int, [P(Params), C.c_int, P(C.c_float), P(C.c_float), P(C.c_float)]),
         "rc_set_restriction": (C.c_int, [C.c_void_p, C.c_int]),
+        "rc_set_estimator": (C.c_int, [C.c_void_p, C.c_int]),
         "rc_ck_tables": (C.c_int, [C.c_int, C.c_int, C.c_int] + [P(C.c_double)] * 6 + [C.c_double, P(TablesC)]),
         "rc_ckdist": (C.c_int, [C.c_int, P(C.c_int64), P(C.c_double), P(C.c_double), P(C.c_double),
                                 P(C.c_double), C.c_int, C.c_double, C.c_int, P(C.c_double), P(C.c_double),

@@ -110,6 +110,11 @@ class Solver:
         splines (interp3D); applies to fields set afterwards."""
         L.check(self.lib.rc_set_restriction(self.ctx, mode), "rc_set_restriction")
 
+    def set_estimator(self, mode: int) -> None:
+        """Band/g pdf: 0 the reference's nearest-row CDFs (default), 1 exact
+        at Tmax; applies to fields set afterwards."""
+        L.check(self.lib.rc_set_estimator(self.ctx, mode), "rc_set_estimator")
+
     def set_graph(self, on: bool = True) -> None:
         L.check(self.lib.rc_set_graph(self.ctx, int(on)), "rc_set_graph")
 
