@@ -1,13 +1,19 @@
 """Correlated-k generator (correlatedKT, nbcorrk/correlatedK.cpp:144-225, and
 the k_avg of calcProbT, calc_prob.cpp:52-56).
 
-CPU: the oracle's restatement pinned by what the mathematics fixes -- the
+CPU: the oracle's restatement equal bit for bit to the reference's own
+correlatedKT and calcProbT built from nbcorrk/ (oracle/_ref/ref_ckdist),
+and pinned by what the mathematics fixes -- the
 inverse-CDF property of kq on a linear spectrum (closed form) and on random
 spectra (brute-force CDF of the point pairs), k_avg as the band mean of the
 pair averages, exact scaling by 2^10 (k^0.1 scales by 2 exactly), the flat
 and one-point spectra.  GPU: rc_ckdist against the oracle on the same
 spectra, ragged sizes and the edge cases.  The reference's own tests hold no
 vectors for this path (nbcorrk/tests holds plots only)."""
+import os
+import struct
+import subprocess
+
 import numpy as np
 import pytest
 
@@ -270,3 +276,44 @@ def test_gpu_generator_end_to_end():
     s.close()
     _, de32d, _, _ = O.Oracle(p, tab, T).solve(O.all_cells(p))
     assert np.abs(got - de32d).max() <= 1e-12 * np.abs(de32d).max()
+
+
+REF_CK = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "ref_ckdist")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_CK), reason="reference generator not built (make -C oracle ref)")
+def test_oracle_equals_the_reference_generator(tmp_path):
+    """The reference's own correlatedKT and calcProbT, compiled from
+    nbcorrk/ with its compile-time sizes (2 temperatures, 20000 bins, 16
+    g-points), against the oracle on the same spectra: kq, k_avg, kP, the
+    band CDF and the g CDFs equal bit for bit."""
+    from radchan_amd.tables import WQ16_F64
+    sp = synthetic_lbl(wv0=150.0, wv1=750.0, deltawv=10.0, temps=[700.0, 1900.0], seed=3)
+    nB, nT = sp["nB"], 2
+    with open(tmp_path / "in.bin", "wb") as f:
+        f.write(struct.pack("<i", nB))
+        f.write(np.asarray(sp["temps"], "<f8").tobytes())
+        for s in range(nT * nB):
+            b, e = sp["off"][s], sp["off"][s + 1]
+            f.write(struct.pack("<idd", int(e - b), sp["wvl"][s], sp["wvr"][s]))
+            f.write(np.asarray(sp["wvn"][b:e], "<f8").tobytes())
+            f.write(np.asarray(sp["kvn"][b:e], "<f8").tobytes())
+    (tmp_path / "errors").mkdir()
+    subprocess.run([REF_CK, "in.bin", "out.bin"], cwd=tmp_path, check=True, capture_output=True)
+    raw = np.fromfile(tmp_path / "out.bin", "<f8")
+    per = 16 + 1 + 1 + 16
+    kP = np.zeros(nT)
+    kq, ka, cu, cg = (np.zeros((nT, nB, 16)), np.zeros((nT, nB)), np.zeros((nT, nB)), np.zeros((nT, nB, 16)))
+    pos = 0
+    for i in range(nT):
+        kP[i] = raw[pos]
+        pos += 1
+        rec = raw[pos:pos + nB * per].reshape(nB, per)
+        pos += nB * per
+        kq[i], ka[i], cu[i], cg[i] = rec[:, :16], rec[:, 16], rec[:, 17], rec[:, 18:]
+    o = O.ck_tables(sp, GQ16_F64, WQ16_F64)
+    assert np.array_equal(o["kq"], kq)
+    assert np.array_equal(o["kavg"], ka)
+    assert np.array_equal(o["kP"], kP)
+    assert np.array_equal(o["cuprob"], cu)
+    assert np.array_equal(o["cuprob_g"], cg)
