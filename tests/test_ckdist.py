@@ -317,3 +317,28 @@ def test_oracle_equals_the_reference_generator(tmp_path):
     assert np.array_equal(o["kP"], kP)
     assert np.array_equal(o["cuprob"], cu)
     assert np.array_equal(o["cuprob_g"], cg)
+
+
+def test_library_rejects_bad_generator_input():
+    """Input checks of rc_ckdist and rc_ck_tables, before any device work
+    (so they run without a GPU)."""
+    from radchan_amd.spectra import ckdist, tables_from_distributions
+    from radchan_amd.tables import WQ16_F64
+    nu = 100.0 + 0.01 * np.arange(8193)
+    with pytest.raises(RuntimeError, match="8193 points"):
+        ckdist([0, 8193], nu, np.ones(8193), [nu[0]], [nu[-1]], GQ16_F64)
+    with pytest.raises(RuntimeError, match="nondecreasing"):
+        ckdist([0, 10], nu[:10], np.ones(10), [nu[0]], [nu[9]], GQ16_F64[::-1].copy())
+    with pytest.raises(RuntimeError, match="wv_right <= wv_left"):
+        ckdist([0, 10], nu[:10], np.ones(10), [nu[9]], [nu[0]], GQ16_F64)
+    with pytest.raises(RuntimeError, match="n_pwrk"):
+        ckdist([0, 10], nu[:10], np.ones(10), [nu[0]], [nu[9]], GQ16_F64, n_pwrk=70000)
+    kq = np.ones((2, 3, 16))
+    kavg = np.ones((2, 3))
+    wl, wr = np.array([1.0, 2.0, 3.0]), np.array([2.0, 3.0, 4.0])
+    with pytest.raises(RuntimeError, match="increasing"):
+        tables_from_distributions([900.0, 800.0], wl, wr, kq, kavg, WQ16_F64)
+    with pytest.raises(RuntimeError, match="no absorption"):
+        tables_from_distributions([800.0, 900.0], wl, wr, kq, np.zeros((2, 3)), WQ16_F64)
+    with pytest.raises(RuntimeError, match="wv_right <= wv_left"):
+        tables_from_distributions([800.0, 900.0], wr, wl, kq, kavg, WQ16_F64)
