@@ -222,9 +222,18 @@ def ckdist_leg(cpu: bool, reps: int = 5):
         calls.append((time.perf_counter() - t0) * 1e3)
         ks.append(g["kernel_ms"])
     kms = float(np.mean(ks))
+    # the table step (rc_ck_tables, calcProbT on the distributions; host)
+    from radchan_amd.spectra import tables_from_distributions
+    from radchan_amd.tables import WQ16_F64
+    nB, nTt = d["nB"], len(d["temps"])
+    t0 = time.perf_counter()
+    tables_from_distributions(d["temps"], d["wvl"][:nB], d["wvr"][:nB], g["kq"].reshape(nTt, nB, nq),
+                              g["kavg"].reshape(nTt, nB), WQ16_F64)
+    table_ms = (time.perf_counter() - t0) * 1e3
     alg = npts * 16 + ns * (8 * (nq + 3) + 8 * 3)       # points in; kq, kavg, kmin, kmax out; edges, offsets
     out = {"spectra": ns, "points": npts, "n_pwrk": 20000, "nq": nq, "kernel_ms": round(kms, 4),
-           "call_ms_with_h2d": round(float(np.mean(calls)), 2), "spectra_per_s": ns / (kms * 1e-3),
+           "call_ms_with_h2d": round(float(np.mean(calls)), 2), "table_step_ms": round(table_ms, 2),
+           "spectra_per_s": ns / (kms * 1e-3),
            "points_per_s": npts / (kms * 1e-3), "algorithmic_bytes": int(alg),
            "hbm_frac": alg / (kms * 1e-3) / 1e9 / HBM_PEAK_GBS, "cpu_baseline": None}
     if cpu:
