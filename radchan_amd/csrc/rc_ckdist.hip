@@ -183,7 +183,15 @@ __global__ __launch_bounds__(CK_THREADS) void k_ckdist(CkArgs a) {
     uint16_t *RB = (uint16_t *)(lds + 16 * (size_t)np2); /* [np2] run bins          */
 
     const int64_t base = a.off[s];
-    const int n = (int)(a.off[s + 1] - base);
+    const int64_t n64 = a.off[s + 1] - base;
+    if (n64 < 1 || n64 - 1 > np2) {
+        /* more pairs than this instance's sort slots (or none): the caller
+         * of the device entry point under-stated the longest spectrum */
+        for (int q = tid; q < a.nq; q += CK_THREADS) a.kq[(size_t)s * a.nq + q] = __builtin_nan("");
+        if (tid == 0) { a.kavg[s] = __builtin_nan(""); a.kmin[s] = __builtin_nan(""); a.kmax[s] = __builtin_nan(""); }
+        return;
+    }
+    const int n = (int)n64;
     const double *wvs = a.wvn + base, *kvs = a.kvn + base;
 
     /* (1) kmin, kmax (the reference's initial values 1e10 and -1000) */
@@ -369,6 +377,79 @@ struct DevBuf {
 
 using namespace rc;
 
+namespace rc {
+namespace {
+/* sort slots per spectrum for spectra of up to nmax points (a power of two,
+ * at least one per thread) */
+int ck_slots(int nmax) {
+    int np2 = CK_THREADS;
+    while (np2 < nmax - 1) np2 <<= 1;
+    return np2;
+}
+
+hipError_t ck_launch(const CkArgs &a, int nspec, int np2, hipStream_t st) {
+    const size_t lds = (size_t)np2 * 18;
+    switch (np2 / CK_THREADS) {
+        case 1: hipLaunchKernelGGL(k_ckdist<1>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
+        case 2: hipLaunchKernelGGL(k_ckdist<2>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
+        case 4: hipLaunchKernelGGL(k_ckdist<4>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
+        case 8: hipLaunchKernelGGL(k_ckdist<8>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
+        case 16: hipLaunchKernelGGL(k_ckdist<16>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
+        default: hipLaunchKernelGGL(k_ckdist<32>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
+    }
+    return hipGetLastError();
+}
+
+int ck_lds_check(int np2, int nmax) {
+    int dev = 0, lds_max = 0;
+    if (hipGetDevice(&dev) != hipSuccess ||
+        hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
+        set_error("rc_ckdist: no HIP device");
+        return -6;
+    }
+    const size_t lds = (size_t)np2 * 18;
+    if (lds + 256 > (size_t)lds_max) {
+        set_error("rc_ckdist: spectra of " + std::to_string(nmax) + " points need " + std::to_string(lds) +
+                  " B of LDS; the device allows " + std::to_string(lds_max));
+        return -6;
+    }
+    return 0;
+}
+}  // namespace
+}  // namespace rc
+
+extern "C" int rc_ckdist_device(int nspec, const int64_t *off, const double *wvn, const double *kvn,
+                                const double *wvl, const double *wvr, int n_pwrk, double pwr, int nq,
+                                const double *gq, double *kq, double *kavg, double *kmin, double *kmax,
+                                int max_points, void *stream) {
+    if (nspec < 1 || !off || !wvn || !kvn || !wvl || !wvr || !gq || !kq || !kavg || !kmin || !kmax) {
+        set_error("rc_ckdist_device: null array or no spectra");
+        return -1;
+    }
+    if (n_pwrk < 2 || n_pwrk > 65535 || !(pwr > 0.0) || !std::isfinite(pwr)) {
+        set_error("rc_ckdist_device: n_pwrk in 2..65535 and pwr > 0");
+        return -2;
+    }
+    if (nq < 1 || nq > 4096) { set_error("rc_ckdist_device: nq in 1..4096"); return -3; }
+    if (max_points < 1 || max_points > CK_NMAX) {
+        set_error("rc_ckdist_device: max_points in 1.." + std::to_string(CK_NMAX));
+        return -4;
+    }
+    const int np2 = ck_slots(max_points);
+    if (int e = ck_lds_check(np2, max_points)) return e;
+    CkArgs a;
+    a.off = off; a.wvn = wvn; a.kvn = kvn; a.wvl = wvl; a.wvr = wvr; a.gq = gq;
+    a.kq = kq; a.kavg = kavg; a.kmin = kmin; a.kmax = kmax;
+    a.pwr = pwr;
+    a.ipwr = 1.0 / pwr;
+    a.ie = (a.ipwr == std::rint(a.ipwr) && a.ipwr >= 1.0 && a.ipwr <= 64.0) ? (int)a.ipwr : 0;
+    a.nq = nq;
+    a.n_pwrk = n_pwrk;
+    const hipError_t e = ck_launch(a, nspec, np2, (hipStream_t)stream);
+    if (e != hipSuccess) { set_error(std::string("rc_ckdist_device: ") + hipGetErrorString(e)); return -7; }
+    return 0;
+}
+
 extern "C" int rc_ckdist(int nspec, const int64_t *off, const double *wvn, const double *kvn, const double *wvl,
                          const double *wvr, int n_pwrk, double pwr, int nq, const double *gq, double *kq,
                          double *kavg, double *kmin, double *kmax, double *kernel_ms) {
@@ -402,20 +483,8 @@ extern "C" int rc_ckdist(int nspec, const int64_t *off, const double *wvn, const
         }
         nmax = std::max(nmax, (int)n);
     }
-    int np2 = CK_THREADS;
-    while (np2 < nmax - 1) np2 <<= 1;
-    const size_t lds = (size_t)np2 * 18;
-    int dev = 0, lds_max = 0;
-    if (hipGetDevice(&dev) != hipSuccess ||
-        hipDeviceGetAttribute(&lds_max, hipDeviceAttributeMaxSharedMemoryPerBlock, dev) != hipSuccess) {
-        set_error("rc_ckdist: no HIP device");
-        return -6;
-    }
-    if (lds + 256 > (size_t)lds_max) {
-        set_error("rc_ckdist: spectra of " + std::to_string(nmax) + " points need " + std::to_string(lds) +
-                  " B of LDS; the device allows " + std::to_string(lds_max));
-        return -6;
-    }
+    const int np2 = ck_slots(nmax);
+    if (int e = ck_lds_check(np2, nmax)) return e;
 
     const int64_t p0 = off[0], np = off[nspec] - p0;
     std::vector<int64_t> hoff(off, off + nspec + 1);
@@ -469,15 +538,7 @@ extern "C" int rc_ckdist(int nspec, const int64_t *off, const double *wvn, const
         a.nq = nq;
         a.n_pwrk = n_pwrk;
         CK_TRY(hipEventRecord(e0, st));
-        switch (np2 / CK_THREADS) {
-            case 1: hipLaunchKernelGGL(k_ckdist<1>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
-            case 2: hipLaunchKernelGGL(k_ckdist<2>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
-            case 4: hipLaunchKernelGGL(k_ckdist<4>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
-            case 8: hipLaunchKernelGGL(k_ckdist<8>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
-            case 16: hipLaunchKernelGGL(k_ckdist<16>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
-            default: hipLaunchKernelGGL(k_ckdist<32>, dim3(nspec), dim3(CK_THREADS), lds, st, a); break;
-        }
-        CK_TRY(hipGetLastError());
+        CK_TRY(ck_launch(a, nspec, np2, st));
         CK_TRY(hipEventRecord(e1, st));
         CK_TRY(hipMemcpyAsync(kq, d_kq.p, nsb * nq, hipMemcpyDeviceToHost, st));
         CK_TRY(hipMemcpyAsync(kavg, d_kavg.p, nsb, hipMemcpyDeviceToHost, st));

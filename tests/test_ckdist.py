@@ -342,3 +342,38 @@ def test_library_rejects_bad_generator_input():
         tables_from_distributions([800.0, 900.0], wl, wr, kq, np.zeros((2, 3)), WQ16_F64)
     with pytest.raises(RuntimeError, match="wv_right <= wv_left"):
         tables_from_distributions([800.0, 900.0], wr, wl, kq, kavg, WQ16_F64)
+
+
+@pytest.mark.gpu
+def test_gpu_ckdist_device_entry_point_on_torch_tensors():
+    """rc_ckdist_device on spectra already in HBM (torch tensors, torch's
+    stream): the same kernel, so the same bits as the host entry point; a
+    spectrum longer than the stated max_points gets NaNs, the others are
+    unaffected."""
+    import torch
+    from radchan_amd.spectra import ckdist, ckdist_torch
+    d = synthetic_lbl(temps=[400.0, 1400.0], wv0=150.0, wv1=2150.0)
+    h = ckdist(d["off"], d["wvn"], d["kvn"], d["wvl"], d["wvr"], GQ16_F64)
+    dev = torch.device("cuda:0")
+    t = {k: torch.from_numpy(np.ascontiguousarray(d[k])).to(dev) for k in ("off", "wvn", "kvn", "wvl", "wvr")}
+    gq = torch.from_numpy(GQ16_F64.copy()).to(dev)
+    print(f"rc_ckdist_device: {len(d['off']) - 1} spectra", flush=True)
+    kq, kavg, kmin, kmax = ckdist_torch(t["off"], t["wvn"], t["kvn"], t["wvl"], t["wvr"], gq)
+    torch.cuda.synchronize()
+    assert np.array_equal(kq.cpu().numpy(), h["kq"]) and np.array_equal(kavg.cpu().numpy(), h["kavg"])
+    assert np.array_equal(kmin.cpu().numpy(), h["kmin"]) and np.array_equal(kmax.cpu().numpy(), h["kmax"])
+    # max_points under-stated for the middle spectrum only
+    rng = np.random.default_rng(1)
+    sizes = [100, 600, 50]
+    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+    wv = np.concatenate([3000.0 + 0.01 * np.arange(n) for n in sizes])
+    kv = 10.0 ** rng.uniform(-2.0, 2.0, off[-1])
+    wl = np.array([wv[off[i]] for i in range(3)])
+    wr = np.array([wv[off[i + 1] - 1] for i in range(3)])
+    ref = ckdist(off, wv, kv, wl, wr, GQ16_F64)
+    cu = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
+    kq2, kavg2, _, _ = ckdist_torch(cu(off), cu(wv), cu(kv), cu(wl), cu(wr), gq, max_points=300)
+    torch.cuda.synchronize()
+    kq2, kavg2 = kq2.cpu().numpy(), kavg2.cpu().numpy()
+    assert np.all(np.isnan(kq2[1])) and np.isnan(kavg2[1])
+    assert np.array_equal(kq2[[0, 2]], ref["kq"][[0, 2]]) and np.array_equal(kavg2[[0, 2]], ref["kavg"][[0, 2]])
