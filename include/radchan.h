@@ -177,9 +177,9 @@ int rc_ck_tables(int nB, int nT, int nQ, const double *T, const double *wvl, con
  * of the face means; interp3D's coarse ghost is 2 Tw - Tc).  Uses the GPU;
  * synchronous. */
 int rc_spline_restrict(const rc_params *p, int level, const float *T, float *Tc, float *Tw);
-/* rc_ckdist on device arrays (spectra already in HBM, e.g. torch tensors),
- * asynchronous on `stream` (a hipStream_t; NULL = the null stream), nothing
- * allocated.  max_points bounds the points of every spectrum (1..8192; it
+/* rc_ckdist on device arrays (spectra already in HBM, e.g. written there by
+ * another kernel), asynchronous on `stream` (a hipStream_t; NULL = the null
+ * stream), nothing allocated.  max_points bounds the points of every spectrum (1..8192; it
  * picks the kernel instance); a spectrum with more, or with none, gets NaN
  * outputs.  gq must be nondecreasing in [0, 1] (not checked here). */
 int rc_ckdist_device(int nspec, const int64_t *off, const double *wvn, const double *kvn,

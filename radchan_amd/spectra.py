@@ -100,27 +100,3 @@ def generate_tables(spec: dict, gq, wq, chanHeight: float = 1.0, n_pwrk: int = 2
                                   g["kavg"].reshape(nT, nB), wq, chanHeight)
     return t, g
 
-
-def ckdist_torch(off, wvn, kvn, wvl, wvr, gq, n_pwrk: int = 20000, pwr: float = 0.1, max_points: int = 0):
-    """rc_ckdist_device on torch tensors already on the GPU (int64 off,
-    float64 rest), on torch's current stream; returns (kq, kavg, kmin,
-    kmax) as new device tensors.  max_points defaults to the longest
-    spectrum (read from off on the host)."""
-    import torch
-
-    from . import lib as L
-
-    ns, nq = off.numel() - 1, gq.numel()
-    if max_points <= 0:
-        max_points = int((off[1:] - off[:-1]).max().item())
-    dev = wvn.device
-    kq = torch.empty((ns, nq), dtype=torch.float64, device=dev)
-    kavg, kmin, kmax = (torch.empty(ns, dtype=torch.float64, device=dev) for _ in range(3))
-    for t in (off, wvn, kvn, wvl, wvr, gq):
-        assert t.is_cuda and t.is_contiguous()
-    assert off.dtype == torch.int64 and all(t.dtype == torch.float64 for t in (wvn, kvn, wvl, wvr, gq))
-    st = torch.cuda.current_stream(dev).cuda_stream
-    L.check(L.load().rc_ckdist_device(ns, off.data_ptr(), wvn.data_ptr(), kvn.data_ptr(), wvl.data_ptr(),
-                                      wvr.data_ptr(), n_pwrk, pwr, nq, gq.data_ptr(), kq.data_ptr(), kavg.data_ptr(),
-                                      kmin.data_ptr(), kmax.data_ptr(), max_points, st), "rc_ckdist_device")
-    return kq, kavg, kmin, kmax

@@ -344,36 +344,77 @@ def test_library_rejects_bad_generator_input():
         tables_from_distributions([800.0, 900.0], wr, wl, kq, kavg, WQ16_F64)
 
 
+class _Hip:
+    """Device buffers through the HIP runtime the library itself links
+    (libamdhip64.so.7), so that its pointers are that runtime's."""
+
+    def __init__(self):
+        import ctypes as C
+        self.C = C
+        self.rt = C.CDLL("libamdhip64.so.7")
+        self.bufs = []
+
+    def put(self, a):
+        C = self.C
+        a = np.ascontiguousarray(a)
+        p = C.c_void_p()
+        assert self.rt.hipMalloc(C.byref(p), C.c_size_t(max(a.nbytes, 8))) == 0
+        self.bufs.append(p)
+        assert self.rt.hipMemcpy(p, a.ctypes.data_as(C.c_void_p), C.c_size_t(a.nbytes), 1) == 0   # H2D
+        return p
+
+    def empty(self, n):
+        return self.put(np.zeros(n))
+
+    def get(self, p, n):
+        C = self.C
+        out = np.zeros(n)
+        assert self.rt.hipDeviceSynchronize() == 0
+        assert self.rt.hipMemcpy(out.ctypes.data_as(C.c_void_p), p, C.c_size_t(out.nbytes), 2) == 0   # D2H
+        return out
+
+    def free(self):
+        for p in self.bufs:
+            self.rt.hipFree(p)
+        self.bufs = []
+
+
+def _ckdist_device(h, off, wv, kv, wl, wr, gq, max_points):
+    from radchan_amd import lib as L
+    ns, nq = len(off) - 1, len(gq)
+    ptr = [h.put(np.asarray(off, np.int64))] + [h.put(np.asarray(a, np.float64)) for a in (wv, kv, wl, wr, gq)]
+    outs = [h.empty(ns * nq), h.empty(ns), h.empty(ns), h.empty(ns)]
+    L.check(L.load().rc_ckdist_device(ns, *ptr[:5], 20000, 0.1, nq, ptr[5], *outs, max_points, None),
+            "rc_ckdist_device")
+    return h.get(outs[0], ns * nq).reshape(ns, nq), h.get(outs[1], ns), h.get(outs[2], ns), h.get(outs[3], ns)
+
+
 @pytest.mark.gpu
-def test_gpu_ckdist_device_entry_point_on_torch_tensors():
-    """rc_ckdist_device on spectra already in HBM (torch tensors, torch's
-    stream): the same kernel, so the same bits as the host entry point; a
-    spectrum longer than the stated max_points gets NaNs, the others are
-    unaffected."""
-    import torch
-    from radchan_amd.spectra import ckdist, ckdist_torch
+def test_gpu_ckdist_device_entry_point():
+    """rc_ckdist_device on spectra already in HBM: the same kernel, so the
+    same bits as the host entry point; a spectrum longer than the stated
+    max_points gets NaNs, the others are unaffected."""
+    from radchan_amd.spectra import ckdist
     d = synthetic_lbl(temps=[400.0, 1400.0], wv0=150.0, wv1=2150.0)
-    h = ckdist(d["off"], d["wvn"], d["kvn"], d["wvl"], d["wvr"], GQ16_F64)
-    dev = torch.device("cuda:0")
-    t = {k: torch.from_numpy(np.ascontiguousarray(d[k])).to(dev) for k in ("off", "wvn", "kvn", "wvl", "wvr")}
-    gq = torch.from_numpy(GQ16_F64.copy()).to(dev)
-    print(f"rc_ckdist_device: {len(d['off']) - 1} spectra", flush=True)
-    kq, kavg, kmin, kmax = ckdist_torch(t["off"], t["wvn"], t["kvn"], t["wvl"], t["wvr"], gq)
-    torch.cuda.synchronize()
-    assert np.array_equal(kq.cpu().numpy(), h["kq"]) and np.array_equal(kavg.cpu().numpy(), h["kavg"])
-    assert np.array_equal(kmin.cpu().numpy(), h["kmin"]) and np.array_equal(kmax.cpu().numpy(), h["kmax"])
-    # max_points under-stated for the middle spectrum only
-    rng = np.random.default_rng(1)
-    sizes = [100, 600, 50]
-    off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
-    wv = np.concatenate([3000.0 + 0.01 * np.arange(n) for n in sizes])
-    kv = 10.0 ** rng.uniform(-2.0, 2.0, off[-1])
-    wl = np.array([wv[off[i]] for i in range(3)])
-    wr = np.array([wv[off[i + 1] - 1] for i in range(3)])
-    ref = ckdist(off, wv, kv, wl, wr, GQ16_F64)
-    cu = lambda a: torch.from_numpy(np.ascontiguousarray(a)).to(dev)
-    kq2, kavg2, _, _ = ckdist_torch(cu(off), cu(wv), cu(kv), cu(wl), cu(wr), gq, max_points=300)
-    torch.cuda.synchronize()
-    kq2, kavg2 = kq2.cpu().numpy(), kavg2.cpu().numpy()
-    assert np.all(np.isnan(kq2[1])) and np.isnan(kavg2[1])
-    assert np.array_equal(kq2[[0, 2]], ref["kq"][[0, 2]]) and np.array_equal(kavg2[[0, 2]], ref["kavg"][[0, 2]])
+    ref = ckdist(d["off"], d["wvn"], d["kvn"], d["wvl"], d["wvr"], GQ16_F64)
+    h = _Hip()
+    try:
+        print(f"rc_ckdist_device: {len(d['off']) - 1} spectra", flush=True)
+        kq, kavg, kmin, kmax = _ckdist_device(h, d["off"], d["wvn"], d["kvn"], d["wvl"], d["wvr"], GQ16_F64,
+                                              int(np.diff(d["off"]).max()))
+        assert np.array_equal(kq, ref["kq"]) and np.array_equal(kavg, ref["kavg"])
+        assert np.array_equal(kmin, ref["kmin"]) and np.array_equal(kmax, ref["kmax"])
+        rng = np.random.default_rng(1)
+        sizes = [100, 600, 50]
+        off = np.concatenate([[0], np.cumsum(sizes)]).astype(np.int64)
+        wv = np.concatenate([3000.0 + 0.01 * np.arange(n) for n in sizes])
+        kv = 10.0 ** rng.uniform(-2.0, 2.0, off[-1])
+        wl = np.array([wv[off[i]] for i in range(3)])
+        wr = np.array([wv[off[i + 1] - 1] for i in range(3)])
+        r2 = ckdist(off, wv, kv, wl, wr, GQ16_F64)
+        print("rc_ckdist_device: max_points below the longest spectrum", flush=True)
+        kq2, kavg2, _, _ = _ckdist_device(h, off, wv, kv, wl, wr, GQ16_F64, 300)
+        assert np.all(np.isnan(kq2[1])) and np.isnan(kavg2[1])
+        assert np.array_equal(kq2[[0, 2]], r2["kq"][[0, 2]]) and np.array_equal(kavg2[[0, 2]], r2["kavg"][[0, 2]])
+    finally:
+        h.free()
