@@ -201,6 +201,21 @@ def ref_generator_time(d):
             "sample": f"nbcorrk correlatedKT + calcProbT on the first 2 of the 52 temperatures ({2 * nB} spectra)"}
 
 
+def ckdist_roofline(ck):
+    """Roofline object of the generator kernel: algorithmic bytes per launch
+    over its event-timed duration against 8 TB/s; HBM traffic from the
+    committed FETCH_SIZE/WRITE_SIZE summary of the same workload."""
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", "traffic_ckdist.json")
+    if os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    ach = ck["algorithmic_bytes"] / (ck["kernel_ms"] * 1e-3) / 1e9
+    return {"bound": "hbm", "achieved": round(ach, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(ach / HBM_PEAK_GBS, 4), "traffic": traffic, "kernel": "k_ckdist<4>",
+            "kernel_ms": ck["kernel_ms"], "algorithmic_bytes": ck["algorithmic_bytes"]}
+
+
 def ckdist_leg(cpu: bool, reps: int = 5):
     """The correlated-k generator (rc_ckdist) on line-by-line spectra of the
     reference generator's layout: 10 cm^-1 bands over 150-9300 cm^-1 at
@@ -375,6 +390,7 @@ def main() -> int:
                           "scaling": "weak", "vs_baseline": None, "dtype": "f64", "data": "synthetic line spectra",
                           "config": {"workload": f"{ck['spectra']} spectra x {ck['points'] // ck['spectra']} points, "
                                                  f"n_pwrk {ck['n_pwrk']}, nq {ck['nq']}"},
+                          "roofline": ckdist_roofline(ck),
                           "ckdist": ck, "cpu_baseline": ck["cpu_baseline"]}), flush=True)
         return 0
 
