@@ -418,3 +418,41 @@ def test_gpu_ckdist_device_entry_point():
         assert np.array_equal(kq2[[0, 2]], r2["kq"][[0, 2]]) and np.array_equal(kavg2[[0, 2]], r2["kavg"][[0, 2]])
     finally:
         h.free()
+
+
+GOLDEN = os.path.join(os.path.dirname(os.path.abspath(__file__)), "golden", "ckdist_ref.npz")
+
+
+def _golden():
+    g = np.load(GOLDEN)
+    sp = {k: g[k] for k in ("off", "wvn", "kvn", "wvl", "wvr", "temps")}
+    sp["nB"] = int(g["nB"])
+    return sp, g
+
+
+def test_oracle_equals_the_reference_generator_golden():
+    """The oracle against the committed outputs of the reference's own
+    correlatedKT + calcProbT (tests/golden/make_ckdist_golden.py): bit for
+    bit, without needing the reference tree."""
+    from radchan_amd.tables import WQ16_F64
+    sp, g = _golden()
+    o = O.ck_tables(sp, GQ16_F64, WQ16_F64)
+    for k in ("kq", "kavg", "kP", "cuprob", "cuprob_g"):
+        assert np.array_equal(o[k], g[k]), k
+
+
+@pytest.mark.gpu
+def test_gpu_generator_against_the_reference_golden():
+    """The GPU generator (rc_ckdist, then rc_ck_tables) against the
+    reference generator's committed outputs: kq and k_avg to FP64 rounding,
+    the tables to FP32 rounding."""
+    from radchan_amd.spectra import generate_tables
+    from radchan_amd.tables import WQ16_F64
+    sp, g = _golden()
+    tab, d = generate_tables(sp, GQ16_F64, WQ16_F64, chanHeight=1.0)
+    nT, nB = 2, sp["nB"]
+    np.testing.assert_allclose(d["kq"].reshape(nT, nB, 16), g["kq"], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(d["kavg"].reshape(nT, nB), g["kavg"], rtol=1e-12, atol=0)
+    np.testing.assert_allclose(tab.kP, g["kP"], rtol=3e-7)
+    np.testing.assert_allclose(tab.prob, g["cuprob"], atol=6e-8)
+    np.testing.assert_allclose(tab.probg, g["cuprob_g"], atol=6e-8)
