@@ -244,8 +244,8 @@ int snapshot_scalars(const rc_params &p, const Tables &t, SnapState &s) {
  * emission 4 pi sum K B w dnu exactly.  Walls: p(b) from B_b(Tmax) dnu_b and
  * Ibmax_s = eps pi Zs / photmax, the banded emission rather than eps sigma
  * T^4.  Sums in FP64 in band order, rounded once. */
-void snapshot_exact(const rc_params &p, const Tables &t, SnapState &s, std::vector<float> &cb, std::vector<float> &cg,
-                    std::vector<float> &cs) {
+int snapshot_exact(const rc_params &p, const Tables &t, SnapState &s, std::vector<float> &cb, std::vector<float> &cg,
+                   std::vector<float> &cs) {
     const int nB = t.nB, nQ = t.nQ, nT = t.nT, tm = s.tm;
     const double f = ((double)s.Tmax - (double)t.Tnb[tm]) / ((double)t.Tnb[tm + 1] - (double)t.Tnb[tm]);
     std::vector<double> qb(nB), qbg((size_t)nB * nQ), bs(nB);
@@ -266,6 +266,10 @@ void snapshot_exact(const rc_params &p, const Tables &t, SnapState &s, std::vect
         bs[b] = B * dnu;
         Zs += bs[b];
     }
+    if (!(Z > 0.0) || !(Zs > 0.0) || !std::isfinite(Z) || !std::isfinite(Zs)) {
+        set_error("exact estimator: the tables give no (or no finite) emission at Tmax");
+        return -6;
+    }
     double run = 0.0, runs = 0.0;
     for (int b = 0; b < nB; ++b) {
         run += qb[b];
@@ -285,6 +289,7 @@ void snapshot_exact(const rc_params &p, const Tables &t, SnapState &s, std::vect
     s.Ibmax = (float)(4.0 * kPi * Z / p.photmax);
     s.Ibmax_w = (float)((double)p.eps[FW] * kPi * Zs / p.photmax);
     s.Ibmax_e = (float)((double)p.eps[FE] * kPi * Zs / p.photmax);
+    return 0;
 }
 
 int snapshot_rows(rc_ctx *c, const Tables &t, SnapState &s) {
@@ -303,7 +308,8 @@ int snapshot_rows(rc_ctx *c, const Tables &t, SnapState &s) {
         cs[b] = acc;
     }
     for (int b = 0; b < nB; ++b) cs[b] = cs[b] / acc;
-    if (c->estimator == 1 && !p.grey) snapshot_exact(p, t, s, cb, cg, cs);
+    if (c->estimator == 1 && !p.grey)
+        if (int e = snapshot_exact(p, t, s, cb, cg, cs)) return e;
     /* guide tables: g[m] = first index with cdf > m/M, m = 0..M */
     auto guide = [&](const std::vector<float> &cdf) {
         const int M = kGuideM;
