@@ -258,15 +258,17 @@ int rc_autotune(rc_ctx *c, int32_t *best_wg, int32_t *best_th, int32_t *best_spi
  * several sessions -- grids of different sizes -- solved by one batched
  * launch.  The sessions must agree in every parameter but the grid (nVar,
  * photmax, srt, table sizes, tolerances, boundary codes, domain lengths,
- * seed, table precision); wall fluxes are not solved in a group.  Each
- * snapshot draws the Philox streams it draws in its own session, and the
- * results land in the sessions: after rc_group_sync, rc_results and
- * rc_tallies of each session return them. */
+ * seed, table precision, calcwalls).  With wall fluxes, the walls of every
+ * snapshot are a second batched launch.  Each snapshot draws the Philox
+ * streams it draws in its own session, and the results land in the
+ * sessions: after rc_group_sync, rc_results, rc_tallies and
+ * rc_surface_results of each session return them. */
 typedef struct rc_group rc_group;
 rc_group *rc_group_create(rc_ctx *const *sessions, int n);
 void      rc_group_destroy(rc_group *g);
 int       rc_group_set_schedule(rc_group *g, int wg_per_cu, int threshold, int steps_per_iter);
-/* launch (asynchronous): one batched kernel over every snapshot */
+/* launch (asynchronous): one batched kernel over every snapshot (and one
+ * over their walls) */
 int       rc_group_run(rc_group *g);
 /* wait for the last group run; fills st (may be NULL) */
 int       rc_group_sync(rc_group *g, rc_stats *st);

@@ -975,7 +975,7 @@ struct rc_group {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evm0 = nullptr, evm1 = nullptr;
     std::vector<DevSnap> h;
-    DevBuf dsnap, de, stats, queue;
+    DevBuf dsnap, de, ds, stats, queue;
     int wg_per_cu = 0, threshold = 0, spi = 0;
     rc_stats last{};
     bool launched = false;
@@ -989,7 +989,7 @@ static int group_compatible(const rc_ctx *a, const rc_ctx *b) {
               x.nT == y.nT && x.grid_num == y.grid_num && x.Lx == y.Lx && x.Ly == y.Ly && x.Lz == y.Lz &&
               x.toll == y.toll && x.tollS == y.tollS && x.k0 == y.k0 && x.k1 == y.k1 &&
               x.adiab[0] == y.adiab[0] && x.adiab[1] == y.adiab[1] && a->p.grey == b->p.grey &&
-              a->table_bf16 == b->table_bf16;
+              a->table_bf16 == b->table_bf16 && a->p.calcwalls == b->p.calcwalls;
     for (int f = 0; f < 6; ++f) ok = ok && x.bd[f] == y.bd[f];
     return ok ? 0 : -1;
 }
@@ -1040,33 +1040,36 @@ int rc_group_run(rc_group *g) {
     /* the sessions' own queued work on the buffers the group writes */
     for (rc_ctx *c : g->cs) {
         if (c->snaps.empty()) { set_error("rc_group_run: a session has no fields"); return -2; }
-        if (c->p.calcwalls == 1 && (c->p.adiabw == 1 || c->p.adiabe == 1)) {
-            set_error("rc_group_run: wall fluxes (calcwalls) are solved per session; run such sessions alone");
-            return -3;
-        }
         if (group_compatible(g->cs[0], c)) { set_error("rc_group_run: sessions no longer compatible"); return -4; }
         RC_HIP(hipStreamSynchronize(c->stream));
     }
     rc_ctx *c0 = g->cs[0];
     const int nVar = c0->p.nVar;
+    /* wall fluxes: the adiabatic walls of every snapshot in a second launch */
+    const bool walls = c0->p.calcwalls == 1 && (c0->p.adiabw == 1 || c0->p.adiabe == 1);
     g->h.clear();
-    int64_t units = 0;
-    for (rc_ctx *c : g->cs) units += (int64_t)c->hsnap.size() * c->ncell * nVar;
-    if (units >= ((int64_t)1 << 31)) {
+    int64_t units = 0, wunits = 0;
+    for (rc_ctx *c : g->cs) {
+        units += (int64_t)c->hsnap.size() * c->ncell * nVar;
+        wunits += (int64_t)c->hsnap.size() * 2 * c->nsurf * nVar;
+    }
+    if (units >= ((int64_t)1 << 31) || (walls && wunits >= ((int64_t)1 << 31))) {
         set_error("rc_group_run: too many (snapshot, cell, run) units for one launch (2^31)");
         return -5;
     }
-    int cells = 0;
+    int cells = 0, wcells = 0;
     for (rc_ctx *c : g->cs)
         for (const DevSnap &d0 : c->hsnap) {
             DevSnap d = d0;
             d.ubase = cells;
-            d.sbase = 0;
+            d.sbase = wcells;
             cells += c->ncell;
+            wcells += 2 * c->nsurf;
             g->h.push_back(d);
         }
     if (int e = upload(g->dsnap, g->h, g->stream)) return e;
     if (int e = g->de.ensure((size_t)cells * nVar * 8)) return e;
+    if (walls) { if (int e = g->ds.ensure((size_t)wcells * nVar * 8)) return e; }
     if (int e = g->stats.ensure(8 * sizeof(unsigned long long))) return e;
     if (int e = g->queue.ensure(32 * sizeof(int))) return e;
     const DevCommon cm = make_common(c0);
@@ -1079,11 +1082,17 @@ int rc_group_run(rc_group *g) {
                    g->de.as<double>(), g->stats.as<unsigned long long>(), g->queue.as<int>(), g->wg_per_cu,
                    g->threshold, g->spi);
     n += 3;
+    if (walls) {
+        launch_batched(g->stream, cm, g->dsnap.as<DevSnap>(), (int)g->h.size(), wcells, 1, c0->table_bf16,
+                       g->ds.as<double>(), g->stats.as<unsigned long long>(), g->queue.as<int>() + 16,
+                       g->wg_per_cu, g->threshold, g->spi);
+        n += 3;
+    }
     RC_HIP(hipEventRecord(g->evm1, g->stream));
     /* each session's tallies, means and deviations into its own buffers */
-    size_t off = 0;
+    size_t off = 0, woff = 0;
     for (rc_ctx *c : g->cs) {
-        const size_t nde = (size_t)c->ncell * nVar;
+        const size_t nde = (size_t)c->ncell * nVar, nds = (size_t)2 * c->nsurf * nVar;
         for (size_t s = 0; s < c->snaps.size(); ++s) {
             RC_HIP(hipMemcpyAsync(c->de.as<double>() + s * nde, g->de.as<double>() + off, nde * 8,
                                   hipMemcpyDeviceToDevice, g->stream));
@@ -1091,6 +1100,14 @@ int rc_group_run(rc_group *g) {
                             c->mean.as<float>() + s * c->ncell, c->sdev.as<float>() + s * c->ncell);
             ++n;
             off += nde;
+            if (walls) {
+                RC_HIP(hipMemcpyAsync(c->ds.as<double>() + s * nds, g->ds.as<double>() + woff, nds * 8,
+                                      hipMemcpyDeviceToDevice, g->stream));
+                launch_finalize(g->stream, c->ds.as<double>() + s * nds, 2 * c->nsurf, nVar,
+                                c->smean.as<float>() + s * 2 * c->nsurf, c->ssdev.as<float>() + s * 2 * c->nsurf);
+                ++n;
+                woff += nds;
+            }
         }
     }
     RC_HIP(hipEventRecord(g->ev1, g->stream));

@@ -251,3 +251,37 @@ def test_autotune_keeps_results(nb_tables):
     s.solve(VARIANT_BATCHED)
     assert np.abs(s.tallies() - ref).max() <= 1e-12 * np.abs(ref).max()
     s.close()
+
+
+def test_group_with_wall_fluxes(nb_tables):
+    """A group whose sessions solve wall fluxes (adiabatic black x walls):
+    the walls of all three snapshots go through one second launch, and every
+    snapshot's fluid and wall tallies equal the oracle's."""
+    from radchan_amd import fields as F
+    from radchan_amd import params as P
+    from radchan_amd.solver import Group
+    p1, tab, _ = cases.nb_case(nb_tables, n=12, photmax=48, nVar=2)
+    p1.calcwalls = 1
+    T1 = [F.channel_field(p1, 300.0, 1500.0 - 300.0 * q, noise=0.03, seed=80 + q) for q in range(2)]
+    p2 = P.base_params(16, 8, 20, photmax=48, nVar=2, fact_mesh=1.2, seed=p1.seed, srt=1)
+    p2.calcwalls = 1
+    T2 = [F.channel_field(p2, 300.0, 1400.0, noise=0.03, seed=90)]
+    print("group with walls: 2 x 12^3 + 1 x 16x8x20", flush=True)
+    s1, s2 = Solver(p1, tab), Solver(p2, tab)
+    s1.set_fields(T1)
+    s2.set_fields(T2)
+    g = Group([s1, s2])
+    g.set_schedule(4, 8, 2)
+    g.solve()
+    for s, p, Ts in ((s1, p1, T1), (s2, p2, T2)):
+        for q, T in enumerate(Ts):
+            o = O.Oracle(p, tab, T, snapshot=q)
+            _, de32d, _, _ = o.solve(O.all_cells(p))
+            got = s.tallies(q)
+            assert np.abs(got - de32d).max() <= 1e-12 * np.abs(de32d).max(), (p.imax, q)
+            _, ds32d, _, _ = o.solve(O.all_wall_cells(p), surface=True)
+            wall = s.surface(q)[2].reshape(-1, p.nVar)
+            assert np.abs(wall - ds32d).max() <= 1e-12 * np.abs(ds32d).max(), (p.imax, q, "walls")
+    g.close()
+    s1.close()
+    s2.close()
