@@ -1334,6 +1334,18 @@ int orc_set_estimator(orc_ctx *c, int mode) {
     return 0;
 }
 
+/* the spline routines on one line (0-based arrays in and out): y2[n] of
+ * the natural spline through (x, y), and its values ys[m] at xs */
+int orc_spline(const float *x, const float *y, int n, float *y2, const float *xs, int m, float *ys) {
+    if (n < 2 || m < 0) return -1;
+    std::vector<float> x1(n + 1), y1(n + 1), d(n + 1);
+    for (int i = 0; i < n; ++i) { x1[i + 1] = x[i]; y1[i + 1] = y[i]; }
+    nr_spline(x1.data(), y1.data(), n, d.data());
+    for (int i = 0; i < n; ++i) y2[i] = d[i + 1];
+    for (int q = 0; q < m; ++q) ys[q] = nr_splint(x1.data(), y1.data(), d.data(), n, xs[q]);
+    return 0;
+}
+
 /* coarse levels of this field by interp3D (mode 1) or by the texture mean
  * (mode 0, the default) */
 int orc_set_restriction(orc_ctx *c, int mode) {

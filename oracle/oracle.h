@@ -170,6 +170,10 @@ int      orc_ckdist(int nspec, const int64_t *off, const double *wvn, const doub
 int      orc_interp3d(int imax, int jmax, int kmax, const float *x0, const float *y0,
                       const float *z0, const float *T0, int im, int jm, int km,
                       const float *xc, const float *yc, const float *zc, float *Tc, float *Tw);
+/* spline / splint (spline.cpp:180-226) on one line, 0-based: y2[n] of the
+ * natural spline through (x, y) and its values ys[m] at xs */
+int      orc_spline(const float *x, const float *y, int n, float *y2, const float *xs, int m,
+                    float *ys);
 /* coarse-level temperatures of a context: 0 = texture mean (default),
  * 1 = interp3D for cells and adiabatic x walls */
 int      orc_set_restriction(orc_ctx *c, int mode);

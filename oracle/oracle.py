@@ -60,6 +60,7 @@ def load():
         "orc_interp3d": (C.c_int, [C.c_int] * 3 + [P(f)] * 4 + [C.c_int] * 3 + [P(f)] * 5),
         "orc_set_restriction": (C.c_int, [vp, C.c_int]),
         "orc_set_estimator": (C.c_int, [vp, C.c_int]),
+        "orc_spline": (C.c_int, [P(f), P(f), C.c_int, P(f), P(f), C.c_int, P(f)]),
         "orc_ck_tables": (C.c_int, [C.c_int, C.c_int, P(C.c_int64)] + [P(C.c_double)] * 5 + [C.c_int, C.c_double, C.c_int]
                           + [P(C.c_double)] * 7 + [C.c_int]),
         "orc_ckdist": (C.c_int, [C.c_int, P(C.c_int64), P(C.c_double), P(C.c_double), P(C.c_double),
@@ -159,6 +160,15 @@ def interp3d(p, level: int, T: np.ndarray, walls: bool = True):
                              _f(gc["x"]), _f(gc["y"]), _f(gc["z"]), _f(tc), _f(tw) if walls else None)
     assert rc == 0
     return tc.reshape(km, jm, im), (tw.reshape(2, km, jm) if walls else None)
+
+
+def spline(x, y, xs):
+    """The spline routines on one line: (y2, values at xs), FP32."""
+    x, y, xs = (np.ascontiguousarray(a, np.float32) for a in (x, y, xs))
+    y2 = np.zeros(len(x), np.float32)
+    ys = np.zeros(len(xs), np.float32)
+    assert load().orc_spline(_f(x), _f(y), len(x), _f(y2), _f(xs), len(xs), _f(ys)) == 0
+    return y2, ys
 
 
 def tmax(p, T: np.ndarray) -> float:

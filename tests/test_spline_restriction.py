@@ -2,13 +2,19 @@
 (interp3D, src/montecarlo/spline.cpp:36-177; an option beside the texture
 mean the reference's kernels use).
 
-CPU: the oracle's restatement against scipy's natural cubic spline
+CPU: the oracle's spline and splint equal to the reference's own
+(spline.cpp compiled as it lies, oracle/_ref/ref_spline) bit for bit, and
+the restatement of interp3D against scipy's natural cubic spline
 (an independent implementation) applied along x, y, then z, on a stretched
 grid with coarsening in every direction, and the x-wall face values
 against the same splines of the face means; a linear field comes back
 linear.  GPU: rc_spline_restrict equal to the oracle bit for bit; a
 three-level solve with the spline restriction equal to the oracle's with
 the same restriction."""
+import os
+import struct
+import subprocess
+
 import numpy as np
 import pytest
 from scipy.interpolate import CubicSpline
@@ -109,3 +115,26 @@ def test_gpu_solve_with_spline_restriction(nb_tables):
     # the restriction changes the coarse temperatures, so the tallies differ
     _, ref32d, _, _ = O.Oracle(p, tab, T).solve(O.all_cells(p))
     assert not np.array_equal(ref32d, de32d)
+
+
+REF_SPL = os.path.join(os.path.dirname(os.path.dirname(os.path.abspath(__file__))), "oracle", "_ref", "ref_spline")
+
+
+@pytest.mark.skipif(not os.path.exists(REF_SPL), reason="reference spline routines not built (make -C oracle ref)")
+@pytest.mark.parametrize("n,seed", [(2, 1), (3, 2), (17, 3), (24, 4), (257, 5)])
+def test_oracle_spline_equals_the_reference_routines(tmp_path, n, seed):
+    """spline and splint of src/montecarlo/spline.cpp compiled as they lie
+    (oracle/_ref/ref_spline) against the oracle's restatement on a stretched
+    random line: second derivatives and values, inside the range, at the
+    knots and beyond the ends, bit for bit."""
+    rng = np.random.default_rng(seed)
+    x = np.cumsum(rng.uniform(0.05, 1.0, n)).astype(np.float32)
+    y = (1000.0 + 300.0 * rng.standard_normal(n)).astype(np.float32)
+    xs = np.concatenate([rng.uniform(x[0] - 0.5, x[-1] + 0.5, 64), x]).astype(np.float32)
+    with open(tmp_path / "in.bin", "wb") as f:
+        f.write(struct.pack("<i", n) + x.tobytes() + y.tobytes() + struct.pack("<i", len(xs)) + xs.tobytes())
+    subprocess.run([REF_SPL, "in.bin", "out.bin"], cwd=tmp_path, check=True, capture_output=True)
+    raw = np.fromfile(tmp_path / "out.bin", "<f4")
+    y2, ys = O.spline(x, y, xs)
+    assert np.array_equal(y2.view(np.uint32), raw[:n].view(np.uint32))
+    assert np.array_equal(ys.view(np.uint32), raw[n:].view(np.uint32))
