@@ -138,6 +138,9 @@ int         rc_set_params(const rc_params *p);       /* used by mc_gpu_        *
 int         rc_get_params(rc_params *p);
 int         rc_set_table_dir(const char *dir);       /* default "properties"   */
 int         rc_set_variant(int variant);             /* used by mc_gpu_        */
+/* options of the mc_gpu_ session (no counterpart in the reference):
+ * rc_set_estimator and rc_set_restriction modes, both 0 by default */
+int         rc_set_dropin_modes(int estimator, int restriction);
 const char *rc_last_error(void);
 const char *rc_version(void);
 

@@ -9,10 +9,10 @@ fields.
 from . import lib
 from .lib import VARIANT_BATCHED, VARIANT_PLAIN, Params
 from .params import base_params, config, reference_params
-from .solver import Group, Solver, get_results, mc_gpu, set_params, set_table_dir
+from .solver import Group, Solver, get_results, mc_gpu, set_dropin_modes, set_params, set_table_dir
 from .tables import Tables, synthetic_nb
 
 __all__ = [
     "lib", "Params", "VARIANT_PLAIN", "VARIANT_BATCHED", "base_params", "config", "reference_params",
-    "Solver", "Group", "mc_gpu", "get_results", "set_params", "set_table_dir", "Tables", "synthetic_nb",
+    "Solver", "Group", "mc_gpu", "get_results", "set_params", "set_table_dir", "set_dropin_modes", "Tables", "synthetic_nb",
 ]

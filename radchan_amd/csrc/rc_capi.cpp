@@ -1144,6 +1144,7 @@ static bool g_params_set = false;
 static rc_params g_params;
 static std::string g_dir = "properties";
 static int g_variant = RC_VARIANT_BATCHED;
+static int g_estimator = 0, g_restriction = 0;   /* options of the drop-in session */
 static rc_ctx *g_ctx = nullptr;
 static bool g_pending = false;
 static std::string g_tab_dir;
@@ -1210,6 +1211,17 @@ int rc_set_variant(int variant) {
     return 0;
 }
 
+int rc_set_dropin_modes(int estimator, int restriction) {
+    if ((estimator != 0 && estimator != 1) || (restriction != 0 && restriction != 1)) {
+        set_error("drop-in modes: estimator and restriction are 0 or 1");
+        return -1;
+    }
+    std::lock_guard<std::mutex> lk(g_mu);
+    g_estimator = estimator;
+    g_restriction = restriction;
+    return 0;
+}
+
 static void report(const char *where) { std::fprintf(stderr, "radchan %s: %s\n", where, last_error()); }
 
 void mc_gpu_(float *Tfort, int *zstart, int *rank) {
@@ -1237,6 +1249,7 @@ void mc_gpu_(float *Tfort, int *zstart, int *rank) {
         if (rc_load_tables(g_ctx, &v)) { report("mc_gpu_"); return; }
         g_tab_dir = g_dir;
     }
+    if (rc_set_estimator(g_ctx, g_estimator) || rc_set_restriction(g_ctx, g_restriction)) { report("mc_gpu_"); return; }
     const float *T = Tfort;
     if (rc_set_fields(g_ctx, 1, &T, *zstart)) { report("mc_gpu_"); return; }
     if (rc_run(g_ctx, g_variant)) { report("mc_gpu_"); return; }

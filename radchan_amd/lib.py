@@ -122,6 +122,7 @@ def load() -> C.CDLL:
         "rc_spline_restrict": (C.c_int, [P(Params), C.c_int, P(C.c_float), P(C.c_float), P(C.c_float)]),
         "rc_set_restriction": (C.c_int, [C.c_void_p, C.c_int]),
         "rc_set_estimator": (C.c_int, [C.c_void_p, C.c_int]),
+        "rc_set_dropin_modes": (C.c_int, [C.c_int, C.c_int]),
         "rc_ckdist_device": (C.c_int, [C.c_int, C.c_void_p] + [C.c_void_p] * 4 + [C.c_int, C.c_double, C.c_int]
                              + [C.c_void_p] * 5 + [C.c_int, C.c_void_p]),
         "rc_ck_tables": (C.c_int, [C.c_int, C.c_int, C.c_int] + [P(C.c_double)] * 6 + [C.c_double, P(TablesC)]),

@@ -153,6 +153,11 @@ def set_table_dir(d: str) -> None:
     L.check(L.load().rc_set_table_dir(d.encode()), "rc_set_table_dir")
 
 
+def set_dropin_modes(estimator: int = 0, restriction: int = 0) -> None:
+    """Estimator and restriction modes of the mc_gpu_ session."""
+    L.check(L.load().rc_set_dropin_modes(estimator, restriction), "rc_set_dropin_modes")
+
+
 def mc_gpu(T: np.ndarray, zstart: int = 1, rank: int = 0) -> None:
     """MC_GPU(TMC, xstart(3), nrank): launch the solve of field T."""
     T = np.ascontiguousarray(T, dtype=np.float32).reshape(-1)

@@ -59,3 +59,33 @@ def test_get_results_without_launch_is_an_error():
     R.get_results(res, var)
     assert np.array_equal(res, before)
     assert "without a preceding mc_gpu_" in R.lib.last_error()
+
+
+def test_mc_gpu_with_exact_estimator_and_spline_restriction(tmp_path, nb_tables):
+    """The drop-in session with both options on equals a session with the
+    same options (three grid levels, so the restriction matters)."""
+    from tests import cases
+    p, tab, T = cases.mg_case(nb_tables)
+    p.calcwalls = 0
+    tab.write(str(tmp_path), p.chanHeight)
+    R.set_params(p)
+    R.set_table_dir(str(tmp_path))
+    R.set_dropin_modes(1, 1)
+    n = (p.imax + 2) * (p.jmax + 2) * (p.kmax + 2)
+    res = np.zeros(n, np.float32)
+    var = np.zeros(n, np.float32)
+    try:
+        R.mc_gpu(T, zstart=1, rank=0)
+        R.get_results(res, var, rank=0)
+    finally:
+        R.set_dropin_modes(0, 0)
+    t2 = R.Tables.read(str(tmp_path), tab.nB, tab.nT, tab.nQ, p.chanHeight)
+    s = R.Solver(p, t2)
+    s.set_estimator(1)
+    s.set_restriction(1)
+    s.set_fields([T])
+    s.solve(R.VARIANT_BATCHED)
+    mean, _ = s.results()
+    s.close()
+    r3 = res.reshape(p.kmax + 2, p.jmax + 2, p.imax + 2)
+    assert np.array_equal(r3[1:-1, 1:-1, 1:-1], mean)
