@@ -456,3 +456,24 @@ def test_gpu_generator_against_the_reference_golden():
     np.testing.assert_allclose(tab.kP, g["kP"], rtol=3e-7)
     np.testing.assert_allclose(tab.prob, g["cuprob"], atol=6e-8)
     np.testing.assert_allclose(tab.probg, g["cuprob_g"], atol=6e-8)
+
+
+@pytest.mark.gpu
+def test_gpu_ckdist_uneven_grids_and_zero_k():
+    """Spectra on uneven wavenumber grids (weights differ per pair), with
+    k = 0 points (k^0.1 = 0 at the bottom of the range) and a range of ten
+    decades, against the oracle."""
+    rng = np.random.default_rng(21)
+    wv, kv, off, wl, wr = [], [], [0], [], []
+    for i, n in enumerate([40, 333, 1000, 1500]):
+        nu = 4000.0 + np.cumsum(rng.uniform(0.002, 0.03, n))
+        k = 10.0 ** rng.uniform(-5.0, 5.0, n)
+        if i % 2 == 0:
+            k[rng.integers(0, n, n // 10)] = 0.0
+        wv.append(nu)
+        kv.append(k)
+        off.append(off[-1] + n)
+        wl.append(nu[0])
+        wr.append(nu[-1])
+    print("rc_ckdist: uneven grids, zero k", flush=True)
+    _gpu_vs_oracle(np.array(off), np.concatenate(wv), np.concatenate(kv), np.array(wl), np.array(wr))

@@ -138,3 +138,18 @@ def test_oracle_spline_equals_the_reference_routines(tmp_path, n, seed):
     y2, ys = O.spline(x, y, xs)
     assert np.array_equal(y2.view(np.uint32), raw[:n].view(np.uint32))
     assert np.array_equal(ys.view(np.uint32), raw[n:].view(np.uint32))
+
+
+def test_library_rejects_bad_restriction_input():
+    """Input checks of rc_spline_restrict and the drop-in modes, before any
+    device work (so they run without a GPU)."""
+    from radchan_amd import lib as L
+    from radchan_amd.solver import set_dropin_modes, spline_restrict
+    p, T = _case()
+    with pytest.raises(RuntimeError, match="level"):
+        spline_restrict(p, 0, T)
+    with pytest.raises(RuntimeError, match="level"):
+        spline_restrict(p, p.grid_num, T)
+    with pytest.raises(RuntimeError, match="0 or 1"):
+        set_dropin_modes(2, 0)
+    set_dropin_modes(0, 0)
