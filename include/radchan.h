@@ -224,7 +224,8 @@ int rc_info(rc_ctx *c, int s, float *f5, int32_t *i2);
 /* schedule of the batched solver: persistent workgroups (one wavefront
  * each) per compute unit, the number of idle lanes that triggers a
  * refill, and march steps per loop iteration (rays that end on an earlier
- * step wait for the next refill); 0 selects the default (32, 8, 3).
+ * step wait for the next refill); 0 selects the default, (32, 8, 3) for a
+ * single grid and (20, 8, 4) for several levels.
  * Chosen by the autotuner. */
 int rc_set_schedule(rc_ctx *c, int wg_per_cu, int threshold, int steps_per_iter);
 int rc_get_schedule(rc_ctx *c, int32_t *wg_per_cu, int32_t *threshold, int32_t *steps_per_iter);

@@ -758,9 +758,13 @@ void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, i
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         if (ncu <= 0) ncu = 256;
     }
-    if (wg_per_cu <= 0) wg_per_cu = 32;
+    /* defaults: what the autotuner picks on 64^3 (single grid) and on the
+     * three-level 192x128x192 case (multigrid: fewer, longer-stepping
+     * wavefronts) */
+    const bool mg = cm.grid_num > 1;
+    if (wg_per_cu <= 0) wg_per_cu = mg ? 20 : 32;
     if (threshold <= 0) threshold = 8;
-    if (spi <= 0) spi = 3;
+    if (spi <= 0) spi = mg ? 4 : 3;
     int n2 = 1;
     while (n2 < cm.photmax) n2 <<= 1;
     const size_t lds = (size_t)batched_lds_words(n2, cm.srt, cm.grid_num > 1) * 4;
