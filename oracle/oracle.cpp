@@ -53,6 +53,21 @@ constexpr double kPi = 3.14159265358979323846;           /* param.h:88 */
 constexpr double kC1 = 3.741771790075259e-16;            /* memory_copy_noint.cu:292 */
 constexpr double kC2 = 0.014387741858429;                /* memory_copy_noint.cu:293 */
 
+/* Planted mistakes (orc_plant): test-only switches that make the oracle
+ * commit one plausible error, so that the tests can show their closed-form
+ * checks reject it.  0 = none (always, outside those tests). */
+enum {
+    PLANT_NONE = 0,
+    PLANT_GREY_ROW = 1,        /* grey band CDF of row i built at Tnb[i+1]          */
+    PLANT_GREY_WVL = 2,        /* grey band CDF with Ib at the band's left edge     */
+    PLANT_WALL_ROW = 3,        /* wall band CDF (readS) at row t*+1                 */
+    PLANT_WALL_G_UNIFORM = 4,  /* wall g CDF uniform instead of cumulative wq       */
+    PLANT_TEX_NO_SHIFT = 5,    /* texture fetch without the half-texel shift        */
+    PLANT_TMAP_NO_GHOST = 6,   /* coarse-cell texel coordinate without the ghost +1 */
+    PLANT_XWALL_GHOST = 7,     /* adiabatic x-wall T at the ghost texel centre      */
+};
+int g_plant = PLANT_NONE;
+
 enum { FW = 0, FE = 1, FS = 2, FN = 3, FB = 4, FT = 5 };
 enum { TAG_FLUID = 0, TAG_WEST = 1, TAG_EAST = 2 };
 
@@ -233,7 +248,7 @@ float tex3_T(const orc_ctx &c, float tx, float ty, float tz) {
     int i0[3], i1[3];
     float w[3];
     for (int d = 0; d < 3; ++d) {
-        const float b = crd[d] - 0.5f;
+        const float b = (g_plant == PLANT_TEX_NO_SHIFT) ? crd[d] : crd[d] - 0.5f;
         const float fl = std::floor(b);
         w[d] = b - fl;
         int lo = (int)fl, hi = lo + 1;
@@ -251,7 +266,17 @@ float tex3_T(const orc_ctx &c, float tx, float ty, float tz) {
 }
 
 /* mapped cell-centre texel coordinate of a coarse index (:334-336) */
-inline float tmap(int ic, int r) { return ((float)ic - 0.5f) * (float)r + 1.0f; }
+inline float tmap(int ic, int r) {
+    return (g_plant == PLANT_TMAP_NO_GHOST) ? ((float)ic - 0.5f) * (float)r : ((float)ic - 0.5f) * (float)r + 1.0f;
+}
+
+/* texel x coordinate of the adiabatic wall temperature: halfway between the
+ * ghost plane and the first (last) cell, x = 1.0 (imax + 1.0)
+ * (mc_gpu_noint.cu:391-399, 422-430) */
+inline float xwall_tex(const orc_params &p, int bound) {
+    if (g_plant == PLANT_XWALL_GHOST) return bound ? (float)p.imax + 1.5f : 0.5f;
+    return bound ? (float)p.imax + 1.0f : 1.0f;
+}
 
 /* Gridn::mk_grid, definitions.h:68-136 */
 void build_level(const orc_params &p, int g, Level &L) {
@@ -472,11 +497,11 @@ int march_level(const orc_ctx &c, int lev, Ray<G> &r, const Energy &e, float tol
                 const size_t wq = (size_t)(r.jc - 1) + (size_t)L.jm * (r.kc - 1);
                 if (face == FW && p.adiabw == 1) {
                     fixed = false;
-                    Tw_ = L.Tws.empty() ? tex3_T(c, 1.0f, tmap(r.jc, L.ry), tmap(r.kc, L.rz)) : L.Tws[wq];
+                    Tw_ = L.Tws.empty() ? tex3_T(c, xwall_tex(p, 0), tmap(r.jc, L.ry), tmap(r.kc, L.rz)) : L.Tws[wq];
                 }
                 if (face == FE && p.adiabe == 1) {
                     fixed = false;
-                    Tw_ = L.Tws.empty() ? tex3_T(c, (float)p.imax + 1.0f, tmap(r.jc, L.ry), tmap(r.kc, L.rz))
+                    Tw_ = L.Tws.empty() ? tex3_T(c, xwall_tex(p, 1), tmap(r.jc, L.ry), tmap(r.kc, L.rz))
                                         : L.Tws[(size_t)L.jm * L.km + wq];
                 }
                 wall_tally<G>(c, e, r, Tw_, t, fixed, fixed ? c.bfix[face][e.rank] : 0.f);
@@ -678,8 +703,7 @@ void surface_ray(const orc_ctx &c, int bound, int j, int k, int v, int h, int ra
     if (std::fabs((double)sz) <= 1e-10) sz = (G)((float)((sz > 0) - (sz <= 0)) * 1e-10f);
     set_dir<G>(r, sx, sy, sz);
     r.tra = 1; r.tra64 = 1.0;
-    const float Tw_ = bound ? tex3_T(c, (float)p.imax + 1.0f, (float)j + 0.5f, (float)k + 0.5f)
-                            : tex3_T(c, 1.0f, (float)j + 0.5f, (float)k + 0.5f);
+    const float Tw_ = tex3_T(c, xwall_tex(p, bound), (float)j + 0.5f, (float)k + 0.5f);
     Energy e;
     energy_setup(c, rank, g, Tw_, true, bound, e);
     /* on-face index adjustment (:1429-1441) */
@@ -865,8 +889,10 @@ int orc_grey_tables(const orc_params *p, float Tmax, orc_tables *t) {
     }
     for (int i = 0; i < nT; ++i) {
         float *row = &t->prob[(size_t)i * nB];
-        row[0] = planck_f(t->Tnb[i], t->wvc[0]) * (wvr[0] - wvl[0]) * 100.f;
-        for (int j = 1; j < nB; ++j) row[j] = row[j - 1] + planck_f(t->Tnb[i], t->wvc[j]) * (wvr[j] - wvl[j]) * 100.f;
+        const float Tr = t->Tnb[(g_plant == PLANT_GREY_ROW) ? std::min(i + 1, nT - 1) : i];
+        const float *nu = (g_plant == PLANT_GREY_WVL) ? wvl.data() : t->wvc;
+        row[0] = planck_f(Tr, nu[0]) * (wvr[0] - wvl[0]) * 100.f;
+        for (int j = 1; j < nB; ++j) row[j] = row[j - 1] + planck_f(Tr, nu[j]) * (wvr[j] - wvl[j]) * 100.f;
         const float last = row[nB - 1];
         for (int j = 0; j < nB; ++j) { row[j] = row[j] / last; t->probg[(size_t)i * nB + j] = 1.0f; }
     }
@@ -949,7 +975,7 @@ orc_ctx *orc_create(const orc_params *pp, const orc_tables *t, const float *T, i
     /* surface CDFs: readS (read_tables.cpp:188-220) at row t*, cumulative wq */
     c->cdf_bs.assign(nB, 0.f);
     {
-        const float Tr = c->Tnb[ts];
+        const float Tr = c->Tnb[(g_plant == PLANT_WALL_ROW) ? std::min(ts + 1, nT - 1) : ts];
         float acc = 0.f;
         for (int b = 0; b < nB; ++b) {
             const float ib = c->grey_true ? 1.0f : planck_f(Tr, t->wvc[b]);
@@ -966,6 +992,8 @@ orc_ctx *orc_create(const orc_params *pp, const orc_tables *t, const float *T, i
         float acc = 0.f;
         for (int g = 0; g < nQ; ++g) { acc = (g == 0) ? t->wq[0] : acc + t->wq[g]; c->cdf_gs[g] = acc; }
         for (int g = 0; g < nQ; ++g) c->cdf_gs[g] = c->cdf_gs[g] / acc;
+        if (g_plant == PLANT_WALL_G_UNIFORM)
+            for (int g = 0; g < nQ; ++g) c->cdf_gs[g] = (float)(g + 1) / (float)nQ;
     }
     /* fixed-temperature wall emission per rank (black_copy, memory_copy_noint.cu:27-52) */
     for (int f = 0; f < 6; ++f) {
@@ -983,6 +1011,38 @@ void orc_destroy(orc_ctx *c) { delete c; }
 
 /* streams of snapshot s of a batch draw with stream tag + 16 s */
 void orc_set_snapshot(orc_ctx *c, int s) { c->tagoff = 16u * (uint32_t)s; }
+
+void orc_plant(int id) { g_plant = id; }
+
+float orc_tex3(const orc_ctx *c, float tx, float ty, float tz) { return tex3_T(*c, tx, ty, tz); }
+
+/* the temperatures the march reads on level g (texture mean; the spline
+ * restriction's values when it is on): cells Tc [km][jm][im], adiabatic x
+ * walls Tw [2][km][jm], bd=4 z walls Tz [2][jm][im] (either may be NULL) */
+int orc_level_T(const orc_ctx *c, int g, float *Tc, float *Tw, float *Tz) {
+    if (!c || g < 0 || g >= c->p.grid_num) return -1;
+    const orc_params &p = c->p;
+    const Level &L = c->lev[g];
+    for (int k = 1; k <= L.km; ++k)
+        for (int j = 1; j <= L.jm; ++j)
+            for (int i = 1; i <= L.im; ++i) {
+                const size_t n = (size_t)(i - 1) + (size_t)L.im * ((size_t)(j - 1) + (size_t)L.jm * (k - 1));
+                if (Tc) Tc[n] = L.Ts.empty() ? tex3_T(*c, tmap(i, L.rx), tmap(j, L.ry), tmap(k, L.rz)) : L.Ts[n];
+            }
+    for (int b = 0; b < 2 && Tw; ++b)
+        for (int k = 1; k <= L.km; ++k)
+            for (int j = 1; j <= L.jm; ++j) {
+                const size_t n = (size_t)(j - 1) + (size_t)L.jm * (k - 1);
+                Tw[(size_t)b * L.jm * L.km + n] = L.Tws.empty()
+                    ? tex3_T(*c, xwall_tex(p, b), tmap(j, L.ry), tmap(k, L.rz)) : L.Tws[(size_t)b * L.jm * L.km + n];
+            }
+    for (int b = 0; b < 2 && Tz; ++b)
+        for (int j = 1; j <= L.jm; ++j)
+            for (int i = 1; i <= L.im; ++i)
+                Tz[(size_t)b * L.im * L.jm + (size_t)(i - 1) + (size_t)L.im * (j - 1)] =
+                    tex3_T(*c, tmap(i, L.rx), tmap(j, L.ry), b ? (float)p.kmax + 0.5f : 1.5f);
+    return 0;
+}
 
 void orc_get_info(const orc_ctx *c, orc_info *info) {
     info->Tmax = c->Tmax; info->kappamax = c->kappamax; info->Ibmax = c->Ibmax;

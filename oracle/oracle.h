@@ -119,6 +119,13 @@ orc_ctx *orc_create(const orc_params *p, const orc_tables *t, const float *T,
                     int zstart, char *err, int errlen);
 void     orc_destroy(orc_ctx *c);
 void     orc_set_snapshot(orc_ctx *c, int s);
+/* test-only: plant mistake `id` (see PLANT_* in oracle.cpp; 0 = none) */
+void     orc_plant(int id);
+/* the linear texture fetch of the temperature volume at texel coordinates */
+float    orc_tex3(const orc_ctx *c, float tx, float ty, float tz);
+/* temperatures read on level g: cells [km][jm][im], adiabatic x walls
+ * [2][km][jm], bd=4 z walls [2][jm][im] (either of the last two may be NULL) */
+int      orc_level_T(const orc_ctx *c, int g, float *Tc, float *Tw, float *Tz);
 void     orc_get_info(const orc_ctx *c, orc_info *info);
 /* band sequence of one (cell, run): photmax (rank, g) pairs in ray order */
 void     orc_band_sequence(const orc_ctx *c, int surface, int bound, int i,

@@ -46,6 +46,9 @@ def load():
         "orc_create": (vp, [P(L.Params), P(L.TablesC), P(f), C.c_int, C.c_char_p, C.c_int]),
         "orc_destroy": (None, [vp]),
         "orc_set_snapshot": (None, [vp, C.c_int]),
+        "orc_plant": (None, [C.c_int]),
+        "orc_tex3": (f, [vp, f, f, f]),
+        "orc_level_T": (C.c_int, [vp, C.c_int, P(f), P(f), P(f)]),
         "orc_get_info": (None, [vp, C.c_void_p]),
         "orc_band_sequence": (None, [vp, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int, C.c_int,
                                      P(C.c_int32), P(C.c_int32)]),
@@ -195,6 +198,24 @@ def grey_tables(p, Tmax: float):
     return t
 
 
+class planted:
+    """Context manager: the oracle commits planted mistake `pid` inside the
+    block (test-only; see PLANT_* in oracle.cpp)."""
+
+    GREY_ROW, GREY_WVL, WALL_ROW, WALL_G_UNIFORM, TEX_NO_SHIFT, TMAP_NO_GHOST, XWALL_GHOST = range(1, 8)
+
+    def __init__(self, pid: int):
+        self.pid = pid
+
+    def __enter__(self):
+        load().orc_plant(self.pid)
+        return self
+
+    def __exit__(self, *exc):
+        load().orc_plant(0)
+        return False
+
+
 # ---- solver ------------------------------------------------------------------
 
 class Oracle:
@@ -238,6 +259,19 @@ class Oracle:
         i = Info()
         self.lib.orc_get_info(self.ctx, C.byref(i))
         return {k: getattr(i, k) for k, _ in Info._fields_}
+
+    def tex3(self, tx: float, ty: float, tz: float) -> float:
+        return self.lib.orc_tex3(self.ctx, tx, ty, tz)
+
+    def level_T(self, g: int):
+        """Temperatures the march reads on level g: cells [km][jm][im],
+        adiabatic x walls [2][km][jm], bd=4 z walls [2][jm][im]."""
+        im, jm, km = self.p.maxi[g], self.p.maxj[g], self.p.maxk[g]
+        tc = np.zeros(im * jm * km, np.float32)
+        tw = np.zeros(2 * jm * km, np.float32)
+        tz = np.zeros(2 * im * jm, np.float32)
+        assert self.lib.orc_level_T(self.ctx, g, _f(tc), _f(tw), _f(tz)) == 0
+        return tc.reshape(km, jm, im), tw.reshape(2, km, jm), tz.reshape(2, jm, im)
 
     def band_sequence(self, i: int, j: int, k: int, v: int, surface: bool = False, bound: int = 0):
         P = self.p.photmax
