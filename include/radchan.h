@@ -180,6 +180,20 @@ int rc_ck_tables(int nB, int nT, int nQ, const double *T, const double *wvl, con
  * of the face means; interp3D's coarse ghost is 2 Tw - Tc).  Uses the GPU;
  * synchronous. */
 int rc_spline_restrict(const rc_params *p, int level, const float *T, float *Tc, float *Tw);
+/* No counterpart in the reference: rc_spline_restrict for callers that
+ * cannot allocate device memory (code under HIP graph capture, serving
+ * loops, harnesses that forbid hipMalloc).  T, Tc and Tw (may be NULL) are
+ * DEVICE arrays of the sizes above; asynchronous on `stream` (a
+ * hipStream_t; NULL = the null stream); calls no hipMalloc/hipFree.  The
+ * grid and the spline's working space live in one fixed device array of
+ * RC_SPLINE_NOALLOC_FLOATS floats (64 MiB, held by every GPU the library
+ * runs on until the process ends); a field that needs more (about 2.7 times
+ * the fine interior at level 1, e.g. 192x128x192 fits, 256x128x256 does
+ * not) is refused with an error.  Calls that overlap on one GPU share that
+ * array and race: order them on one stream. */
+#define RC_SPLINE_NOALLOC_FLOATS (16u << 20)
+int rc_spline_restrict_noalloc(const rc_params *p, int level, const float *T, float *Tc, float *Tw,
+                               void *stream);
 /* rc_ckdist on device arrays (spectra already in HBM, e.g. written there by
  * another kernel), asynchronous on `stream` (a hipStream_t; NULL = the null
  * stream), nothing allocated.  max_points bounds the points of every spectrum (1..8192; it

@@ -26,6 +26,7 @@
 
 #include <algorithm>
 #include <cstdint>
+#include <mutex>
 
 #include "rc_internal.h"
 #include "rc_kernels.h"
@@ -185,4 +186,146 @@ void launch_spline_level(hipStream_t st, const float *T0, const float *x0, const
     }
 }
 
+/* ---- the allocation-free entry point --------------------------------
+
+rc_spline_restrict_noalloc works in `scratch`, a file-scope device array
+of RC_SPLINE_NOALLOC_FLOATS floats that every GPU the library runs on holds
+until the process ends.  The host never passes the array's name to a kernel
+(in host code it is a stand-in, not a device address): the device address
+is looked up once per device with hipGetSymbolAddress, with that device
+current, and kept in scratch_ptr[device]. */
+
+namespace {
+
+constexpr size_t kNoallocFloats = RC_SPLINE_NOALLOC_FLOATS;
+static __device__ __align__(256) float scratch[kNoallocFloats];
+
+constexpr int kMaxDevices = 64;
+std::mutex scratch_mu;
+float *scratch_ptr[kMaxDevices] = {};
+
+/* the device address of this device's scratch (nullptr and a message on failure) */
+float *device_scratch(int dev) {
+    if (dev < 0 || dev >= kMaxDevices) {
+        set_error("rc_spline_restrict_noalloc: device ordinal beyond 63");
+        return nullptr;
+    }
+    std::lock_guard<std::mutex> lk(scratch_mu);
+    if (!scratch_ptr[dev]) {
+        void *p = nullptr;
+        const hipError_t e = hipGetSymbolAddress(&p, HIP_SYMBOL(scratch));
+        if (e != hipSuccess || !p) {
+            set_error(std::string("rc_spline_restrict_noalloc: hipGetSymbolAddress: ") + hipGetErrorString(e));
+            return nullptr;
+        }
+        scratch_ptr[dev] = static_cast<float *>(p);
+    }
+    return scratch_ptr[dev];
+}
+
+/* working floats of the grid step: f[im0+1], then the centres x0, y0, z0
+ * of level 0 and xc, yc, zc of the coarse level (n+2 each), rounded up to
+ * 64 floats so the spline scratch after it stays 256-byte aligned */
+size_t grid_floats(int im0, int jm0, int km0, int im, int jm, int km) {
+    const size_t n = (size_t)(im0 + 1) + (im0 + 2) + (jm0 + 2) + (km0 + 2) + (im + 2) + (jm + 2) + (km + 2);
+    return (n + 63) / 64 * 64;
+}
+
+struct GridArgs {
+    int im0, jm0, km0, im, jm, km;
+    float Lx, Ly, Lz;
+    double fact_mesh;
+    float *f, *x0, *y0, *z0, *xc, *yc, *zc;
+};
+
+/* centres of n uniform cells of length L with mirrored ghosts (the y and
+ * z directions of Gridn::mk_grid, definitions.h:85-105) */
+__device__ void uniform_centres(float L, int n, float *c) {
+    const float d = L * 1.0f / (float)n;
+    c[0] = -d / 2.0f;
+    for (int j = 1; j <= n; ++j) c[j] = ((float)(j - 1) * d + (float)j * d) / 2.0f;
+    c[n + 1] = c[n] + d;
+}
+
+/* x centres of a level with im cells from the fine faces f[0..im0]
+ * (definitions.h:125-134) */
+__device__ void x_centres(const float *f, int im0, int im, float *x) {
+    float prev = 0.f;
+    for (int i = 1; i <= im; ++i) {
+        const float xu = f[(size_t)i * im0 / im];
+        x[i] = (float)(0.5 * (double)(xu + prev));
+        prev = xu;
+    }
+    x[0] = -x[1];
+    x[im + 1] = prev + (prev - x[im]);
+}
+
+/* Gridn::mk_grid (definitions.h:68-136) of level 0 and of the coarse
+ * level, in one thread: the operations of the host's build_grid in its
+ * order, so the centres are equal to its bit for bit */
+__global__ void k_level_grid(GridArgs a) {
+    if (blockIdx.x != 0 || threadIdx.x != 0) return;
+    float *f = a.f;
+    const int half = a.im0 / 2;
+    f[0] = 0.f;
+    for (int i = 1; i <= half; ++i) {
+        const double r = (double)((float)i / (float)a.im0) - 0.5;
+        f[i] = f[i - 1] + (float)(0.5 - a.fact_mesh * r * r);
+    }
+    const double norm = (double)f[half];
+    const double scale = (double)a.Lx / 2.;
+    for (int i = 1; i <= half; ++i) f[i] = (float)(scale * (double)f[i] / norm);
+    for (int i = a.im0; i > half; --i) f[i] = a.Lx - f[a.im0 - i];
+    x_centres(f, a.im0, a.im0, a.x0);
+    x_centres(f, a.im0, a.im, a.xc);
+    uniform_centres(a.Ly, a.jm0, a.y0);
+    uniform_centres(a.Lz, a.km0, a.z0);
+    uniform_centres(a.Ly, a.jm, a.yc);
+    uniform_centres(a.Lz, a.km, a.zc);
+}
+
+}  // namespace
+
 }  // namespace rc
+
+/* allocation-free rc_spline_restrict on device arrays (declared in
+ * include/radchan.h): grid and spline work in this device's scratch,
+ * asynchronous on `stream` */
+extern "C" int rc_spline_restrict_noalloc(const rc_params *p, int level, const float *T, float *Tc, float *Tw,
+                                          void *stream) {
+    using namespace rc;
+    if (!p || !T || !Tc) { set_error("rc_spline_restrict_noalloc: null argument"); return -1; }
+    if (int e = check_params(*p)) return e;
+    if (level < 1 || level >= p->grid_num) { set_error("rc_spline_restrict_noalloc: level in 1..grid_num-1"); return -2; }
+    if (p->imax < 2 || p->jmax < 2 || p->kmax < 2) { set_error("cubic-spline restriction needs imax, jmax, kmax >= 2"); return -2; }
+    const int im = p->maxi[level], jm = p->maxj[level], km = p->maxk[level];
+    const size_t ng = grid_floats(p->imax, p->jmax, p->kmax, im, jm, km);
+    const size_t need = ng + spline_scratch_floats(p->imax, p->jmax, p->kmax, im, jm, km);
+    if (need > kNoallocFloats) {
+        set_error("rc_spline_restrict_noalloc: the field needs " + std::to_string(need) +
+                  " floats of scratch, beyond its fixed " + std::to_string(kNoallocFloats) +
+                  " (use rc_spline_restrict)");
+        return -2;
+    }
+    int dev = 0;
+    if (hipGetDevice(&dev) != hipSuccess) { set_error("no HIP device available"); return -3; }
+    float *w = device_scratch(dev);
+    if (!w) return -3;
+    GridArgs g;
+    g.im0 = p->imax; g.jm0 = p->jmax; g.km0 = p->kmax; g.im = im; g.jm = jm; g.km = km;
+    g.Lx = p->Lx; g.Ly = p->Ly; g.Lz = p->Lz; g.fact_mesh = p->fact_mesh;
+    g.f = w;
+    g.x0 = g.f + (p->imax + 1);
+    g.y0 = g.x0 + (p->imax + 2);
+    g.z0 = g.y0 + (p->jmax + 2);
+    g.xc = g.z0 + (p->kmax + 2);
+    g.yc = g.xc + (im + 2);
+    g.zc = g.yc + (jm + 2);
+    hipStream_t st = static_cast<hipStream_t>(stream);
+    hipLaunchKernelGGL(k_level_grid, dim3(1), dim3(1), 0, st, g);
+    launch_spline_level(st, T, g.x0, g.y0, g.z0, g.xc, g.yc, g.zc, p->imax, p->jmax, p->kmax, im, jm, km, Tc, Tw,
+                        w + ng);
+    const hipError_t e = hipGetLastError();
+    if (e != hipSuccess) { set_error(std::string("rc_spline_restrict_noalloc: ") + hipGetErrorString(e)); return -4; }
+    return 0;
+}

@@ -120,6 +120,7 @@ def load() -> C.CDLL:
         "rc_sort_idx": (C.c_int, [P(C.c_float), C.c_int, P(C.c_int32)]),
         "rc_grid": (C.c_int, [P(Params), C.c_int] + [P(C.c_float)] * 6),
         "rc_spline_restrict": (C.c_int, [P(Params), C.c_int, P(C.c_float), P(C.c_float), P(C.c_float)]),
+        "rc_spline_restrict_noalloc": (C.c_int, [P(Params), C.c_int, vp, vp, vp, vp]),
         "rc_set_restriction": (C.c_int, [C.c_void_p, C.c_int]),
         "rc_set_estimator": (C.c_int, [C.c_void_p, C.c_int]),
         "rc_set_dropin_modes": (C.c_int, [C.c_int, C.c_int]),
