@@ -524,6 +524,94 @@ __device__ __forceinline__ void march_step(const DevCommon &cm, const S &sn, con
     if (march_core<TM>(cm, sn, L, r, e, tol, de, status)) boundary<TM>(cm, sn, L, r, e, de, status);
 }
 
+/* The boundary condition of the face crossed, as a branch over the face:
+ * the same outcome and tally as boundary(), in the form a thread that
+ * marches its own ray (plain variant, traced rays) runs fastest -- its
+ * lanes cross faces at steps of their own, so the face code is read only
+ * where one is crossed. */
+__device__ __forceinline__ void boundary_thread(const DevCommon &cm, const DevSnap &sn, const DevLevel &L, Ray &r,
+                                                const En &e, float &de, int &status) {
+    const int nT = cm.nT;
+    int face;
+    if (r.ic == 0) face = FW;
+    else if (r.ic == L.im + 1) face = FE;
+    else if (r.jc == 0) face = FS;
+    else if (r.jc == L.jm + 1) face = FN;
+    else if (r.kc == 0) face = FB;
+    else face = FT;
+    const int bd = cm.bd[face];
+    if (bd == 1) {
+        switch (face) {
+            case FW: r.ic = L.im; r.xp = cm.Lx; break;
+            case FE: r.ic = 1; r.xp = 0.f; break;
+            case FS: r.jc = L.jm; r.yp = cm.Ly; break;
+            case FN: r.jc = 1; r.yp = 0.f; break;
+            case FB: r.kc = L.km; r.zp = cm.Lz; break;
+            default: r.kc = 1; r.zp = 0.f; break;
+        }
+    } else if (bd == 3) {
+        switch (face) {
+            case FW: r.xp = 0.f; r.sx = -r.sx; r.isx = -r.isx; r.ic = 1; r.f0 = r.sx < 0.f; break;
+            case FE: r.xp = cm.Lx; r.sx = -r.sx; r.isx = -r.isx; r.ic = L.im; r.f0 = r.sx < 0.f; break;
+            case FS: r.yp = 0.f; r.sy = -r.sy; r.isy = -r.isy; r.jc = 1; r.f1 = r.sy < 0.f; break;
+            case FN: r.yp = cm.Ly; r.sy = -r.sy; r.isy = -r.isy; r.jc = L.jm; r.f1 = r.sy < 0.f; break;
+            case FB: r.zp = 0.f; r.sz = -r.sz; r.isz = -r.isz; r.kc = 1; r.f2 = r.sz < 0.f; break;
+            default: r.zp = cm.Lz; r.sz = -r.sz; r.isz = -r.isz; r.kc = L.km; r.f2 = r.sz < 0.f; break;
+        }
+    } else if (bd == 2 || bd == 4) {
+        float Bw, Kw;
+        if (bd == 4) {
+            const float Tw = gld(L.Tz, (size_t)(face == FT) * L.im * L.jm + (size_t)(r.ic - 1) + (size_t)L.im * (r.jc - 1));
+            kb_at(sn, e.row, e.rank, nT, Tw, Kw, Bw);
+        } else if ((face == FW && cm.adiab[0]) || (face == FE && cm.adiab[1])) {
+            const float Tw = gld(L.Tw, (size_t)(face == FE) * L.jm * L.km + (size_t)(r.jc - 1) + (size_t)L.jm * (r.kc - 1));
+            kb_at(sn, e.row, e.rank, nT, Tw, Kw, Bw);
+        } else {
+            Bw = gld(sn.bfix, (size_t)face * cm.nB + e.rank);
+        }
+        wall_hit(e, r, Bw, de);
+        status = RC_RAY_WALL;
+    } else {
+        r.tra = 0.f;
+        status = RC_RAY_LOST;
+    }
+}
+
+/* one march step of a thread that marches its own ray: march_core's
+ * arithmetic with the temperature load issued as soon as the cell index is
+ * known, then boundary_thread */
+__device__ __forceinline__ void march_step_thread(const DevCommon &cm, const DevSnap &sn, const DevLevel &L, Ray &r,
+                                                  const En &e, float tol, float &de, int &status) {
+    const float fx = gld32(L.xu, min((uint32_t)(r.ic - r.f0), (uint32_t)L.im));
+    const float fy = (float)(r.jc - r.f1) * L.dy;
+    const float fz = (float)(r.kc - r.f2) * L.dz;
+    const float dsx = (fx - r.xp) * r.isx;
+    const float dsy = (fy - r.yp) * r.isy;
+    const float dsz = (fz - r.zp) * r.isz;
+    const uint32_t cell = min((uint32_t)(r.ic - 1) + __umul24((uint32_t)L.im, (uint32_t)(r.jc - 1) +
+                                                                 __umul24((uint32_t)L.jm, (uint32_t)(r.kc - 1))),
+                              (uint32_t)L.ncell - 1u);
+    const float Tc = gld32(L.T, cell);
+    float ds = dsx;
+    ds = (ds > dsy) ? dsy : ds;
+    ds = (ds > dsz) ? dsz : ds;
+    r.xp = (ds == dsx) ? fx : fmaf(ds, r.sx, r.xp);
+    r.yp = (ds == dsy) ? fy : fmaf(ds, r.sy, r.yp);
+    r.zp = (ds == dsz) ? fz : fmaf(ds, r.sz, r.zp);
+    float K, B;
+    kb_at(sn, e.row, e.rank, cm.nT, Tc, K, B);
+    phys_core(K, B, ds, e, tol, r.tra, de, status);
+    const bool xa = dsx < dsy && dsx < dsz;
+    const bool ya = !(dsx < dsy) && dsy < dsz;
+    const bool za = !xa && !ya;
+    r.ic += xa ? (r.f0 ? -1 : 1) : 0;
+    r.jc += ya ? (r.f1 ? -1 : 1) : 0;
+    r.kc += za ? (r.f2 ? -1 : 1) : 0;
+    if ((uint32_t)(r.ic - 1) >= (uint32_t)L.im || (uint32_t)(r.jc - 1) >= (uint32_t)L.jm ||
+        (uint32_t)(r.kc - 1) >= (uint32_t)L.km)
+        boundary_thread(cm, sn, L, r, e, de, status);
+}
+
 /* one grid level of the march (march_ray, mc_gpu_noint.cu:245-658);
  * returns the steps taken on this level */
 template <bool TR>
@@ -533,7 +621,7 @@ __device__ __forceinline__ int march_level(const DevCommon &cm, const DevSnap &s
     int counter = 0;
     while (r.tra > tol && counter < L.maxs) {
         if (TR) hash = fnv(fnv(fnv(hash, (uint32_t)r.ic), (uint32_t)r.jc), (uint32_t)r.kc);
-        march_step(cm, sn, L, r, e, tol, de, status);
+        march_step_thread(cm, sn, L, r, e, tol, de, status);
         steps += 1;
         counter += 1;
     }
