@@ -5,22 +5,28 @@ A step is one solve of the narrow-band correlated-k OERM estimator over the
 whole field: every emitting cell x photmax rays x nVar independent runs,
 inputs (field, tables, grids) resident in HBM before the timed region.
 
-Workload (default): 64x64x64 cells, x stretched (fact_mesh 1.5), synthetic
-H2O-like tables of 360 bands x 16 g-points x 20 temperatures, photmax 200,
-sorted bands (srt=1), nVar 8, FP32 march with FP64 tallies.
+Workload (default, the headline): the 256x128x256 narrow-band field, x
+stretched (fact_mesh 1.5), synthetic H2O-like tables of 360 bands x 16
+g-points x 20 temperatures, photmax 100, sorted bands (srt=1), nVar 8, FP32
+march with FP64 tallies.
 
-Legs: an untimed autotune of the batched schedule; the batched solver with
-FP32 and with BF16 k-tables, each replayed as a hipGraph; the batched march
-kernel alone (HIP events, no graph) for the roofline; the plain solver; one
-solve at 4x photmax (checks eps ~ rays^-1/2, which the time-to-1%
-projection assumes); the 256x128x256 field, batched against plain; the
-correlated-k table generator (rc_ckdist) on 47 580 line-by-line spectra.
+Legs on the headline field: an untimed autotune of the batched schedule;
+the batched solver with FP32 and with BF16 k-tables, each replayed as a
+hipGraph; the batched march kernel alone (HIP events, no graph) for the
+roofline; the plain solver (the reference's launch shape, one launch per
+run); one solve at 4x photmax (checks eps ~ rays^-1/2, which the time-to-1%
+projection assumes).  Beside it: the 64^3 field (photmax 200) with the same
+legs and a measured run at the projected budget for eps = 1% (one launch
+of as many runs as the projection asks); the correlated-k table generator
+(rc_ckdist) on 47 580 line-by-line spectra; the CPU baseline (the C++
+oracle in FP32 on the host's CPU share and on one core, bounded samples of
+the grey 32^3, the 64^3 and the 256x128x256 fields).
 
-Printed (one JSON line on stdout): rays/s of the batched solver (FP32
-k-tables unless --tables bf16), the plain solver's rays/s beside it, the
-measured std-error eps of div(q_r) and the projected time to eps = 1% for
-every leg, the roofline of the dominant kernel and a CPU baseline (the C++
-oracle in FP32 on the host cores, a bounded sample).
+Printed (one JSON line on stdout): rays/s of the batched solver on the
+headline field (FP32 k-tables unless --tables bf16), the plain solver's
+rays/s beside it, the measured std-error eps of div(q_r) and the projected
+time to eps = 1% for every leg, the roofline of the dominant kernel and the
+CPU baseline.
 """
 from __future__ import annotations
 
@@ -73,33 +79,104 @@ def std_error(de: np.ndarray):
     return eps, p95
 
 
-def cpu_baseline(p, tab, T, seconds: float):
-    """The C++ oracle (FP32, reference semantics) on a sample of cells of the
-    same workload, on the host cores."""
-    from oracle import oracle as O
+def host_cpu() -> dict:
+    """The host's CPU: model name (lscpu, else /proc/cpuinfo), nproc, the
+    CPUs this process may run on, and the share the baseline uses: those
+    CPUs, capped by OMP_NUM_THREADS when set (the GPU box gives one GPU's
+    job a share of its cores that way)."""
+    import subprocess
+    model = None
     try:
-        ncores = len(os.sched_getaffinity(0))
+        out = subprocess.run(["lscpu"], capture_output=True, text=True, timeout=10).stdout
+        for line in out.splitlines():
+            if line.startswith("Model name:"):
+                model = line.split(":", 1)[1].strip()
+                break
+    except Exception:
+        pass
+    if model is None:
+        try:
+            with open("/proc/cpuinfo") as f:
+                for line in f:
+                    if line.startswith("model name"):
+                        model = line.split(":", 1)[1].strip()
+                        break
+        except OSError:
+            pass
+    try:
+        aff = len(os.sched_getaffinity(0))
     except AttributeError:
-        ncores = os.cpu_count() or 1
-    ncores = max(1, min(ncores, 16))
-    o = O.Oracle(p, tab, T)
-    cells = O.all_cells(p)
-    rng = np.random.default_rng(123)
+        aff = os.cpu_count() or 1
+    omp = int(os.environ.get("OMP_NUM_THREADS", "0") or 0)
+    share = max(1, min(aff, omp) if omp > 0 else aff)
+    return {"model": model, "nproc": os.cpu_count(), "affinity_cpus": aff, "omp_num_threads": omp or None,
+            "threads_used": share}
+
+
+def cpu_sample(o, p, cells, seconds: float, threads: int, seed: int = 123) -> dict:
+    """The oracle (FP32, reference semantics) on a random sample of the
+    listed cells, sized to about `seconds` of work at `threads` threads:
+    rays/s, ray-steps/s and the wall time of the sample."""
+    rng = np.random.default_rng(seed)
     order = rng.permutation(len(cells))
-    n = 16 * ncores
+    n = min(len(cells), max(4, 4 * threads))
     t0 = time.perf_counter()
-    o.solve(cells[order[:n]], nthreads=ncores)
+    o.solve(cells[order[:n]], nthreads=threads)
     dt = time.perf_counter() - t0
     n = int(min(len(cells), max(n, n * seconds / max(dt, 1e-3))))
-    sample = cells[order[:n]]
     t0 = time.perf_counter()
-    _, _, _, steps = o.solve(sample, nthreads=ncores)
+    _, _, _, steps = o.solve(cells[order[:n]], nthreads=threads)
     dt = time.perf_counter() - t0
     rays = n * p.photmax * p.nVar
-    return {"value": rays / dt, "unit": "rays/s", "cores": ncores, "kind": "port",
-            "sample": f"{n} of {len(cells)} cells (random), all {p.photmax} rays x {p.nVar} runs each: "
-                      f"{rays} rays, {int(steps.sum())} steps in {dt:.1f} s",
-            "steps_per_s": float(steps.sum() / dt)}
+    return {"value": rays / dt, "unit": "rays/s", "steps_per_s": float(steps.sum() / dt), "cores": threads,
+            "wall_s": dt, "sample": f"{n} of {len(cells)} cells (random), all {p.photmax} rays x {p.nVar} runs "
+                                    f"each: {rays} rays, {int(steps.sum())} steps"}
+
+
+def cpu_baseline(p, tab, T, seconds: float, host: dict) -> dict:
+    """The C++ oracle on the host: the CPU share and one core, each on a
+    bounded sample of the same workload."""
+    from oracle import oracle as O
+    o = O.Oracle(p, tab, T)
+    cells = O.all_cells(p)
+    share = cpu_sample(o, p, cells, seconds, host["threads_used"])
+    one = cpu_sample(o, p, cells, seconds, 1)
+    o.close()
+    out = dict(share)
+    out.update({"kind": "port", "host": host, "one_core": one,
+                "scaling_vs_one_core": share["value"] / one["value"]})
+    return out
+
+
+def cpu_configs(tab, seconds: float, host: dict) -> dict:
+    """The oracle on the grey 32^3 and narrow-band 64^3 fields (the CPU
+    share, bounded samples), beside the headline field's baseline."""
+    from oracle import oracle as O
+    from radchan_amd import config
+    from radchan_amd import fields as F
+    out = {}
+    for name in ("grey32", "nb64"):
+        p = config(name)
+        p.nVar = 8
+        if p.grey:
+            T = F.channel_field(p, 500.0, 1000.0, noise=0.02, seed=0)
+            o = O.Oracle(p, None, T)
+        else:
+            T = F.channel_field(p, 300.0, 1500.0, noise=0.02, seed=0)
+            o = O.Oracle(p, tab, T)
+        out[name] = cpu_sample(o, p, O.all_cells(p), seconds, host["threads_used"])
+        o.close()
+    return out
+
+
+def lane_use(st) -> dict:
+    """Lane use of the batched march: the fraction of the bulk march's lane
+    slots (64 per wavefront sub-step) that stepped a live ray, and the
+    fraction of all steps taken in the drain (after a wavefront has no unit
+    left to start)."""
+    slots, bulk = st.get("bulk_slots", 0), st.get("bulk_steps", 0)
+    return {"bulk_active_lane_frac": bulk / slots if slots else None,
+            "drain_step_frac": 1.0 - bulk / st["steps"] if st["steps"] else None}
 
 
 def timed_leg(s, lib, variant: int, steps: int, warmup: int):
@@ -140,36 +217,113 @@ def eps_scaling(p, tab, T, sched, eps, bf16: bool):
             "march_ms": st["march_ms"]}
 
 
-def target_leg(lib, tab):
-    """The 256 x 128 x 256 narrow-band field: batched (autotuned schedule,
-    FP32 k tables, hipGraph) against the plain solver, rays/s and time to
-    eps = 1%."""
-    from radchan_amd import VARIANT_BATCHED, VARIANT_PLAIN, Solver, config
-    from radchan_amd import fields as F
-    p = config("big256")
-    p.nVar = 8
-    T = F.channel_field(p, 300.0, 1500.0, noise=0.02, seed=0)
+def field_legs(args, lib, p, tab, T, bf16: bool, steps: int, warmup: int, plain_steps: int, scaling: bool = True):
+    """Every leg of one field: autotune, the batched solver with FP32 and
+    BF16 k-tables (hipGraph replays, `steps` timed), the march kernel alone
+    (HIP events on the solver's stream, no graph) for the roofline, the
+    plain solver, and (scaling) one solve at 4x photmax."""
+    from radchan_amd import VARIANT_BATCHED, VARIANT_PLAIN
     s = new_session(p, tab, [T])
-    sched = s.autotune()
-    s.set_graph(True)
-    wall, st, _ = timed_leg(s, lib, VARIANT_BATCHED, 2, 1)
-    eps, _ = std_error(s.tallies())
+    sched = None
+    if not args.no_autotune:
+        sched = s.autotune()                       # untimed: picks wavefronts per CU and refill threshold
+        log(f"autotune {p.imax}x{p.jmax}x{p.kmax}: {sched}")
+    legs = {}
+    for name, use_bf16 in (("fp32_tables", False), ("bf16_tables", True)):
+        if use_bf16 and p.grey:
+            continue
+        s.set_table_precision(use_bf16)
+        s.set_graph(not args.no_graph)
+        wall, st, _ = timed_leg(s, lib, VARIANT_BATCHED, steps, warmup)
+        eps, p95 = std_error(s.tallies())
+        legs[name] = {"rays_per_s": st["rays"] / wall, "ms_per_step": 1e3 * wall, "eps": eps, "eps_p95_cell": p95,
+                      "time_to_1pct_s": wall * (eps / 0.01) ** 2, "rays": st["rays"], "steps": st["steps"],
+                      "timed_solves": steps, **lane_use(st)}
+        log(f"batched {name}: {1e3 * wall:.2f} ms/step, {st['rays'] / wall / 1e9:.3f} Grays/s, "
+            f"{st['steps'] / st['rays']:.2f} steps/ray, eps {eps:.4g}, bulk active lanes "
+            f"{legs[name]['bulk_active_lane_frac']:.3f}, drain steps {legs[name]['drain_step_frac']:.4f}")
+    head = legs["bf16_tables" if bf16 else "fp32_tables"]
+
+    # the march kernel alone, outside a graph: HIP events on the solver's stream
+    s.set_table_precision(bf16)
     s.set_graph(False)
-    pwall, sp, _ = timed_leg(s, lib, VARIANT_PLAIN, 1, 1)
-    eps_p, _ = std_error(s.tallies())
+    _, st, march = timed_leg(s, lib, VARIANT_BATCHED, max(3, steps), 1)
+    rays, nsteps = st["rays"], st["steps"]
+
+    plain = None
+    if not args.no_plain and plain_steps > 0:
+        s.set_table_precision(False)
+        pwall, sp, _ = timed_leg(s, lib, VARIANT_PLAIN, plain_steps, 1)
+        eps_p, _ = std_error(s.tallies())
+        plain = {"rays_per_s": sp["rays"] / pwall, "ms_per_step": 1e3 * pwall, "eps": eps_p,
+                 "time_to_1pct_s": pwall * (eps_p / 0.01) ** 2, "timed_solves": plain_steps}
+        log(f"plain:   {1e3 * pwall:.2f} ms/step, {sp['rays'] / pwall / 1e9:.3f} Grays/s, eps {eps_p:.4g}")
+    ncell = s.ncell
     s.close()
-    t1 = wall * (eps / 0.01) ** 2
-    tp = pwall * (eps_p / 0.01) ** 2
-    log(f"target 256x128x256: batched {1e3 * wall:.1f} ms, plain {1e3 * pwall:.1f} ms, "
-        f"time-to-1% speed-up {tp / t1:.2f}x")
-    return {"workload": f"narrow-band correlated-k OERM {p.imax}x{p.jmax}x{p.kmax}, photmax {p.photmax}, "
-                        f"nVar {p.nVar}, FP32 k tables",
-            "schedule": sched, "rays": int(st["rays"]), "steps": int(st["steps"]),
-            "batched": {"ms_per_step": 1e3 * wall, "rays_per_s": st["rays"] / wall, "eps": eps,
-                        "time_to_1pct_s": t1},
-            "plain": {"ms_per_step": 1e3 * pwall, "rays_per_s": sp["rays"] / pwall, "eps": eps_p,
-                      "time_to_1pct_s": tp},
-            "time_to_1pct_speedup_vs_plain": tp / t1}
+
+    b_step, b_ray = (B_STEP_BF16, B_RAY_BF16) if bf16 else (B_STEP, B_RAY)
+    alg_bytes = nsteps * b_step + rays * b_ray + ncell * B_CELL
+    march_ms = float(np.mean(march))
+    achieved = alg_bytes / (march_ms * 1e-3) / 1e9
+    wl = {(64, 64, 64): "nb64", (256, 128, 256): "big256", (192, 128, 192): "mg192"}.get((p.imax, p.jmax, p.kmax))
+    traffic = None
+    tpath = os.path.join(ROOT, "profiles", f"traffic_{wl}_{'bf16' if bf16 else 'fp32'}.json")
+    if wl and os.path.exists(tpath):
+        with open(tpath) as f:
+            traffic = json.load(f).get("hbm_bytes_per_launch")
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
+            "kernel": "%s<false,%d> (persistent march, %s k-tables)" % (
+                "k_batched_mg" if p.grid_num > 1 else "k_batched", 1 if bf16 else 0, "BF16" if bf16 else "FP32"),
+            "kernel_ms": round(march_ms, 4), "kernel_ms_each": [round(float(m), 4) for m in march],
+            "algorithmic_bytes": int(alg_bytes), "steps_per_launch": int(nsteps), "rays_per_launch": int(rays),
+            "bytes_per_step": b_step, "bytes_per_ray": b_ray, "bytes_per_cell": B_CELL}
+    out = {"workload": f"narrow-band correlated-k OERM {p.imax}x{p.jmax}x{p.kmax}, {p.nB} bands x {p.nQ} g x "
+                       f"{p.nT} T, photmax {p.photmax}, nVar {p.nVar}, srt {p.srt}, grid_num {p.grid_num}"
+                       if not p.grey else f"grey OERM {p.imax}x{p.jmax}x{p.kmax}, photmax {p.photmax}, nVar {p.nVar}",
+           "cells": ncell, "schedule": sched, "head": head, "batched": legs, "plain": plain, "roofline": roof,
+           "speedup_vs_plain": (head["rays_per_s"] / plain["rays_per_s"]) if plain else None,
+           "time_to_1pct_speedup_vs_plain": (plain["time_to_1pct_s"] / head["time_to_1pct_s"]) if plain else None}
+    if scaling:
+        sc = eps_scaling(p, tab, T, sched, head["eps"], bf16)
+        log(f"eps at photmax {sc['photmax']}: {sc['eps']:.4g} (ratio {sc['eps_ratio']:.3f}, expected 0.5), "
+            f"{sc['march_ms']:.1f} ms")
+        out["eps_scaling"] = sc
+    return out
+
+
+def confirm_leg(lib, p, tab, T, sched, eps: float, ms: float):
+    """The projected time to eps = 1% measured: one batched launch of as
+    many independent runs per cell as the projection from `eps` (nVar
+    runs per solve) asks, plus 10% (the eps of one solve is itself an
+    estimate); eps of the result from the per-cell means and the
+    reference's sqrt-variance (the standard error of the mean is
+    sdev sqrt((n - 1/2) / n))."""
+    import copy
+    import math
+    from radchan_amd import VARIANT_BATCHED
+    q = copy.deepcopy(p)
+    q.nVar = int(math.ceil(p.nVar * (eps / 0.01) ** 2 * 1.1))
+    s = new_session(q, tab, [T])
+    if sched:
+        s.set_schedule(sched["wg_per_cu"], sched["threshold"], sched["steps_per_iter"])
+    s.set_graph(False)
+    lib.rc_device_synchronize()
+    t0 = time.perf_counter()
+    st = s.solve(VARIANT_BATCHED)
+    lib.rc_device_synchronize()
+    wall = time.perf_counter() - t0
+    mean, sd = s.results()
+    s.close()
+    n = q.nVar
+    m = mean.astype(np.float64).reshape(-1)
+    se = sd.astype(np.float64).reshape(-1) * math.sqrt((n - 0.5) / n)
+    eps_m = float(np.sqrt((se ** 2).sum()) / np.sqrt((m ** 2).sum()))
+    proj = ms * 1e-3 * (eps / 0.01) ** 2
+    log(f"time to 1%: {q.nVar} runs per cell in one launch, {wall:.2f} s, eps {eps_m:.5f} (projected {proj:.2f} s)")
+    return {"runs_per_cell": n, "rays": int(st["rays"]), "steps": int(st["steps"]), "wall_s": wall,
+            "kernel_s": st["march_ms"] * 1e-3, "eps": eps_m, "projected_s": proj,
+            "measured_over_projected": wall / proj}
 
 
 def ref_generator_time(d):
@@ -350,14 +504,15 @@ def bench_c4(args) -> int:
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
-    ap.add_argument("--steps", type=int, default=10)
-    ap.add_argument("--warmup", type=int, default=2)
-    ap.add_argument("--workload", default="nb64", choices=["nb64", "nb128", "big256", "grey32", "mg192", "c4",
-                                                               "ckdist"])
+    ap.add_argument("--steps", type=int, default=5)
+    ap.add_argument("--warmup", type=int, default=1)
+    ap.add_argument("--workload", default="big256", choices=["nb64", "nb128", "big256", "grey32", "mg192", "c4",
+                                                                 "ckdist"])
     ap.add_argument("--nvar", type=int, default=8)
     ap.add_argument("--tables", default="fp32", choices=["bf16", "fp32"],
                     help="k-table storage of the headline (batched) leg; both are measured")
-    ap.add_argument("--plain-steps", type=int, default=3)
+    ap.add_argument("--plain-steps", type=int, default=None,
+                    help="timed solves of the plain solver (default 2 on fields of 4 M cells or more, else 3)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -367,8 +522,10 @@ def main() -> int:
     ap.add_argument("--restriction", default="texture", choices=["texture", "spline"],
                     help="coarse-level temperatures of multigrid workloads: texture mean (as the reference) or "
                          "cubic splines (interp3D)")
-    ap.add_argument("--no-target", action="store_true",
-                    help="skip the 256x128x256 batched-vs-plain leg")
+    ap.add_argument("--no-secondary", action="store_true",
+                    help="skip the 64^3 legs and the measured time to 1%% beside the headline")
+    ap.add_argument("--no-confirm", action="store_true",
+                    help="--workload nb64: skip the measured run at the projected budget for eps = 1%%")
     ap.add_argument("--probe", action="store_true",
                     help="no timing: one elementwise copy of 512 MiB (counter calibration) and one batched "
                          "solve, for hardware-counter runs")
@@ -394,28 +551,29 @@ def main() -> int:
                           "ckdist": ck, "cpu_baseline": ck["cpu_baseline"]}), flush=True)
         return 0
 
-    from radchan_amd import VARIANT_BATCHED, VARIANT_PLAIN, Solver, config
+    from radchan_amd import VARIANT_BATCHED, config
     from radchan_amd import fields as F
     from radchan_amd import lib as L
     from radchan_amd.tables import synthetic_nb
 
-    p = config(args.workload)
-    p.nVar = args.nvar
-    tab = None
-    if p.grey:
-        T = F.channel_field(p, 500.0, 1000.0, noise=0.02, seed=0)
-    else:
-        tab = synthetic_nb()
-        T = F.channel_field(p, 300.0, 1500.0, noise=0.02, seed=0)
-    log(f"bench: workload {args.workload} {p.imax}x{p.jmax}x{p.kmax} photmax {p.photmax} nVar {p.nVar}")
-    s = new_session(p, tab, [T])
+    def field_of(name):
+        q = config(name)
+        q.nVar = args.nvar
+        if q.grey:
+            return q, None, F.channel_field(q, 500.0, 1000.0, noise=0.02, seed=0)
+        return q, tab, F.channel_field(q, 300.0, 1500.0, noise=0.02, seed=0)
+
+    tab = synthetic_nb()
+    p, ptab, T = field_of(args.workload)
     lib = L.load()
     bf16 = args.tables == "bf16" and not p.grey
+    log(f"bench: workload {args.workload} {p.imax}x{p.jmax}x{p.kmax} photmax {p.photmax} nVar {p.nVar}")
 
     if args.probe:
         # a device-to-device copy of 512 MiB through the HIP runtime (a blit
         # kernel: reads and writes 512 MiB), then one batched solve
         import ctypes
+        s = new_session(p, ptab, [T])
         hip = ctypes.CDLL("libamdhip64.so")
         n = 512 << 20
         a, b = ctypes.c_void_p(), ctypes.c_void_p()
@@ -433,72 +591,36 @@ def main() -> int:
         s.close()
         return 0
 
-    sched = None
-    if not args.no_autotune:
-        sched = s.autotune()                       # untimed: picks wavefronts per CU and refill threshold
-        log(f"autotune: {sched}")
+    steps = args.steps
+    big = p.imax * p.jmax * p.kmax >= 4 << 20
+    main_leg = field_legs(args, lib, p, ptab, T, bf16, steps, args.warmup,
+                          args.plain_steps if args.plain_steps is not None else (2 if big else 3))
+    head = main_leg["head"]
 
-    legs = {}
-    for name, use_bf16 in (("fp32_tables", False), ("bf16_tables", True)):
-        if use_bf16 and p.grey:
-            continue
-        s.set_table_precision(use_bf16)
-        s.set_graph(not args.no_graph)
-        wall, st, _ = timed_leg(s, lib, VARIANT_BATCHED, args.steps, args.warmup)
-        eps, p95 = std_error(s.tallies())
-        legs[name] = {"rays_per_s": st["rays"] / wall, "ms_per_step": 1e3 * wall, "eps": eps, "eps_p95_cell": p95,
-                      "time_to_1pct_s": wall * (eps / 0.01) ** 2, "rays": st["rays"], "steps": st["steps"]}
-        log(f"batched {name}: {1e3 * wall:.2f} ms/step, {st['rays'] / wall / 1e9:.3f} Grays/s, "
-            f"{st['steps'] / st['rays']:.2f} steps/ray, eps {eps:.4g}")
-    head = legs["bf16_tables" if bf16 else "fp32_tables"]
-
-    # the march kernel alone, outside a graph: HIP events on the solver's stream
-    s.set_table_precision(bf16)
-    s.set_graph(False)
-    _, st, march = timed_leg(s, lib, VARIANT_BATCHED, max(3, args.steps // 2), 1)
-    rays, steps = st["rays"], st["steps"]
-
-    plain = None
-    if not args.no_plain and args.plain_steps > 0:
-        s.set_table_precision(False)
-        pwall, sp, _ = timed_leg(s, lib, VARIANT_PLAIN, args.plain_steps, 1)
-        eps_p, _ = std_error(s.tallies())
-        plain = {"rays_per_s": sp["rays"] / pwall, "ms_per_step": 1e3 * pwall, "eps": eps_p,
-                 "time_to_1pct_s": pwall * (eps_p / 0.01) ** 2}
-        log(f"plain:   {1e3 * pwall:.2f} ms/step, {sp['rays'] / pwall / 1e9:.3f} Grays/s, eps {eps_p:.4g}")
-
-    ncell = s.ncell
-    b_step, b_ray = (B_STEP_BF16, B_RAY_BF16) if bf16 else (B_STEP, B_RAY)
-    alg_bytes = steps * b_step + rays * b_ray + ncell * B_CELL
-    march_ms = float(np.mean(march))
-    achieved = alg_bytes / (march_ms * 1e-3) / 1e9
-    traffic = None
-    tpath = os.path.join(ROOT, "profiles", f"traffic_{args.workload}_{args.tables}.json")
-    if os.path.exists(tpath):
-        with open(tpath) as f:
-            traffic = json.load(f).get("hbm_bytes_per_launch")
-    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "%s<false,%d> (persistent march, %s k-tables)" % (
-                "k_batched_mg" if p.grid_num > 1 else "k_batched", 1 if bf16 else 0, "BF16" if bf16 else "FP32"),
-            "kernel_ms": round(march_ms, 4),
-            "algorithmic_bytes": int(alg_bytes),
-            "bytes_per_step": b_step, "bytes_per_ray": b_ray, "bytes_per_cell": B_CELL}
-
-    scal = eps_scaling(p, tab, T, sched, head["eps"], bf16)
-    log(f"eps at photmax {scal['photmax']}: {scal['eps']:.4g} (ratio {scal['eps_ratio']:.3f}, expected 0.5), "
-        f"{scal['march_ms']:.1f} ms")
-    target = None
-    if not args.no_target and args.workload != "big256" and not p.grey:
-        target = target_leg(lib, tab)
+    # the 64^3 field beside the headline, with the measured time to 1%
+    second = None
+    if not args.no_secondary and args.workload != "nb64" and not p.grey:
+        q, qtab, qT = field_of("nb64")
+        log("bench: 64x64x64 photmax 200 beside the headline")
+        second = field_legs(args, lib, q, qtab, qT, bf16, 10, 2, 3, scaling=False)
+        nb = second["head"]
+        second["time_to_1pct_measured"] = confirm_leg(lib, q, qtab, qT, second["schedule"], nb["eps"],
+                                                      nb["ms_per_step"])
+    elif args.workload == "nb64" and not args.no_confirm:
+        main_leg["time_to_1pct_measured"] = confirm_leg(lib, p, ptab, T, main_leg["schedule"], head["eps"],
+                                                        head["ms_per_step"])
 
     ck = None if args.no_ckdist else ckdist_leg(not args.no_cpu)
 
     cpu = None
+    cpu_other = None
     if not args.no_cpu:
         try:
-            cpu = cpu_baseline(p, tab, T, args.cpu_seconds)
-            log(f"cpu:     {cpu['value'] / 1e6:.2f} Mrays/s on {cpu['cores']} cores")
+            host = host_cpu()
+            cpu = cpu_baseline(p, ptab, T, args.cpu_seconds, host)
+            log(f"cpu:     {cpu['value'] / 1e6:.2f} Mrays/s on {cpu['cores']} threads ({host['model']}, nproc "
+                f"{host['nproc']}), {cpu['one_core']['value'] / 1e6:.3f} Mrays/s on one core")
+            cpu_other = cpu_configs(tab, args.cpu_seconds / 3, host)
         except Exception as e:  # the baseline is reported, never required
             log(f"cpu baseline unavailable: {e}")
 
@@ -508,7 +630,7 @@ def main() -> int:
         "value": value,
         "unit": "rays/s",
         "n_gpus": 1,
-        "steps": args.steps,
+        "steps": steps,
         "warmup": args.warmup,
         "ms_per_step": head["ms_per_step"],
         "higher_is_better": True,
@@ -516,27 +638,25 @@ def main() -> int:
         "vs_baseline": None,
         "dtype": "fp32 march, bf16 k-tables, fp64 tallies" if bf16 else "fp32 march, fp32 k-tables, fp64 tallies",
         "data": "synthetic (H2O-like tables, parabolic channel T field with 2% smoothed noise)",
-        "config": {"workload": f"narrow-band correlated-k OERM {p.imax}x{p.jmax}x{p.kmax}, {p.nB} bands x "
-                               f"{p.nQ} g x {p.nT} T, photmax {p.photmax}, nVar {p.nVar}, srt {p.srt}, "
-                               f"grid_num {p.grid_num}" if not p.grey else
-                               f"grey OERM {p.imax}x{p.jmax}x{p.kmax}, photmax {p.photmax}, nVar {p.nVar}",
-                   "cells": ncell, "rays_per_step": int(rays), "steps_per_ray": steps / rays,
-                   "schedule": sched, "graph": not args.no_graph, "restriction": args.restriction},
+        "config": {"workload": main_leg["workload"], "cells": main_leg["cells"],
+                   "rays_per_step": int(head["rays"]), "steps_per_ray": head["steps"] / head["rays"],
+                   "schedule": main_leg["schedule"], "graph": not args.no_graph, "restriction": args.restriction},
         "eps": head["eps"],
         "eps_p95_cell": head["eps_p95_cell"],
         "time_to_1pct_s": head["time_to_1pct_s"],
-        "batched": legs,
-        "plain": plain,
-        "speedup_vs_plain": (value / plain["rays_per_s"]) if plain else None,
-        "time_to_1pct_speedup_vs_plain": (plain["time_to_1pct_s"] / head["time_to_1pct_s"]) if plain else None,
-        "eps_scaling": scal,
-        "target_256x128x256": target,
+        "batched": main_leg["batched"],
+        "plain": main_leg["plain"],
+        "speedup_vs_plain": main_leg["speedup_vs_plain"],
+        "time_to_1pct_speedup_vs_plain": main_leg["time_to_1pct_speedup_vs_plain"],
+        "eps_scaling": main_leg.get("eps_scaling"),
+        "time_to_1pct_measured": main_leg.get("time_to_1pct_measured"),
+        "nb64": second,
         "ckdist": ck,
-        "roofline": roof,
+        "roofline": main_leg["roofline"],
         "cpu_baseline": cpu,
+        "cpu_baseline_other_configs": cpu_other,
     }
     print(json.dumps(out), flush=True)
-    s.close()
     return 0
 
 

@@ -103,6 +103,12 @@ typedef struct rc_stats {
     int64_t  capped;            /* rays that hit the last level's step cap    */
     int32_t  launches;          /* kernel launches in the last run            */
     int32_t  variant;
+    /* batched variant: lane slots of the bulk march (64 per sub-step of a */
+    /* wavefront, live ray or not) and the steps of live rays among them;  */
+    /* their ratio is the active-lane fraction of the bulk march, and      */
+    /* steps - bulk_steps are the steps of the drain                       */
+    int64_t  bulk_slots;
+    int64_t  bulk_steps;
 } rc_stats;
 
 /* ---- 1. reference entry points --------------------------------------- */

@@ -384,6 +384,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
     kc.w = 0u;
     kc.a = 0.f;
     int nsteps = 0, nrays = 0;
+    int bsub = 0;                 /* bulk march sub-steps of the wavefront (every lane steps) */
 
     for (;;) {
         /* (a) once the current unit is fully generated, take the next unit
@@ -581,6 +582,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                  * right after the step that ends it.  Idle lanes step too, on
                  * stale state that the next pick-up overwrites; the step
                  * bounds its loads. */
+                bsub += spi;
 #pragma unroll 1
                 for (int q = 0; q < spi; ++q) {
                     const DevLevel L = MULTI ? llev[lev] : l0;
@@ -621,6 +623,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
      * march; below that, the wavefront marches the remaining
      * rays one at a time, all lanes on one ray.  Then the
      * wavefront is done. */
+    const int nbulk = nsteps;     /* steps of the bulk march, before the drain */
     if (__ballot(active) != 0ull) {
         const SnapView sv = snap_view(snaps + snap);
         DevLevel l0;
@@ -697,13 +700,20 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
         if (t.done == P) de[t.tid] = t.acc;
         else atomicAdd(&de[t.tid], t.acc);
     }
+    /* per-lane counts stay 32-bit in the loop (a lane marches far fewer
+     * than 2^32 steps: 1e13 steps over 16 k wavefronts is 1e7 per lane);
+     * the wavefront's sums are 64-bit */
+    unsigned long long ws = (uint32_t)nsteps, wr = (uint32_t)nrays, wb = (uint32_t)nbulk;
     for (int o = 32; o > 0; o >>= 1) {
-        nsteps += __shfl_xor(nsteps, o);
-        nrays += __shfl_xor(nrays, o);
+        ws += __shfl_xor(ws, o);
+        wr += __shfl_xor(wr, o);
+        wb += __shfl_xor(wb, o);
     }
     if (lane == 0) {
-        atomicAdd(&stats[0], (unsigned long long)(uint32_t)nrays);
-        atomicAdd(&stats[1], (unsigned long long)(uint32_t)nsteps);
+        atomicAdd(&stats[0], wr);
+        atomicAdd(&stats[1], ws);
+        atomicAdd(&stats[3], 64ull * (uint32_t)bsub);
+        atomicAdd(&stats[4], wb);
     }
 }
 

@@ -857,6 +857,8 @@ int rc_sync(rc_ctx *c, rc_stats *st) {
         c->last.rays = (int64_t)h[0];
         c->last.steps = (int64_t)h[1];
         c->last.capped = (int64_t)h[2];
+        c->last.bulk_slots = (int64_t)h[3];
+        c->last.bulk_steps = (int64_t)h[4];
         c->launched = false;
     }
     if (st) *st = c->last;
@@ -1133,6 +1135,8 @@ int rc_group_sync(rc_group *g, rc_stats *st) {
         g->last.rays = (int64_t)h[0];
         g->last.steps = (int64_t)h[1];
         g->last.capped = (int64_t)h[2];
+        g->last.bulk_slots = (int64_t)h[3];
+        g->last.bulk_steps = (int64_t)h[4];
         g->launched = false;
     }
     if (st) *st = g->last;

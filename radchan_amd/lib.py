@@ -86,6 +86,7 @@ class Stats(C.Structure):
     _fields_ = [
         ("kernel_ms", C.c_double), ("march_ms", C.c_double), ("rays", C.c_int64), ("steps", C.c_int64),
         ("capped", C.c_int64), ("launches", C.c_int32), ("variant", C.c_int32),
+        ("bulk_slots", C.c_int64), ("bulk_steps", C.c_int64),
     ]
 
 
