@@ -12,8 +12,9 @@
  *
  * The main loop generates rays in bulk (all 64 lanes set up one ray each),
  * refills idle lanes from the queue once enough of them are idle, and
- * advances every lane by a few steps (march_core_cached, the loop body of
- * march_ray).  Rays keep their (sorted) order of dispatch, so neighbouring
+ * advances every lane by a few steps (march_core_bulk, the loop body of
+ * march_ray; a ray that meets a black wall takes the wall's emission on
+ * its next step, through that step's own table lookup).  Rays keep their (sorted) order of dispatch, so neighbouring
  * lanes march neighbouring bands.  Each finished ray adds its FP32 tally to
  * its unit's FP64 total in an LDS slot; a slot's total goes to HBM when the
  * slot is reused.  When no unit is left the wavefront drains its last rays
@@ -156,7 +157,10 @@ __device__ __forceinline__ SnapView snap_view(const DevSnap *snp) {
     v.kb = uni(snp->kb);
     v.kp16 = uni(snp->kp16);
     v.bp = uni(snp->bp);
-    v.bfix = uni(snp->bfix);
+    v.xu_all = uni(snp->xu_all);
+    v.P_all = uni(snp->P_all);
+    v.nxu = uni(snp->nxu);
+    v.npos = uni(snp->npos);
     return v;
 }
 
@@ -164,8 +168,9 @@ __device__ __forceinline__ DevLevel level_view(const DevLevel *lp) {
     DevLevel L;
     L.im = uni(lp->im); L.jm = uni(lp->jm); L.km = uni(lp->km); L.maxs = uni(lp->maxs);
     L.dy = uni(lp->dy); L.dz = uni(lp->dz);
-    L.ncell = uni(lp->ncell); L.pad1 = 0;
-    L.xu = uni(lp->xu); L.T = uni(lp->T); L.Tw = uni(lp->Tw); L.Tz = uni(lp->Tz);
+    L.ncell = uni(lp->ncell); L.xoff = uni(lp->xoff); L.poff = uni(lp->poff); L.pad1 = 0;
+    L.xu = uni(lp->xu); L.P = uni(lp->P);
+    for (int d = 0; d < 3; ++d) { L.cr[d] = uni(lp->cr[d]); L.cmul[d] = uni(lp->cmul[d]); }
     return L;
 }
 
@@ -383,8 +388,15 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
     kc.q = make_float4(0.f, 0.f, 0.f, 0.f);
     kc.w = 0u;
     kc.a = 0.f;
+    int wall = 0;                 /* the ray waits at a black wall, entry widx of the level's positions */
+    uint32_t widx = 0u;
+    LaneLevel ll{};               /* multigrid: the level of the lane's ray */
+    CellWalk cw;                  /* the ray's linear cell index and its steps along y and z */
+    cw.cell = 0u;
+    cw.dcy = cw.dcz = 0;
     int nsteps = 0, nrays = 0;
     int bsub = 0;                 /* bulk march sub-steps of the wavefront (every lane steps) */
+    int64_t bsteps = 0;           /* steps of live rays in the bulk march (wavefront total) */
 
     for (;;) {
         /* (a) once the current unit is fully generated, take the next unit
@@ -564,6 +576,13 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                     lev = 0;
                     counter = 0;
                     kt = -1;
+                    wall = 0;
+                    if (MULTI) {
+                        ll = lane_level(llev[0]);
+                        cell_walk_setup(ll, r, cw);
+                    } else {
+                        cell_walk_setup(level_view(&snaps[snap].lev[0]), r, cw);
+                    }
                 }
                 qhead = qwrap(qhead + npop);
                 qcount -= npop;
@@ -573,43 +592,58 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
             if (__ballot(active) != 0ull) {
                 /* (e) march steps of every active lane; the snapshot's
                  * fields (and the level, single grid) are wavefront-uniform
-                 * values, a lane's level comes from LDS (multigrid) */
+                 * values, a lane's level is held in its registers
+                 * (multigrid: ll, set at pick-up and at each coarsening) */
                 const SnapView sv = snap_view(snaps + snap);
                 DevLevel l0;
                 if (!MULTI) l0 = level_view(&snaps[snap].lev[0]);
+                const float *xb = MULTI ? sv.xu_all : l0.xu;
+                const float *pb = MULTI ? sv.P_all : l0.P;
+                const uint32_t xmax = MULTI ? (uint32_t)(sv.nxu - 1) : (uint32_t)l0.im;
+                const uint32_t pmax = MULTI ? (uint32_t)(sv.npos - 1)
+                                            : (uint32_t)(level_pos_count(l0.im, l0.jm, l0.km) - 1);
                 /* spi march steps.  Every active lane holds a live ray under
                  * its level's step cap: a ray is finished (or coarsened)
                  * right after the step that ends it.  Idle lanes step too, on
                  * stale state that the next pick-up overwrites; the step
                  * bounds its loads. */
                 bsub += spi;
-#pragma unroll 1
-                for (int q = 0; q < spi; ++q) {
-                    const DevLevel L = MULTI ? llev[lev] : l0;
-                    int status = 0;
-                    const bool out = march_core_cached<TM>(cm, sv, L, r, e, tol, dacc, status, kt, kc);
-                    if (active && out) boundary<TM>(cm, sv, L, r, e, dacc, status);
-                    counter += 1;
-                    nsteps += active ? 1 : 0;
+                auto substep = [&](const auto &L, uint32_t xo, uint32_t po) {
+                    /* a wall step is no march step of the reference (its
+                     * tally belongs to the step that crossed the face): it
+                     * neither counts nor meets the step cap */
+                    const bool ws = wall != 0;
+                    const bool out = march_core_bulk<TM>(cm, sv, L, xb, xo, xmax, pb, po, pmax, r, e, tol, dacc, kt,
+                                                         kc, ws, widx, cw);
+                    boundary_defer(cm, L, r, active && out, wall, widx, cw);
+                    counter += ws ? 0 : 1;
+                    bsteps += __popcll(__ballot(active && !ws));
                     const bool alive = r.tra > tol;
-                    const bool done = active && (!alive || counter >= L.maxs);
+                    const bool done = active && (!alive || (counter >= L.maxs && wall == 0));
                     const bool fin = done && (!MULTI || !alive || lev == cm.grid_num - 1);
                     const bool crs = MULTI && done && !fin;
                     if (fin) {
                         finish_ray(sl, seq, runit, rtid, dacc, de);
                         nrays += 1;
                         active = false;
+                        wall = 0;
                     }
                     if (crs) {
                         /* coarsen the indices for the next grid level (:215-223) */
-                        const DevLevel Nl = llev[lev + 1];
-                        const int ri = L.im / Nl.im, rj = L.jm / Nl.jm, rk = L.km / Nl.km;
-                        r.ic = (r.ic + ri - 1) / ri;
-                        r.jc = (r.jc + rj - 1) / rj;
-                        r.kc = (r.kc + rk - 1) / rk;
+                        const DevLevel &Lc = llev[lev];
+                        r.ic = coarsen(r.ic, Lc.cr[0], Lc.cmul[0]);
+                        r.jc = coarsen(r.jc, Lc.cr[1], Lc.cmul[1]);
+                        r.kc = coarsen(r.kc, Lc.cr[2], Lc.cmul[2]);
+                        ll = lane_level(llev[lev + 1]);
+                        cell_walk_setup(ll, r, cw);
                         lev += 1;
                         counter = 0;
                     }
+                };
+#pragma unroll 1
+                for (int q = 0; q < spi; ++q) {
+                    if (MULTI) substep(ll, ll.xoff, ll.poff);
+                    else substep(l0, 0u, 0u);
                 }
             }
         }
@@ -623,7 +657,19 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
      * march; below that, the wavefront marches the remaining
      * rays one at a time, all lanes on one ray.  Then the
      * wavefront is done. */
-    const int nbulk = nsteps;     /* steps of the bulk march, before the drain */
+
+    if (active && wall) {
+        /* a ray still waiting at its wall: the wall's emission, and the ray is done */
+        const SnapView sv = snap_view(snaps + snap);
+        const DevLevel *lp = MULTI ? &llev[lev] : &snaps[snap].lev[0];
+        float Kw, Bw;
+        kb_pos<TM>(sv, e.row, e.rank, cm.nT, gld(lp->P, widx), Kw, Bw);
+        wall_hit(e, r, Bw, dacc);
+        finish_ray(sl, seq, runit, rtid, dacc, de);
+        nrays += 1;
+        active = false;
+        wall = 0;
+    }
     if (__ballot(active) != 0ull) {
         const SnapView sv = snap_view(snaps + snap);
         DevLevel l0;
@@ -648,11 +694,9 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                     march_ahead<TM, true>(cm, sv, L, r, e, tol, dacc, counter, nsteps, xul + xoff, L.maxs);
                     if (!(r.tra > tol) || lev == cm.grid_num - 1) break;
                     /* coarsen the indices for the next grid level */
-                    const DevLevel Nl = llev[lev + 1];
-                    const int ri = L.im / Nl.im, rj = L.jm / Nl.jm, rk = L.km / Nl.km;
-                    r.ic = (r.ic + ri - 1) / ri;
-                    r.jc = (r.jc + rj - 1) / rj;
-                    r.kc = (r.kc + rk - 1) / rk;
+                    r.ic = coarsen(r.ic, L.cr[0], L.cmul[0]);
+                    r.jc = coarsen(r.jc, L.cr[1], L.cmul[1]);
+                    r.kc = coarsen(r.kc, L.cr[2], L.cmul[2]);
                     xoff += L.im + 1;
                     lev += 1;
                     counter = 0;
@@ -700,20 +744,19 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
         if (t.done == P) de[t.tid] = t.acc;
         else atomicAdd(&de[t.tid], t.acc);
     }
-    /* per-lane counts stay 32-bit in the loop (a lane marches far fewer
-     * than 2^32 steps: 1e13 steps over 16 k wavefronts is 1e7 per lane);
-     * the wavefront's sums are 64-bit */
-    unsigned long long ws = (uint32_t)nsteps, wr = (uint32_t)nrays, wb = (uint32_t)nbulk;
+    /* the drain's per-lane step counts stay 32-bit (a lane marches far
+     * fewer than 2^32 steps: 1e13 steps over 16 k wavefronts is 1e7 per
+     * lane); the wavefront's sums are 64-bit */
+    unsigned long long ws = (uint32_t)nsteps, wr = (uint32_t)nrays;
     for (int o = 32; o > 0; o >>= 1) {
         ws += __shfl_xor(ws, o);
         wr += __shfl_xor(wr, o);
-        wb += __shfl_xor(wb, o);
     }
     if (lane == 0) {
         atomicAdd(&stats[0], wr);
-        atomicAdd(&stats[1], ws);
+        atomicAdd(&stats[1], ws + (unsigned long long)bsteps);
         atomicAdd(&stats[3], 64ull * (uint32_t)bsub);
-        atomicAdd(&stats[4], wb);
+        atomicAdd(&stats[4], (unsigned long long)bsteps);
     }
 }
 
@@ -729,7 +772,7 @@ k_batched(DevCommon cm, const DevSnap *__restrict__ snaps, int nsnap, double *__
 
 /* multigrid (registers held to 4 wavefronts per SIMD) */
 template <bool SURF, int TM>
-__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(4, 8)))
+__global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_batched_mg(DevCommon cm, const DevSnap *__restrict__ snaps, int nsnap, double *__restrict__ de,
              unsigned long long *__restrict__ stats, int *__restrict__ queue, int threshold, int spi) {
     batched_body<SURF, true, TM>(cm, snaps, nsnap, de, stats, queue, threshold, spi);

@@ -80,7 +80,7 @@ struct TableSet {
     float Tnb0 = 0.f, invdT = 0.f;
     int bf16 = 0;
     std::vector<float> Tnb;
-    DevBuf kb, bfix, cdf_gs, rank_of, kp16, bp;
+    DevBuf kb, cdf_gs, rank_of, kp16, bp;
 };
 
 /* round-to-nearest-even FP32 -> BF16 (finite inputs) */
@@ -95,7 +95,7 @@ struct SnapState {
     float Tmax = 0.f, kappamax = 0.f, Ibmax = 0.f, Ibmax_w = 0.f, Ibmax_e = 0.f;
     int tm = 0, tstar = 0;
     DevBuf T0, cdf_b, cdf_g, cdf_bs, guide_b, guide_bs;
-    DevBuf lvT[RC_MAXLEV], lvTw[RC_MAXLEV], lvTz[RC_MAXLEV];
+    DevBuf Pall;                          /* table positions of every level (DevLevel.P) */
     std::unique_ptr<TableSet> grey;     /* grey runs: tables follow Tmax */
     Tables greyhost;
 };
@@ -112,7 +112,9 @@ struct rc_ctx {
     std::vector<int32_t> idx, rank_of;
     TableSet nbset;
     std::vector<Grid> grids;
-    DevBuf xu[RC_MAXLEV], x0, y0, z0;
+    DevBuf xuall, x0, y0, z0;               /* x faces of every level, level g at xoff[g] */
+    int xoff[RC_MAXLEV] = {}, nxu = 0;
+    size_t poff[RC_MAXLEV] = {}, npos = 0;  /* level g's table positions at poff[g] */
     DevBuf cx[RC_MAXLEV], cy[RC_MAXLEV], cz[RC_MAXLEV];   /* cell centres per level */
     DevBuf spl;                             /* spline restriction scratch */
     int restriction = 0;                    /* coarse-level T: 0 texture mean, 1 cubic spline */
@@ -152,7 +154,16 @@ int build_tableset(rc_ctx *c, const Tables &t, TableSet &ts, std::vector<int32_t
     ts.Tnb0 = t.Tnb[0];
     ts.invdT = 1.0f / dT;
     const bool gt = p.grey && p.grey_true;
-    std::vector<float2> kb((size_t)nB * nQ * (nT + 1));
+    /* Ib of the fixed-temperature walls per face and band (black_copy,
+     * memory_copy_noint.cu:27-87): the rows' fixed-wall entries */
+    std::vector<float> bfix((size_t)6 * nB);
+    for (int f = 0; f < 6; ++f)
+        for (int r = 0; r < nB; ++r) {
+            const double Tw = p.Tw[f];
+            bfix[(size_t)f * nB + r] = gt ? (float)(kStefan * Tw * Tw * Tw * Tw / kPi) : planck(p.Tw[f], t.wvc[idx[r]]);
+        }
+    const int S = tab_stride(nT);
+    std::vector<float2> kb((size_t)nB * nQ * S);
     std::vector<float> brow(nT + 1);
     for (int r = 0; r < nB; ++r) {
         const int b = idx[r];
@@ -160,41 +171,42 @@ int build_tableset(rc_ctx *c, const Tables &t, TableSet &ts, std::vector<int32_t
             brow[q] = gt ? (float)(kStefan * std::pow((double)t.Tnb[q], 4) / kPi) : planck(t.Tnb[q], t.wvc[b]);
         brow[nT] = brow[nT - 1];
         for (int g = 0; g < nQ; ++g) {
-            float2 *row = &kb[((size_t)r * nQ + g) * (nT + 1)];
+            float2 *row = &kb[((size_t)r * nQ + g) * S];
             for (int q = 0; q < nT; ++q) row[q] = make_float2(t.kq[((size_t)b * nT + q) * nQ + g], brow[q]);
             row[nT] = row[nT - 1];
+            for (int f = 0; f < 6; ++f)
+                row[nT + 1 + 2 * f] = row[nT + 2 + 2 * f] = make_float2(0.f, bfix[(size_t)f * nB + r]);
         }
     }
     ts.bf16 = c->table_bf16;
     if (ts.bf16) {
-        /* k as BF16 pairs with a power-of-two scale per band: 2^floor(log2 max k) */
-        std::vector<uint32_t> kp((size_t)nB * nQ * nT);
-        std::vector<float4> bp((size_t)nB * nT);
+        /* k as BF16 pairs with a power-of-two scale per band: 2^floor(log2 max k);
+         * the fixed-wall entries hold k = 0 and the wall's Ib pair */
+        std::vector<uint32_t> kp((size_t)nB * nQ * S, 0u);
+        std::vector<float4> bp((size_t)nB * S, make_float4(0.f, 0.f, 0.f, 0.f));
         for (int r = 0; r < nB; ++r) {
             float mx = 0.f;
             for (int g = 0; g < nQ; ++g)
-                for (int q = 0; q < nT; ++q) mx = std::max(mx, kb[((size_t)r * nQ + g) * (nT + 1) + q].x);
+                for (int q = 0; q < nT; ++q) mx = std::max(mx, kb[((size_t)r * nQ + g) * S + q].x);
             int ex = 0;
             (void)std::frexp(mx > 0.f ? mx : 1.f, &ex);
             const float sc = std::ldexp(1.0f, ex - 1), inv = std::ldexp(1.0f, 1 - ex);
             for (int g = 0; g < nQ; ++g) {
-                const float2 *row = &kb[((size_t)r * nQ + g) * (nT + 1)];
+                const float2 *row = &kb[((size_t)r * nQ + g) * S];
                 for (int q = 0; q < nT; ++q)
-                    kp[((size_t)r * nQ + g) * nT + q] =
+                    kp[((size_t)r * nQ + g) * S + q] =
                         (uint32_t)to_bf16(row[q].x * inv) | ((uint32_t)to_bf16(row[q + 1].x * inv) << 16);
             }
-            const float2 *row0 = &kb[(size_t)r * nQ * (nT + 1)];
-            for (int q = 0; q < nT; ++q) bp[(size_t)r * nT + q] = make_float4(row0[q].y, row0[q + 1].y, sc, 0.f);
+            const float2 *row0 = &kb[(size_t)r * nQ * S];
+            for (int q = 0; q < nT; ++q) bp[(size_t)r * S + q] = make_float4(row0[q].y, row0[q + 1].y, sc, 0.f);
+            for (int f = 0; f < 6; ++f) {
+                const float w = bfix[(size_t)f * nB + r];
+                bp[(size_t)r * S + nT + 1 + 2 * f] = bp[(size_t)r * S + nT + 2 + 2 * f] = make_float4(w, w, sc, 0.f);
+            }
         }
         if (int e = upload(ts.kp16, kp, c->stream)) return e;
         if (int e = upload(ts.bp, bp, c->stream)) return e;
     }
-    std::vector<float> bfix((size_t)6 * nB);
-    for (int f = 0; f < 6; ++f)
-        for (int r = 0; r < nB; ++r) {
-            const double Tw = p.Tw[f];
-            bfix[(size_t)f * nB + r] = gt ? (float)(kStefan * Tw * Tw * Tw * Tw / kPi) : planck(p.Tw[f], t.wvc[idx[r]]);
-        }
     std::vector<float> cgs(nQ);
     {
         std::vector<float> wq = t.wq;
@@ -205,7 +217,6 @@ int build_tableset(rc_ctx *c, const Tables &t, TableSet &ts, std::vector<int32_t
         for (int g = 0; g < nQ; ++g) cgs[g] = cgs[g] / acc;
     }
     if (int e = upload(ts.kb, kb, c->stream)) return e;
-    if (int e = upload(ts.bfix, bfix, c->stream)) return e;
     if (int e = upload(ts.cdf_gs, cgs, c->stream)) return e;
     if (int e = upload(ts.rank_of, rank_of, c->stream)) return e;
     RC_HIP(hipStreamSynchronize(c->stream));
@@ -352,13 +363,17 @@ int push_snaps(rc_ctx *c) {
         DevSnap d{};
         d.Tmax = S.Tmax; d.Ibmax = S.Ibmax; d.Ibmax_w = S.Ibmax_w; d.Ibmax_e = S.Ibmax_e;
         d.Tnb0 = ts.Tnb0; d.invdT = ts.invdT; d.tstar = S.tstar; d.snap = (int)s;
-        d.kb = ts.kb.as<float2>(); d.bfix = ts.bfix.as<float>();
+        d.kb = ts.kb.as<float2>();
         d.kbf16 = ts.bf16;
         d.kp16 = ts.kp16.as<uint32_t>(); d.bp = ts.bp.as<float4>();
         d.cdf_b = S.cdf_b.as<float>(); d.cdf_g = S.cdf_g.as<float>(); d.cdf_bs = S.cdf_bs.as<float>();
         d.cdf_gs = ts.cdf_gs.as<float>(); d.rank_of = ts.rank_of.as<int>();
         d.guide_b = S.guide_b.as<int>(); d.guide_bs = S.guide_bs.as<int>(); d.gM = kGuideM;
         d.T0 = S.T0.as<float>();
+        d.xu_all = c->xuall.as<float>();
+        d.P_all = S.Pall.as<float>();
+        d.nxu = c->nxu;
+        d.npos = (int)c->npos;
         d.x0 = c->x0.as<float>(); d.y0 = c->y0.as<float>(); d.z0 = c->z0.as<float>();
         d.imax = p.imax; d.jmax = p.jmax; d.kmax = p.kmax; d.zstart = c->zstart;
         d.ncell = c->ncell; d.nsurf = c->nsurf;
@@ -369,10 +384,26 @@ int push_snaps(rc_ctx *c) {
             L.ncell = L.im * L.jm * L.km;
             L.dy = p.Ly * 1.0f / (float)L.jm;
             L.dz = p.Lz * 1.0f / (float)L.km;
-            L.xu = c->xu[g].as<float>();
-            L.T = S.lvT[g].as<float>();
-            L.Tw = S.lvTw[g].p ? S.lvTw[g].as<float>() : nullptr;
-            L.Tz = S.lvTz[g].p ? S.lvTz[g].as<float>() : nullptr;
+            L.xoff = c->xoff[g];
+            L.xu = c->xuall.as<float>() + c->xoff[g];
+            L.poff = (int)c->poff[g];
+            L.P = S.Pall.as<float>() + c->poff[g];
+            /* coarsening to level g + 1: ratios and their multipliers
+             * m = ceil(2^32 / r), exact for every index n of the level
+             * when n (m r - 2^32) < 2^32 (else the kernel divides) */
+            const int dims[3] = {p.maxi[g], p.maxj[g], p.maxk[g]};
+            for (int a = 0; a < 3; ++a) {
+                const int next = g + 1 < p.grid_num ? (a == 0 ? p.maxi[g + 1] : a == 1 ? p.maxj[g + 1] : p.maxk[g + 1])
+                                                    : dims[a];
+                const int r = dims[a] / next;
+                L.cr[a] = r;
+                L.cmul[a] = 0u;
+                if (r >= 2) {
+                    const uint64_t m = (((uint64_t)1 << 32) + (uint64_t)r - 1) / (uint64_t)r;
+                    const uint64_t e = m * (uint64_t)r - ((uint64_t)1 << 32);
+                    if ((uint64_t)(dims[a] + r - 1) * e < ((uint64_t)1 << 32)) L.cmul[a] = (uint32_t)m;
+                }
+            }
         }
         v[s] = d;
     }
@@ -510,12 +541,20 @@ rc_ctx *rc_create(const rc_params *pp) {
         return nullptr;
     }
     c->grids.resize(p.grid_num);
+    std::vector<float> xuall;
     for (int g = 0; g < p.grid_num; ++g) {
         build_grid(p, g, c->grids[g]);
-        if (upload(c->xu[g], c->grids[g].xu, c->stream) || upload(c->cx[g], c->grids[g].x, c->stream) ||
-            upload(c->cy[g], c->grids[g].y, c->stream) || upload(c->cz[g], c->grids[g].z, c->stream))
+        c->xoff[g] = (int)xuall.size();
+        xuall.insert(xuall.end(), c->grids[g].xu.begin(), c->grids[g].xu.end());
+        c->poff[g] = c->npos;
+        c->npos += level_pos_count(p.maxi[g], p.maxj[g], p.maxk[g]);
+        if (upload(c->cx[g], c->grids[g].x, c->stream) || upload(c->cy[g], c->grids[g].y, c->stream) ||
+            upload(c->cz[g], c->grids[g].z, c->stream))
             return nullptr;
     }
+    c->nxu = (int)xuall.size();
+    if (c->npos >= ((size_t)1 << 31)) { set_error("grid too large: table positions of all levels beyond 2^31"); return nullptr; }
+    if (upload(c->xuall, xuall, c->stream)) return nullptr;
     if (upload(c->x0, c->grids[0].x, c->stream) || upload(c->y0, c->grids[0].y, c->stream) ||
         upload(c->z0, c->grids[0].z, c->stream))
         return nullptr;
@@ -589,23 +628,25 @@ int rc_set_fields(rc_ctx *c, int nsnap, const float *const *T, int zstart) {
         if (int e = snapshot_rows(c, *tb, S)) return e;
         if (int e = S.T0.ensure(nT0 * 4)) return e;
         RC_HIP(hipMemcpyAsync(S.T0.p, T[s], nT0 * 4, hipMemcpyHostToDevice, c->stream));
+        const TableSet &tset = S.grey ? *S.grey : c->nbset;
+        if (int e = S.Pall.ensure(c->npos * 4)) return e;
         for (int g = 0; g < p.grid_num; ++g) {
             const size_t nl = (size_t)p.maxi[g] * p.maxj[g] * p.maxk[g];
-            if (int e = S.lvT[g].ensure(nl * 4)) return e;
-            const bool wx = (p.bd[FW] == 2 && p.adiabw == 1) || (p.bd[FE] == 2 && p.adiabe == 1);
-            const bool wz = p.bd[FB] == 4 || p.bd[FT] == 4;
-            if (wx) { if (int e = S.lvTw[g].ensure((size_t)2 * p.maxj[g] * p.maxk[g] * 4)) return e; }
-            if (wz) { if (int e = S.lvTz[g].ensure((size_t)2 * p.maxi[g] * p.maxj[g] * 4)) return e; }
-            launch_level_T(c->stream, S.T0.as<float>(), p, g, S.lvT[g].as<float>(),
-                           wx ? S.lvTw[g].as<float>() : nullptr, wz ? S.lvTz[g].as<float>() : nullptr);
+            float *Pg = S.Pall.as<float>() + c->poff[g];
+            launch_level_P(c->stream, S.T0.as<float>(), p, g, tset.Tnb0, tset.invdT, p.nT, Pg);
             if (c->restriction == 1 && g > 0) {
-                /* cells and x-wall faces by cubic splines; bd=4 z walls keep the texture mean */
+                /* cells and adiabatic x-wall faces by cubic splines (then to
+                 * positions); bd=4 z walls keep the texture mean */
+                const bool wx = (p.bd[FW] == 2 && p.adiabw == 1) || (p.bd[FE] == 2 && p.adiabe == 1);
+                const size_t nw = (size_t)2 * p.maxj[g] * p.maxk[g];
                 const size_t ns = spline_scratch_floats(p.imax, p.jmax, p.kmax, p.maxi[g], p.maxj[g], p.maxk[g]);
-                if (int e = c->spl.ensure(ns * 4)) return e;
+                if (int e = c->spl.ensure((ns + nl + nw) * 4)) return e;
+                float *tl = c->spl.as<float>() + ns, *tw = tl + nl;
                 launch_spline_level(c->stream, S.T0.as<float>(), c->cx[0].as<float>(), c->cy[0].as<float>(),
                                     c->cz[0].as<float>(), c->cx[g].as<float>(), c->cy[g].as<float>(),
                                     c->cz[g].as<float>(), p.imax, p.jmax, p.kmax, p.maxi[g], p.maxj[g], p.maxk[g],
-                                    S.lvT[g].as<float>(), wx ? S.lvTw[g].as<float>() : nullptr, c->spl.as<float>());
+                                    tl, wx ? tw : nullptr, c->spl.as<float>());
+                launch_pos_from_T(c->stream, p, g, tl, wx ? tw : nullptr, tset.Tnb0, tset.invdT, p.nT, Pg);
             }
         }
         RC_HIP(hipGetLastError());

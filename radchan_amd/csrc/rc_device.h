@@ -153,6 +153,31 @@ __device__ __forceinline__ void expm1_exp(float x, float &em1, float &e) {
     em1 = under ? -1.0f : mv;
 }
 
+/* expm1_exp for x <= 88 (the march's exponents: minus an optical depth,
+ * which is never below -0 by more than a rounding): the same values,
+ * without the clamp at the top of the range */
+__device__ __forceinline__ void expm1_exp_le88(float x, float &em1, float &e) {
+    const bool under = x < -87.0f;
+    const float xc = under ? -87.0f : x;
+    const float n = rintf(xc * 1.44269502f);
+    float r = fmaf(-n, 0.693145752f, xc);
+    r = fmaf(-n, 1.42860677e-6f, r);
+    float p = fmaf(r, (float)(1.0 / 362880.0), (float)(1.0 / 40320.0));
+    p = fmaf(r, p, (float)(1.0 / 5040.0));
+    p = fmaf(r, p, (float)(1.0 / 720.0));
+    p = fmaf(r, p, (float)(1.0 / 120.0));
+    p = fmaf(r, p, (float)(1.0 / 24.0));
+    p = fmaf(r, p, (float)(1.0 / 6.0));
+    p = fmaf(r, p, 0.5f);
+    p = fmaf(r * r, p, r);
+    const int ni = (int)n;
+    const float sc = __int_as_float((ni + 127) << 23);
+    const float ev = fmaf(sc, p, sc);
+    const float mv = (ni == 0) ? p : fmaf(sc, p, sc - 1.0f);
+    e = under ? 0.0f : ev;
+    em1 = under ? -1.0f : mv;
+}
+
 /* first index with cdf[i] > R; R == 1 and past-the-end map to n-1 */
 __device__ __forceinline__ int upper_bound(const float *__restrict__ cdf, int n, float R) {
     int lo = 0, hi = n;
@@ -205,6 +230,22 @@ __device__ __forceinline__ float lerp(float a, float b, float w) { return fmaf(w
 
 /* row t and weight of temperature T in a table starting at Tnb0 with
  * spacing 1/invdT, clamped to the first and last rows */
+/* the table position of temperature T: (T - Tnb0) invdT clamped to
+ * [0, nT - 1] (0 for NaN), so that ppos() of it gives exactly the t and a
+ * of tpos(T) */
+__device__ __forceinline__ float table_pos(float T, float Tnb0, float invdT, int nT) {
+    const float pos = (T - Tnb0) * invdT;
+    const bool lo = !(pos > 0.0f);
+    const bool hi = pos >= (float)(nT - 1);
+    return lo ? 0.0f : (hi ? (float)(nT - 1) : pos);
+}
+
+/* row t and interpolation weight a of a table position p >= 0 */
+__device__ __forceinline__ void ppos(float p, int &t, float &a) {
+    t = (int)p;
+    a = p - (float)t;
+}
+
 __device__ __forceinline__ void tpos(float T, float Tnb0, float invdT, int nT, int &t, float &a) {
     const float pos = (T - Tnb0) * invdT;
     const bool lo = !(pos > 0.0f);

@@ -26,28 +26,51 @@ struct DevCommon {
     uint32_t k0, k1;               /* Philox key */
 };
 
-/* one grid level of one snapshot */
+/* Table rows: nT temperatures, one pad entry (the clamped top row needs
+ * no branch), then two entries per face (W,E,S,N,B,T) holding the Ib of a
+ * black wall of fixed temperature with k = 0, so that every black wall's
+ * emission is a table lookup at a position: row stride tab_stride(nT),
+ * fixed wall of face f at position nT + 1 + 2 f */
+__host__ __device__ inline int tab_stride(int nT) { return nT + 13; }
+__host__ __device__ inline float fixed_wall_pos(int nT, int face) { return (float)(nT + 1 + 2 * face); }
+
+/* one grid level of one snapshot.  P holds table positions (temperature
+ * mapped to the table's row coordinate and clamped, the row t and weight
+ * a being (int)p and p - t): the cells [km][jm][im], then the black walls
+ * -- x faces [2][km][jm] (adiabatic: the texture T at the wall; fixed T:
+ * the face's fixed position), y faces [2][km][im] (fixed), z faces
+ * [2][jm][im] (bd 4: the adjacent-cell texture T; fixed) -- see
+ * wall_index */
 struct DevLevel {
     int im, jm, km, maxs;
     float dy, dz;                  /* uniform y/z spacing: face j at j*dy */
-    int ncell, pad1;               /* ncell = im jm km                    */
+    int ncell, xoff;               /* ncell = im jm km; xu = xu_all + xoff */
     const float *xu;               /* im+1 x faces                        */
-    const float *T;                /* [km][jm][im] cell temperatures      */
-    const float *Tw;               /* [2][km][jm] adiabatic x-wall T      */
-    const float *Tz;               /* [2][jm][im] bd=4 z-wall T           */
+    const float *P;                /* table positions, cells then walls   */
+    int poff, pad1;                /* P = P_all + poff                    */
+    /* coarsening to the next level (mc_gpu_noint.cu:215-223): the index
+     * ratios along x, y, z and, per ratio r >= 2, a multiplier m with
+     * (i + r - 1) / r == umulhi(i + r - 1, m) for every index of the
+     * level (0: divide) */
+    int cr[3];
+    uint32_t cmul[3];
 };
+
+/* entries of a level's position array */
+__host__ __device__ inline size_t level_pos_count(int im, int jm, int km) {
+    return (size_t)im * jm * km + 2 * ((size_t)jm * km + (size_t)im * km + (size_t)im * jm);
+}
 
 /* one temperature snapshot (device memory) */
 struct DevSnap {
     float Tmax, Ibmax, Ibmax_w, Ibmax_e;
     float Tnb0, invdT;
     int tstar, snap;
-    const float2 *kb;              /* [(rank*nQ+g)*(nT+1)+t] = {k, Ib}    */
-    const uint32_t *kp16;          /* BF16 mode: [(rank*nQ+g)*nT+t] = k_t | k_t+1 << 16 */
-    const float4 *bp;              /* BF16 mode: [rank*nT+t] = {Ib_t, Ib_t+1, scale, 0}: */
-                                   /* Ib pair and the band's power-of-two k scale       */
-    int kbf16, pad3;               /* 1: k tables stored as BF16                       */
-    const float *bfix;             /* [6][nB] fixed-temperature wall Ib   */
+    const float2 *kb;              /* [(rank*nQ+g)*S+t] = {k, Ib}, S = tab_stride(nT) */
+    const uint32_t *kp16;          /* BF16 mode: [(rank*nQ+g)*S+t] = k_t | k_t+1 << 16 */
+    const float4 *bp;              /* BF16 mode: [rank*S+t] = {Ib_t, Ib_t+1, scale, 0}: */
+                                   /* Ib pair and the band's power-of-two k scale      */
+    int kbf16, pad3;               /* 1: k tables stored as BF16                      */
     const float *cdf_b;            /* [nB] fluid band CDF, row t*         */
     const float *cdf_g;            /* [nB][nQ] g CDFs, row t*             */
     const float *cdf_bs;           /* [nB] wall band CDF, row t*          */
@@ -57,6 +80,9 @@ struct DevSnap {
     const int *guide_bs;           /* [gM+1] same for cdf_bs               */
     int gM, pad2;
     const float *T0;               /* ghost-padded input field            */
+    const float *xu_all;           /* x faces of every level, level g at lev[g].xoff */
+    const float *P_all;            /* table positions of every level, g at lev[g].poff */
+    int nxu, npos;                 /* their lengths                       */
     const float *x0, *y0, *z0;     /* level-0 cell centres                */
     /* the snapshot's grid (snapshots of one batched launch may differ in
      * it): global sizes, first z plane and counts of emitting cells and
@@ -68,7 +94,15 @@ struct DevSnap {
     DevLevel lev[RC_MAXLEV];
 };
 
-void launch_level_T(hipStream_t st, const float *T0, const rc_params &p, int g, float *Tl, float *Tw, float *Tz);
+/* table positions of level g (cells and black walls, layout of DevLevel.P)
+ * from the ghost-padded field T0; the table's first temperature, inverse
+ * spacing and row count map temperatures to positions */
+void launch_level_P(hipStream_t st, const float *T0, const rc_params &p, int g, float Tnb0, float invdT, int nT,
+                    float *P);
+/* cell and x-wall temperatures of level g (Tl [km][jm][im], Tw [2][km][jm]
+ * or NULL) converted to positions in P (the spline restriction's output) */
+void launch_pos_from_T(hipStream_t st, const rc_params &p, int g, const float *Tl, const float *Tw, float Tnb0,
+                       float invdT, int nT, float *P);
 /* cubic-spline restriction of level g (interp3D): cells Tl [km][jm][im] and,
  * if Tw, x-wall faces [2][km][jm]; scratch of spline_scratch_floats floats */
 size_t spline_scratch_floats(int imax, int jmax, int kmax, int im, int jm, int km);

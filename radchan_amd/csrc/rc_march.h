@@ -54,7 +54,7 @@ enum { TAB_RT = -1, TAB_F32 = 0, TAB_BF16 = 1 };
  * textures, memory_copy_noint.cu:241-247, mc_gpu_noint.cu:784-791).  FP32
  * tables hold {k, Ib} pairs; BF16 tables hold k_t and k_t+1 as a BF16 pair
  * times a power-of-two band scale, and per band the Ib pairs in FP32 beside
- * the scale (one 16-byte word). */
+ * the scale (one 16-byte word).  Rows are tab_stride(nT) entries long. */
 /* the snapshot fields the march reads, held as wavefront-uniform values by
  * the batched kernel (the march functions accept it or a DevSnap) */
 struct SnapView {
@@ -63,30 +63,47 @@ struct SnapView {
     const float2 *kb;
     const uint32_t *kp16;
     const float4 *bp;
-    const float *bfix;
+    const float *xu_all, *P_all;
+    int nxu, npos;
 };
 
+/* k and Ib of table row `row` at row t with weight a */
 template <int TM = TAB_RT, class S>
-__device__ __forceinline__ void kb_at(const S &sn, uint32_t row, int rank, int nT, float T, float &K, float &B) {
-    int t; float a;
-    tpos(T, sn.Tnb0, sn.invdT, nT, t, a);
+__device__ __forceinline__ void kb_ta(const S &sn, uint32_t row, int rank, int nT, int t, float a, float &K, float &B) {
+    const uint32_t st = (uint32_t)tab_stride(nT);
     if (TM == TAB_RT ? sn.kbf16 != 0 : TM == TAB_BF16) {
-        const uint32_t w = gld32(sn.kp16, row * (uint32_t)nT + (uint32_t)t);
-        const float4 b = gld32(sn.bp, (uint32_t)rank * (uint32_t)nT + (uint32_t)t);
+        const uint32_t w = gld32(sn.kp16, row * st + (uint32_t)t);
+        const float4 b = gld32(sn.bp, (uint32_t)rank * st + (uint32_t)t);
         const float k0 = __uint_as_float(w << 16) * b.z;
         const float k1 = __uint_as_float(w & 0xFFFF0000u) * b.z;
         K = lerp(k0, k1, a);
         B = lerp(b.x, b.y, a);
     } else {
-        const float4 q = gld32_pair(sn.kb, row * (uint32_t)(nT + 1) + (uint32_t)t);
+        const float4 q = gld32_pair(sn.kb, row * st + (uint32_t)t);
         K = lerp(q.x, q.z, a);
         B = lerp(q.y, q.w, a);
     }
 }
 
-/* kb_at in two halves, for a fixed storage mode: the loads of the two
+/* at temperature T (the emission of a ray) */
+template <int TM = TAB_RT, class S>
+__device__ __forceinline__ void kb_at(const S &sn, uint32_t row, int rank, int nT, float T, float &K, float &B) {
+    int t; float a;
+    tpos(T, sn.Tnb0, sn.invdT, nT, t, a);
+    kb_ta<TM>(sn, row, rank, nT, t, a, K, B);
+}
+
+/* at a table position (a cell or wall entry of a level's position array) */
+template <int TM = TAB_RT, class S>
+__device__ __forceinline__ void kb_pos(const S &sn, uint32_t row, int rank, int nT, float pos, float &K, float &B) {
+    int t; float a;
+    ppos(pos, t, a);
+    kb_ta<TM>(sn, row, rank, nT, t, a, K, B);
+}
+
+/* kb_pos in two halves, for a fixed storage mode: the loads of the two
  * table rows t, t+1 (issued at once), and the interpolation (when the
- * values are needed); finish(fetch(...)) computes exactly kb_at */
+ * values are needed); finish(fetch(...)) computes exactly kb_pos */
 struct KbRaw {
     float4 q;        /* FP32: {k_t, Ib_t, k_t+1, Ib_t+1}; BF16: {Ib_t, Ib_t+1, scale, 0} */
     uint32_t w;      /* BF16: k_t | k_t+1 << 16 */
@@ -94,18 +111,24 @@ struct KbRaw {
 };
 
 template <int TM, class S>
-__device__ __forceinline__ KbRaw kb_fetch(const S &sn, uint32_t row, int rank, int nT, float T) {
+__device__ __forceinline__ void kb_load(const S &sn, uint32_t row, int rank, int nT, int t, KbRaw &k) {
     static_assert(TM == TAB_F32 || TM == TAB_BF16, "fixed storage mode");
-    KbRaw k;
-    int t;
-    tpos(T, sn.Tnb0, sn.invdT, nT, t, k.a);
+    const uint32_t st = (uint32_t)tab_stride(nT);
     if (TM == TAB_BF16) {
-        k.w = gld32(sn.kp16, row * (uint32_t)nT + (uint32_t)t);
-        k.q = gld32(sn.bp, (uint32_t)rank * (uint32_t)nT + (uint32_t)t);
+        k.w = gld32(sn.kp16, row * st + (uint32_t)t);
+        k.q = gld32(sn.bp, (uint32_t)rank * st + (uint32_t)t);
     } else {
         k.w = 0u;
-        k.q = gld32_pair(sn.kb, row * (uint32_t)(nT + 1) + (uint32_t)t);
+        k.q = gld32_pair(sn.kb, row * st + (uint32_t)t);
     }
+}
+
+template <int TM, class S>
+__device__ __forceinline__ KbRaw kb_fetch(const S &sn, uint32_t row, int rank, int nT, float pos) {
+    KbRaw k;
+    int t;
+    ppos(pos, t, k.a);
+    kb_load<TM>(sn, row, rank, nT, t, k);
     return k;
 }
 
@@ -137,64 +160,80 @@ __device__ __forceinline__ void wall_hit(const En &e, Ray &r, float Bw, float &d
 /* outcome of the boundary condition of a face */
 enum { RB_ON = 0, RB_WALL = 1, RB_LOST = 2 };
 
+/* the entry of the black wall the ray left the grid through (face `face`,
+ * the other two indices in range) in the level's position array */
+template <class G>
+__device__ __forceinline__ uint32_t wall_index(const G &L, const Ray &r, int face) {
+    const uint32_t im = (uint32_t)L.im, jm = (uint32_t)L.jm, km = (uint32_t)L.km;
+    const uint32_t i = (uint32_t)(r.ic - 1), j = (uint32_t)(r.jc - 1), k = (uint32_t)(r.kc - 1);
+    const uint32_t side = (uint32_t)(face & 1);
+    const uint32_t nx = __umul24(jm, km), ny = __umul24(im, km), nz = __umul24(im, jm);
+    const uint32_t x0 = __umul24(im, nx), y0 = x0 + 2u * nx, z0 = y0 + 2u * ny;
+    return face < FS ? x0 + side * nx + j + jm * k : (face < FB ? y0 + side * ny + i + im * k : z0 + side * nz + i + im * j);
+}
+
+/* the face a ray that left the grid crossed (exactly one index left 1..n
+ * in the step) */
+template <class G>
+__device__ __forceinline__ int exit_face(const G &L, const Ray &r) {
+    const bool xo = (uint32_t)(r.ic - 1) >= (uint32_t)L.im;
+    const bool yo = !xo && (uint32_t)(r.jc - 1) >= (uint32_t)L.jm;
+    const bool lo = xo ? r.ic == 0 : (yo ? r.jc == 0 : r.kc == 0);
+    return (xo ? FW : (yo ? FS : FB)) + (lo ? 0 : 1);
+}
+
+/* the condition of face f, from the six face codes packed 3 bits each (a
+ * wavefront-uniform word, no per-lane memory access) */
+__device__ __forceinline__ int face_bd(const DevCommon &cm, int face) {
+    const uint32_t pack = (uint32_t)cm.bd[0] | ((uint32_t)cm.bd[1] << 3) | ((uint32_t)cm.bd[2] << 6) |
+                          ((uint32_t)cm.bd[3] << 9) | ((uint32_t)cm.bd[4] << 12) | ((uint32_t)cm.bd[5] << 15);
+    return (int)((pack >> (3 * face)) & 7u);
+}
+
+/* periodic (bd 1): to the opposite end (low face -> last cell, coordinate
+ * at the high end); specular (bd 3): back into the edge cell, coordinate on
+ * the face, direction component reversed */
+template <class G>
+__device__ __forceinline__ void face_move(const DevCommon &cm, const G &L, Ray &r, int face, bool per) {
+    const bool xo = face < FS, yo = face >= FS && face < FB, zo = face >= FB;
+    const bool lo = (face & 1) == 0;
+    const bool hi_end = per == lo;
+    r.ic = xo ? (hi_end ? L.im : 1) : r.ic;
+    r.xp = xo ? (hi_end ? cm.Lx : 0.f) : r.xp;
+    r.jc = yo ? (hi_end ? L.jm : 1) : r.jc;
+    r.yp = yo ? (hi_end ? cm.Ly : 0.f) : r.yp;
+    r.kc = zo ? (hi_end ? L.km : 1) : r.kc;
+    r.zp = zo ? (hi_end ? cm.Lz : 0.f) : r.zp;
+    const bool fx = !per && xo, fy = !per && yo, fz = !per && zo;
+    r.sx = fx ? -r.sx : r.sx;
+    r.isx = fx ? -r.isx : r.isx;
+    r.sy = fy ? -r.sy : r.sy;
+    r.isy = fy ? -r.isy : r.isy;
+    r.sz = fz ? -r.sz : r.sz;
+    r.isz = fz ? -r.isz : r.isz;
+    r.f0 = r.sx < 0.f;
+    r.f1 = r.sy < 0.f;
+    r.f2 = r.sz < 0.f;
+}
+
 /* boundary condition of the face the ray just crossed (the one index that
  * left 1..n), without its tally: periodic and specular faces move the ray
  * on (RB_ON); a black face -- reference wall, adiabatic midpoint or fixed
- * temperature -- or a bd 4 z face ends it with the wall's Ib in Bw
- * (RB_WALL); a bd 5 face loses it (RB_LOST) -- mc_gpu_noint.cu:375-654 */
+ * temperature -- or a bd 4 z face ends it with the wall's Ib in Bw, the
+ * table at the wall's position (RB_WALL); a bd 5 face loses it (RB_LOST)
+ * -- mc_gpu_noint.cu:375-654 */
 template <int TM = TAB_RT, class S>
 __device__ __forceinline__ int boundary_move(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r,
                                              const En &e, float &Bw) {
-    const int nT = cm.nT;
-    /* the face crossed: exactly one index left 1..n in the step */
-    const bool xo = (uint32_t)(r.ic - 1) >= (uint32_t)L.im;
-    const bool yo = !xo && (uint32_t)(r.jc - 1) >= (uint32_t)L.jm;
-    const bool zo = !xo && !yo;
-    const bool lo = xo ? r.ic == 0 : (yo ? r.jc == 0 : r.kc == 0);
-    const int face = (xo ? FW : (yo ? FS : FB)) + (lo ? 0 : 1);
-    /* its condition, from the six face codes packed 3 bits each (a
-     * wavefront-uniform word, no per-lane memory access) */
-    const uint32_t pack = (uint32_t)cm.bd[0] | ((uint32_t)cm.bd[1] << 3) | ((uint32_t)cm.bd[2] << 6) |
-                          ((uint32_t)cm.bd[3] << 9) | ((uint32_t)cm.bd[4] << 12) | ((uint32_t)cm.bd[5] << 15);
-    const int bd = (int)((pack >> (3 * face)) & 7u);
+    const int face = exit_face(L, r);
+    const int bd = face_bd(cm, face);
     if (bd == 1 || bd == 3) {
-        /* periodic: to the opposite end (low face -> last cell, coordinate
-         * at the high end); specular: back into the edge cell, coordinate on
-         * the face, direction component reversed */
-        const bool per = bd == 1;
-        const bool hi_end = per == lo;
-        r.ic = xo ? (hi_end ? L.im : 1) : r.ic;
-        r.xp = xo ? (hi_end ? cm.Lx : 0.f) : r.xp;
-        r.jc = yo ? (hi_end ? L.jm : 1) : r.jc;
-        r.yp = yo ? (hi_end ? cm.Ly : 0.f) : r.yp;
-        r.kc = zo ? (hi_end ? L.km : 1) : r.kc;
-        r.zp = zo ? (hi_end ? cm.Lz : 0.f) : r.zp;
-        const bool fx = !per && xo, fy = !per && yo, fz = !per && zo;
-        r.sx = fx ? -r.sx : r.sx;
-        r.isx = fx ? -r.isx : r.isx;
-        r.sy = fy ? -r.sy : r.sy;
-        r.isy = fy ? -r.isy : r.isy;
-        r.sz = fz ? -r.sz : r.sz;
-        r.isz = fz ? -r.isz : r.isz;
-        r.f0 = r.sx < 0.f;
-        r.f1 = r.sy < 0.f;
-        r.f2 = r.sz < 0.f;
+        face_move(cm, L, r, face, bd == 1);
         return RB_ON;
     }
-    if (bd == 2) {
-        if ((face == FW && cm.adiab[0]) || (face == FE && cm.adiab[1])) {
-            const float Tw = gld(L.Tw, (size_t)(face == FE) * L.jm * L.km + (size_t)(r.jc - 1) + (size_t)L.jm * (r.kc - 1));
-            float Kw;
-            kb_at<TM>(sn, e.row, e.rank, nT, Tw, Kw, Bw);
-        } else {
-            Bw = gld(sn.bfix, (size_t)face * cm.nB + e.rank);
-        }
-        return RB_WALL;
-    }
-    if (bd == 4) {
-        const float Tw = gld(L.Tz, (size_t)(face == FT) * L.im * L.jm + (size_t)(r.ic - 1) + (size_t)L.im * (r.jc - 1));
+    if (bd == 2 || bd == 4) {
         float Kw;
-        kb_at<TM>(sn, e.row, e.rank, nT, Tw, Kw, Bw);
+        kb_pos<TM>(sn, e.row, e.rank, cm.nT, gld(L.P, wall_index(L, r, face)), Kw, Bw);
         return RB_WALL;
     }
     return RB_LOST;
@@ -216,38 +255,155 @@ __device__ __forceinline__ void boundary(const DevCommon &cm, const S &sn, const
     }
 }
 
+/* The batched march carries the ray's linear cell index from step to step
+ * (a step moves it by +-1, +-im or +-im jm) instead of forming it from the
+ * three indices: the per-ray steps along y and z, set at the ray's start,
+ * after a reflection or wrap and at a level change. */
+struct CellWalk {
+    uint32_t cell;
+    int dcy, dcz;
+};
+
+template <class G>
+__device__ __forceinline__ void cell_walk_setup(const G &L, const Ray &r, CellWalk &w) {
+    w.cell = (uint32_t)(r.ic - 1) + __umul24((uint32_t)L.im, (uint32_t)(r.jc - 1) +
+                                                              __umul24((uint32_t)L.jm, (uint32_t)(r.kc - 1)));
+    const int imjm = (int)__umul24((uint32_t)L.im, (uint32_t)L.jm);
+    w.dcy = r.f1 ? -L.im : L.im;
+    w.dcz = r.f2 ? -imjm : imjm;
+}
+
+/* The boundary condition of the batched march, without memory access:
+ * periodic and specular faces move the ray on; at a black or bd 4 face the
+ * ray keeps its transmissivity and gets the index of the wall's entry in
+ * the level's position array (wall = 1), and the next march step takes the
+ * wall's emission through the step's own table lookup (march_core_bulk);
+ * a bd 5 face loses the ray (tra = 0, no tally).  The moves are those of
+ * face_move, and they move the ray's cell walk with it; the wall entries
+ * are wall_index's. */
+template <class G>
+__device__ __forceinline__ void boundary_defer(const DevCommon &cm, const G &L, Ray &r, bool left, int &wall,
+                                               uint32_t &widx, CellWalk &cw) {
+    /* one block per axis, each taken only when a lane left through one of
+     * its two faces; the face codes are wavefront-uniform */
+    const uint32_t im = (uint32_t)L.im, jm = (uint32_t)L.jm, km = (uint32_t)L.km;
+    const uint32_t imjm = __umul24(im, jm), ncell = __umul24(im, __umul24(jm, km));
+    const bool xo = left && (uint32_t)(r.ic - 1) >= im;
+    const bool yo = left && !xo && (uint32_t)(r.jc - 1) >= jm;
+    const bool zo = left && !xo && !yo;
+    if (xo) {
+        const bool lo = r.ic == 0;
+        const int bd = lo ? cm.bd[FW] : cm.bd[FE];
+        if (bd == 1) {                     /* periodic: to the opposite end */
+            r.ic = lo ? L.im : 1;
+            r.xp = lo ? cm.Lx : 0.f;
+            cw.cell += lo ? im : (uint32_t)-(int)im;
+        } else if (bd == 3) {              /* specular: back into the edge cell, sx reversed */
+            r.ic = lo ? 1 : L.im;
+            r.xp = lo ? 0.f : cm.Lx;
+            r.sx = -r.sx;
+            r.isx = -r.isx;
+            r.f0 = r.sx < 0.f;
+            cw.cell += lo ? 1u : (uint32_t)-1;
+        } else {                           /* black */
+            wall = 1;
+            widx = ncell + (lo ? 0u : __umul24(jm, km)) + (uint32_t)(r.jc - 1) +
+                   __umul24(jm, (uint32_t)(r.kc - 1));
+        }
+    }
+    if (yo) {
+        const bool lo = r.jc == 0;
+        const int bd = lo ? cm.bd[FS] : cm.bd[FN];
+        if (bd == 1) {
+            r.jc = lo ? L.jm : 1;
+            r.yp = lo ? cm.Ly : 0.f;
+            const uint32_t w = __umul24(jm, im);
+            cw.cell += lo ? w : (uint32_t)-(int)w;
+        } else if (bd == 3) {
+            r.jc = lo ? 1 : L.jm;
+            r.yp = lo ? 0.f : cm.Ly;
+            r.sy = -r.sy;
+            r.isy = -r.isy;
+            r.f1 = r.sy < 0.f;
+            cw.cell += lo ? im : (uint32_t)-(int)im;
+            cw.dcy = -cw.dcy;
+        } else {
+            wall = 1;
+            widx = ncell + 2u * __umul24(jm, km) + (lo ? 0u : __umul24(im, km)) +
+                   (uint32_t)(r.ic - 1) + __umul24(im, (uint32_t)(r.kc - 1));
+        }
+    }
+    if (zo) {
+        const bool lo = r.kc == 0;
+        const int bd = lo ? cm.bd[FB] : cm.bd[FT];
+        if (bd == 1) {
+            r.kc = lo ? L.km : 1;
+            r.zp = lo ? cm.Lz : 0.f;
+            const uint32_t w = ncell;
+            cw.cell += lo ? w : (uint32_t)-(int)w;
+        } else if (bd == 3) {
+            r.kc = lo ? 1 : L.km;
+            r.zp = lo ? 0.f : cm.Lz;
+            r.sz = -r.sz;
+            r.isz = -r.isz;
+            r.f2 = r.sz < 0.f;
+            cw.cell += lo ? imjm : (uint32_t)-(int)imjm;
+            cw.dcz = -cw.dcz;
+        } else if (bd == 2 || bd == 4) {
+            wall = 1;
+            widx = ncell + 2u * (__umul24(jm, km) + __umul24(im, km)) + (lo ? 0u : imjm) +
+                   (uint32_t)(r.ic - 1) + __umul24(im, (uint32_t)(r.jc - 1));
+        } else {                           /* bd 5: lost, no tally */
+            r.tra = 0.f;
+        }
+    }
+}
+
 /* The geometry of one march step (one iteration of march_ray's loop,
  * mc_gpu_noint.cu:252-654), given the x face the ray heads for
  * (fx = xu[ic - f0]): distances to the three candidate faces, the segment
  * length ds, the position update and the index move along the crossed
- * axis; `cell` is the cell the segment crosses.  Nothing here depends on
- * the temperature.  Returns whether an index left the grid (the boundary
- * condition is then due). */
-__device__ __forceinline__ bool geo_core_fx(const DevLevel &L, Ray &r, float fx, uint32_t &cell, float &ds) {
+ * axis (xa: x, ya: y, else z).  Nothing here depends on the temperature.
+ * Returns whether an index left the grid (the boundary condition is then
+ * due). */
+template <class G>
+__device__ __forceinline__ bool geo_move(const G &L, Ray &r, float fx, float &ds, bool &xa, bool &ya) {
     const float fy = (float)(r.jc - r.f1) * L.dy;
     const float fz = (float)(r.kc - r.f2) * L.dz;
     const float dsx = (fx - r.xp) * r.isx;
     const float dsy = (fy - r.yp) * r.isy;
     const float dsz = (fz - r.zp) * r.isz;
-    /* 24-bit products (full rate): the factors are below 2^24 for every
-     * grid whose jm km and im are, and the product is the cell index */
-    cell = min((uint32_t)(r.ic - 1) + __umul24((uint32_t)L.im, (uint32_t)(r.jc - 1) +
-                                                                  __umul24((uint32_t)L.jm, (uint32_t)(r.kc - 1))),
-               (uint32_t)L.ncell - 1u);
     ds = dsx;
     ds = (ds > dsy) ? dsy : ds;
     ds = (ds > dsz) ? dsz : ds;
     r.xp = (ds == dsx) ? fx : fmaf(ds, r.sx, r.xp);
     r.yp = (ds == dsy) ? fy : fmaf(ds, r.sy, r.yp);
     r.zp = (ds == dsz) ? fz : fmaf(ds, r.sz, r.zp);
-    const bool xa = dsx < dsy && dsx < dsz;
-    const bool ya = !(dsx < dsy) && dsy < dsz;
+    xa = dsx < dsy && dsx < dsz;
+    ya = !(dsx < dsy) && dsy < dsz;
     const bool za = !xa && !ya;
     r.ic += xa ? (r.f0 ? -1 : 1) : 0;
     r.jc += ya ? (r.f1 ? -1 : 1) : 0;
     r.kc += za ? (r.f2 ? -1 : 1) : 0;
     return (uint32_t)(r.ic - 1) >= (uint32_t)L.im || (uint32_t)(r.jc - 1) >= (uint32_t)L.jm ||
            (uint32_t)(r.kc - 1) >= (uint32_t)L.km;
+}
+
+/* the linear index of the ray's cell on its level (clamped: a lane whose
+ * state is stale reads in bounds) */
+__device__ __forceinline__ uint32_t cell_index(const DevLevel &L, const Ray &r) {
+    /* 24-bit products (full rate): the factors are below 2^24 for every
+     * grid whose jm km and im are, and the product is the cell index */
+    return min((uint32_t)(r.ic - 1) + __umul24((uint32_t)L.im, (uint32_t)(r.jc - 1) +
+                                                                  __umul24((uint32_t)L.jm, (uint32_t)(r.kc - 1))),
+               (uint32_t)L.ncell - 1u);
+}
+
+/* geo_move, with `cell` the cell the segment crosses */
+__device__ __forceinline__ bool geo_core_fx(const DevLevel &L, Ray &r, float fx, uint32_t &cell, float &ds) {
+    cell = cell_index(L, r);
+    bool xa, ya;
+    return geo_move(L, r, fx, ds, xa, ya);
 }
 
 /* geo_core_fx with the x face looked up: in global memory, or with XL in
@@ -263,13 +419,13 @@ __device__ __forceinline__ bool geo_core(const DevLevel &L, Ray &r, uint32_t &ce
     return geo_core_fx(L, r, fx, cell, ds);
 }
 
-/* The absorption of one step given the crossed cell's k and Ib: attenuation
- * and reciprocity tally, transmissivity cut-off (the dump tally when it
- * falls below tol). */
-__device__ __forceinline__ void phys_core(float K, float B, float ds, const En &e, float tol, float &tra,
-                                          float &de, int &status) {
+/* The absorption of one step with optical depth -x (x = -K ds): attenuation
+ * and reciprocity tally at the cell's Ib B, transmissivity cut-off (the
+ * dump tally when it falls below tol). */
+__device__ __forceinline__ void phys_core_x(float x, float B, const En &e, float tol, float &tra, float &de,
+                                            int &status) {
     float em1, ex;
-    expm1_exp(-(K * ds), em1, ex);
+    expm1_exp_le88(x, em1, ex);
     const float alpha = -em1;
     const float fac = (B - e.Bi) * e.invBi;
     de -= ((e.W * tra) * alpha) * fac;
@@ -281,8 +437,13 @@ __device__ __forceinline__ void phys_core(float K, float B, float ds, const En &
     status = dump ? RC_RAY_ABSORBED : status;
 }
 
+__device__ __forceinline__ void phys_core(float K, float B, float ds, const En &e, float tol, float &tra,
+                                          float &de, int &status) {
+    phys_core_x(-(K * ds), B, e, tol, tra, de, status);
+}
+
 /* one march step: geometry, then the absorption at the crossed cell's
- * temperature (table interpolation, FP32).  Select-based, no branch;
+ * table position (table interpolation, FP32).  Select-based, no branch;
  * returns whether an index left the grid. */
 template <int TM = TAB_RT, class S>
 __device__ __forceinline__ bool march_core(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r,
@@ -291,40 +452,78 @@ __device__ __forceinline__ bool march_core(const DevCommon &cm, const S &sn, con
     float ds;
     const bool out = geo_core(L, r, cell, ds);
     float K, B;
-    kb_at<TM>(sn, e.row, e.rank, cm.nT, gld32(L.T, cell), K, B);
+    kb_pos<TM>(sn, e.row, e.rank, cm.nT, gld32(L.P, cell), K, B);
     phys_core(K, B, ds, e, tol, r.tra, de, status);
     return out;
 }
 
-/* march_core with the ray's last table pair kept in registers: a ray's
- * table row is fixed, and along its path the temperature row t changes
- * only every few cells, so the pair is loaded only when t changes (kt:
- * the cached t, -1 for none; the loads of the other lanes are masked off).
- * Same values, so the same results as march_core. */
-template <int TM, class S>
-__device__ __forceinline__ bool march_core_cached(const DevCommon &cm, const S &sn, const DevLevel &L, Ray &r,
-                                                  const En &e, float tol, float &de, int &status, int &kt,
-                                                  KbRaw &kc) {
-    uint32_t cell;
-    float ds;
-    const bool out = geo_core(L, r, cell, ds);
+/* A level of the batched march held per lane (the lanes of a multigrid
+ * wavefront march rays on different levels): sizes, step cap, spacings and
+ * the offsets of the level's x faces and table positions in the
+ * snapshot's arrays of every level */
+struct LaneLevel {
+    int im, jm, km, maxs;
+    float dy, dz;
+    uint32_t xoff, poff;
+};
+
+__device__ __forceinline__ LaneLevel lane_level(const DevLevel &L) {
+    LaneLevel l;
+    l.im = L.im; l.jm = L.jm; l.km = L.km; l.maxs = L.maxs;
+    l.dy = L.dy; l.dz = L.dz;
+    l.xoff = (uint32_t)L.xoff; l.poff = (uint32_t)L.poff;
+    return l;
+}
+
+/* the index ic on the next level, (ic + r - 1) / r (mc_gpu_noint.cu:215-223),
+ * by the level's multiplier */
+__device__ __forceinline__ int coarsen(int ic, int r, uint32_t m) {
+    const uint32_t n = (uint32_t)(ic + r - 1);
+    if (r == 1) return ic;
+    return (int)(m ? __umulhi(n, m) : n / (uint32_t)r);
+}
+
+/* One step of the batched march.  A ray's table row is fixed, and along
+ * its path the row t changes only every few cells, so the ray's last table
+ * pair stays in registers and is loaded again only when t changes (kt: the
+ * cached t, -1 for none; the loads of the other lanes are masked off).
+ * The x face is loaded before the table position, so that the wait for the
+ * position covers it and the geometry runs while a table pair is in
+ * flight.  A ray waiting at a black wall (wall = 1, set by boundary_defer)
+ * takes the wall's emission instead: the table at the wall's position,
+ * with the whole transmissivity absorbed (optical depth infinite: alpha =
+ * 1, the transmissivity then 0 and dumped), which is wall_hit's tally to
+ * the bit; its geometry is discarded.  The cell index is carried in `cw`.
+ * The level's x faces and positions are read at xu[xoff + i] and
+ * P[poff + i], the indices clamped to xmax and pmax (a lane whose state is
+ * stale reads in bounds).  Returns whether an index left the grid (never
+ * for a wall step).  Same values as march_core and boundary, so the same
+ * tallies. */
+template <int TM, class S, class G>
+__device__ __forceinline__ bool march_core_bulk(const DevCommon &cm, const S &sn, const G &L, const float *xu,
+                                                uint32_t xoff, uint32_t xmax, const float *P, uint32_t poff,
+                                                uint32_t pmax, Ray &r, const En &e, float tol, float &de, int &kt,
+                                                KbRaw &kc, bool wall, uint32_t widx, CellWalk &cw) {
+    const uint32_t ix = min(xoff + (uint32_t)(r.ic - r.f0), xmax);
+    const float fx = gld32(xu, ix);
+    asm volatile("" ::: "memory");
+    const float pos = gld32(P, min(poff + (wall ? widx : cw.cell), pmax));
     int t;
-    float a;
-    tpos(gld32(L.T, cell), sn.Tnb0, sn.invdT, cm.nT, t, a);
-    if (t != kt) {
-        if (TM == TAB_BF16) {
-            kc.w = gld32(sn.kp16, e.row * (uint32_t)cm.nT + (uint32_t)t);
-            kc.q = gld32(sn.bp, (uint32_t)e.rank * (uint32_t)cm.nT + (uint32_t)t);
-        } else {
-            kc.q = gld32_pair(sn.kb, e.row * (uint32_t)(cm.nT + 1) + (uint32_t)t);
-        }
-        kt = t;
+    ppos(pos, t, kc.a);
+    if (t != kt || wall) {
+        kb_load<TM>(sn, e.row, e.rank, cm.nT, t, kc);
+        kt = wall ? -1 : t;
     }
-    kc.a = a;
+    const int stx = r.f0 ? -1 : 1;
+    float ds;
+    bool xa, ya;
+    const bool out = geo_move(L, r, fx, ds, xa, ya);
+    cw.cell += xa ? (uint32_t)stx : (uint32_t)(ya ? cw.dcy : cw.dcz);
     float K, B;
     kb_finish<TM>(kc, K, B);
-    phys_core(K, B, ds, e, tol, r.tra, de, status);
-    return out;
+    int status = 0;
+    phys_core_x(wall ? -INFINITY : -(K * ds), B, e, tol, r.tra, de, status);
+    return out && !wall;
 }
 
 /* A ray marched on one grid level to its end, or until its step counter
@@ -349,7 +548,7 @@ __device__ __forceinline__ void march_ahead(const DevCommon &cm, const S &sn, co
     float d0, w0;
     int b0 = RB_ON;
     if (geo_core<XL>(L, r, c0, d0, xul)) b0 = boundary_move<TM>(cm, sn, L, r, e, w0);
-    KbRaw k0 = kb_fetch<TM>(sn, e.row, e.rank, nT, gld32(L.T, c0));
+    KbRaw k0 = kb_fetch<TM>(sn, e.row, e.rank, nT, gld32(L.P, c0));
     /* step i+1 */
     /* The loads are issued on every path (cell 0 for a step not taken,
      * whose values are then never used), so that the number of loads in
@@ -361,7 +560,7 @@ __device__ __forceinline__ void march_ahead(const DevCommon &cm, const S &sn, co
     int b1 = RB_ON;
     if (h1)
         if (geo_core<XL>(L, r, c1, d1, xul)) b1 = boundary_move<TM>(cm, sn, L, r, e, w1);
-    float T1 = gld32(L.T, c1);
+    float p1 = gld32(L.P, c1);
     for (;;) {
         /* step i+2: geometry and temperature */
         const bool h2 = h1 && b1 == RB_ON && counter + 2 < limit;
@@ -370,9 +569,9 @@ __device__ __forceinline__ void march_ahead(const DevCommon &cm, const S &sn, co
         int b2 = RB_ON;
         if (h2)
             if (geo_core<XL>(L, r, c2, d2, xul)) b2 = boundary_move<TM>(cm, sn, L, r, e, w2);
-        const float T2 = gld32(L.T, c2);
+        const float p2 = gld32(L.P, c2);
         /* step i+1: table entries */
-        const KbRaw k1 = kb_fetch<TM>(sn, e.row, e.rank, nT, T1);
+        const KbRaw k1 = kb_fetch<TM>(sn, e.row, e.rank, nT, p1);
         /* step i: absorption, then the end of the ray, if any */
         float K0, B0;
         kb_finish<TM>(k0, K0, B0);
@@ -389,7 +588,7 @@ __device__ __forceinline__ void march_ahead(const DevCommon &cm, const S &sn, co
         }
         if (!(r.tra > tol) || !h1) break;
         k0 = k1; d0 = d1; w0 = w1; b0 = b1;
-        h1 = h2; d1 = d2; w1 = w2; b1 = b2; T1 = T2;
+        h1 = h2; d1 = d2; w1 = w2; b1 = b2; p1 = p2;
     }
 }
 
@@ -477,7 +676,7 @@ __device__ __forceinline__ void march_coop(const DevCommon &cm, const S &sn, con
         /* the absorption terms of the n steps, one per lane (the lanes
          * from n up work on cell 0 and are not read) */
         float K, B;
-        kb_finish<TM>(kb_fetch<TM>(sn, e.row, e.rank, cm.nT, gld32(L.T, mcell)), K, B);
+        kb_finish<TM>(kb_fetch<TM>(sn, e.row, e.rank, cm.nT, gld32(L.P, mcell)), K, B);
         float em1, ex;
         expm1_exp(-(K * mds), em1, ex);
         const float alpha = -em1;
@@ -531,7 +730,6 @@ __device__ __forceinline__ void march_step(const DevCommon &cm, const S &sn, con
  * where one is crossed. */
 __device__ __forceinline__ void boundary_thread(const DevCommon &cm, const DevSnap &sn, const DevLevel &L, Ray &r,
                                                 const En &e, float &de, int &status) {
-    const int nT = cm.nT;
     int face;
     if (r.ic == 0) face = FW;
     else if (r.ic == L.im + 1) face = FE;
@@ -560,15 +758,7 @@ __device__ __forceinline__ void boundary_thread(const DevCommon &cm, const DevSn
         }
     } else if (bd == 2 || bd == 4) {
         float Bw, Kw;
-        if (bd == 4) {
-            const float Tw = gld(L.Tz, (size_t)(face == FT) * L.im * L.jm + (size_t)(r.ic - 1) + (size_t)L.im * (r.jc - 1));
-            kb_at(sn, e.row, e.rank, nT, Tw, Kw, Bw);
-        } else if ((face == FW && cm.adiab[0]) || (face == FE && cm.adiab[1])) {
-            const float Tw = gld(L.Tw, (size_t)(face == FE) * L.jm * L.km + (size_t)(r.jc - 1) + (size_t)L.jm * (r.kc - 1));
-            kb_at(sn, e.row, e.rank, nT, Tw, Kw, Bw);
-        } else {
-            Bw = gld(sn.bfix, (size_t)face * cm.nB + e.rank);
-        }
+        kb_pos(sn, e.row, e.rank, cm.nT, gld(L.P, wall_index(L, r, face)), Kw, Bw);
         wall_hit(e, r, Bw, de);
         status = RC_RAY_WALL;
     } else {
@@ -578,8 +768,8 @@ __device__ __forceinline__ void boundary_thread(const DevCommon &cm, const DevSn
 }
 
 /* one march step of a thread that marches its own ray: march_core's
- * arithmetic with the temperature load issued as soon as the cell index is
- * known, then boundary_thread */
+ * arithmetic with the table-position load issued as soon as the cell index
+ * is known, then boundary_thread */
 __device__ __forceinline__ void march_step_thread(const DevCommon &cm, const DevSnap &sn, const DevLevel &L, Ray &r,
                                                   const En &e, float tol, float &de, int &status) {
     const float fx = gld32(L.xu, min((uint32_t)(r.ic - r.f0), (uint32_t)L.im));
@@ -591,7 +781,7 @@ __device__ __forceinline__ void march_step_thread(const DevCommon &cm, const Dev
     const uint32_t cell = min((uint32_t)(r.ic - 1) + __umul24((uint32_t)L.im, (uint32_t)(r.jc - 1) +
                                                                  __umul24((uint32_t)L.jm, (uint32_t)(r.kc - 1))),
                               (uint32_t)L.ncell - 1u);
-    const float Tc = gld32(L.T, cell);
+    const float pc = gld32(L.P, cell);
     float ds = dsx;
     ds = (ds > dsy) ? dsy : ds;
     ds = (ds > dsz) ? dsz : ds;
@@ -599,7 +789,7 @@ __device__ __forceinline__ void march_step_thread(const DevCommon &cm, const Dev
     r.yp = (ds == dsy) ? fy : fmaf(ds, r.sy, r.yp);
     r.zp = (ds == dsz) ? fz : fmaf(ds, r.sz, r.zp);
     float K, B;
-    kb_at(sn, e.row, e.rank, cm.nT, Tc, K, B);
+    kb_pos(sn, e.row, e.rank, cm.nT, pc, K, B);
     phys_core(K, B, ds, e, tol, r.tra, de, status);
     const bool xa = dsx < dsy && dsx < dsz;
     const bool ya = !(dsx < dsy) && dsy < dsz;

@@ -9,14 +9,14 @@
  *   kernel_surface :1316-1482; tables src/montecarlo/memory_copy_noint.cu.
  *
  * Device data layout (all structure-of-arrays in HBM):
- *   per snapshot and grid level: interior temperatures T_g[km][jm][im]
- *   (level 0 is the interior of the input; coarse levels hold the value the
- *   reference samples with a texture at the mapped cell centre), the
- *   adiabatic x-wall temperatures [2][km][jm] and the bd=4 z-wall
- *   temperatures [2][jm][im]; x faces per level;
- *   tables in sorted band ("rank") order: kb[(rank*nQ+g)*(nT+1)+t] = {k, Ib}
- *   pairs, one 16-byte load per march step, the last row repeated so that the
- *   clamped top row needs no branch.
+ *   per snapshot and grid level: the table positions of the cells
+ *   [km][jm][im] (the temperature the reference samples with a texture at
+ *   the mapped cell centre, mapped to the table's row coordinate) and of
+ *   the black walls (DevLevel); x faces per level;
+ *   tables in sorted band ("rank") order: kb[(rank*nQ+g)*S+t] = {k, Ib}
+ *   pairs, S = tab_stride(nT), one 16-byte load per march step, the last
+ *   row repeated so that the clamped top row needs no branch, then the
+ *   fixed-temperature walls' entries.
  *
  * Variants:
  *   plain   -- the reference's launch shape: one thread per emitting cell,
@@ -79,36 +79,90 @@ __device__ float tex3_T(const float *__restrict__ T, int nx, int ny, int nz, flo
 /* mapped texel coordinate of coarse index ic at ratio r (:334-336) */
 __device__ __forceinline__ float tmap(int ic, int r) { return ((float)ic - 0.5f) * (float)r + 1.0f; }
 
-/* per-level temperatures: cell values, adiabatic x walls, bd=4 z walls */
-__global__ void __launch_bounds__(256) k_level_T(const float *__restrict__ T0, int imax, int jmax, int kmax,
-                                                 int im, int jm, int km, float *__restrict__ Tl,
-                                                 float *__restrict__ Tw, float *__restrict__ Tz) {
+/* how the table position of a black wall face is found: 0 not a black
+ * wall, 1 the field's texture at the wall (adiabatic x walls, bd 4 z
+ * walls), 2 the face's fixed temperature */
+struct FaceModes {
+    int m[6];
+};
+
+FaceModes face_modes(const rc_params &p) {
+    FaceModes f{};
+    for (int s = 0; s < 2; ++s) {
+        const int adiab = s ? p.adiabe : p.adiabw;
+        f.m[FW + s] = p.bd[FW + s] == 2 ? (adiab == 1 ? 1 : 2) : 0;
+        f.m[FS + s] = p.bd[FS + s] == 2 ? 2 : 0;
+        f.m[FB + s] = p.bd[FB + s] == 4 ? 1 : (p.bd[FB + s] == 2 ? 2 : 0);
+    }
+    return f;
+}
+
+/* per-level table positions (layout of DevLevel.P): cells at the texture
+ * sample of the mapped cell centre (:334-338), adiabatic x walls at texel
+ * x = 1.0 / imax + 1.0 (:391-399, :422-430), bd 4 z walls at the adjacent
+ * cell's centre plane (:466-475, :504-513), fixed-temperature walls at
+ * their face's fixed position */
+__global__ void __launch_bounds__(256) k_level_P(const float *__restrict__ T0, int imax, int jmax, int kmax, int im,
+                                                 int jm, int km, FaceModes fm, float Tnb0, float invdT, int nT,
+                                                 float *__restrict__ P) {
     const int rx = imax / im, ry = jmax / jm, rz = kmax / km;
     const int nx = imax + 2, ny = jmax + 2, nz = kmax + 2;
     const size_t ncell = (size_t)im * jm * km;
-    const size_t nw = (size_t)jm * km, nzf = (size_t)im * jm;
-    const size_t total = ncell + (Tw ? 2 * nw : 0) + (Tz ? 2 * nzf : 0);
+    const size_t nfx = (size_t)jm * km, nfy = (size_t)im * km, nfz = (size_t)im * jm;
+    const size_t total = level_pos_count(im, jm, km);
     for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total; id += (size_t)gridDim.x * blockDim.x) {
         if (id < ncell) {
             const int i = (int)(id % im) + 1, j = (int)((id / im) % jm) + 1, k = (int)(id / ((size_t)im * jm)) + 1;
-            Tl[id] = tex3_T(T0, nx, ny, nz, tmap(i, rx), tmap(j, ry), tmap(k, rz));
+            P[id] = table_pos(tex3_T(T0, nx, ny, nz, tmap(i, rx), tmap(j, ry), tmap(k, rz)), Tnb0, invdT, nT);
             continue;
         }
         size_t q = id - ncell;
-        if (Tw && q < 2 * nw) {
-            const int b = (int)(q / nw);
-            const size_t r = q % nw;
+        float v = 0.f;
+        if (q < 2 * nfx) {
+            const int b = (int)(q / nfx);
+            const size_t r = q % nfx;
             const int j = (int)(r % jm) + 1, k = (int)(r / jm) + 1;
-            const float tx = b ? (float)imax + 1.0f : 1.0f;
-            Tw[q] = tex3_T(T0, nx, ny, nz, tx, tmap(j, ry), tmap(k, rz));
-            continue;
+            const int mode = fm.m[FW + b];
+            if (mode == 1) {
+                const float tx = b ? (float)imax + 1.0f : 1.0f;
+                v = table_pos(tex3_T(T0, nx, ny, nz, tx, tmap(j, ry), tmap(k, rz)), Tnb0, invdT, nT);
+            } else if (mode == 2) {
+                v = fixed_wall_pos(nT, FW + b);
+            }
+        } else if ((q -= 2 * nfx) < 2 * nfy) {
+            const int b = (int)(q / nfy);
+            if (fm.m[FS + b] == 2) v = fixed_wall_pos(nT, FS + b);
+        } else {
+            q -= 2 * nfy;
+            const int b = (int)(q / nfz);
+            const size_t r = q % nfz;
+            const int i = (int)(r % im) + 1, j = (int)(r / im) + 1;
+            const int mode = fm.m[FB + b];
+            if (mode == 1) {
+                const float tz = b ? (float)kmax + 0.5f : 1.5f;
+                v = table_pos(tex3_T(T0, nx, ny, nz, tmap(i, rx), tmap(j, ry), tz), Tnb0, invdT, nT);
+            } else if (mode == 2) {
+                v = fixed_wall_pos(nT, FB + b);
+            }
         }
-        if (Tw) q -= 2 * nw;
-        const int b = (int)(q / nzf);
-        const size_t r = q % nzf;
-        const int i = (int)(r % im) + 1, j = (int)(r / im) + 1;
-        const float tz = b ? (float)kmax + 0.5f : 1.5f;
-        Tz[q] = tex3_T(T0, nx, ny, nz, tmap(i, rx), tmap(j, ry), tz);
+        P[id] = v;
+    }
+}
+
+/* cell (and adiabatic x-wall) temperatures of a level, e.g. the spline
+ * restriction's, to table positions in P */
+__global__ void __launch_bounds__(256) k_pos_from_T(const float *__restrict__ Tl, const float *__restrict__ Tw,
+                                                    int im, int jm, int km, FaceModes fm, float Tnb0, float invdT,
+                                                    int nT, float *__restrict__ P) {
+    const size_t ncell = (size_t)im * jm * km, nfx = (size_t)jm * km;
+    const size_t total = ncell + (Tw ? 2 * nfx : 0);
+    for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total; id += (size_t)gridDim.x * blockDim.x) {
+        if (id < ncell) {
+            P[id] = table_pos(Tl[id], Tnb0, invdT, nT);
+        } else {
+            const size_t q = id - ncell;
+            if (fm.m[FW + (int)(q / nfx)] == 1) P[id] = table_pos(Tw[q], Tnb0, invdT, nT);
+        }
     }
 }
 
@@ -278,11 +332,20 @@ __global__ void k_finalize(const double *__restrict__ de, int n, int nVar, float
 /* ------------------------------------------------------------------------ */
 /* launchers used by the session code                                        */
 
-void launch_level_T(hipStream_t st, const float *T0, const rc_params &p, int g, float *Tl, float *Tw, float *Tz) {
-    const size_t n = (size_t)p.maxi[g] * p.maxj[g] * p.maxk[g];
+void launch_level_P(hipStream_t st, const float *T0, const rc_params &p, int g, float Tnb0, float invdT, int nT,
+                    float *P) {
+    const size_t n = level_pos_count(p.maxi[g], p.maxj[g], p.maxk[g]);
     const int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
-    hipLaunchKernelGGL(k_level_T, dim3(blocks), dim3(256), 0, st, T0, p.imax, p.jmax, p.kmax, p.maxi[g], p.maxj[g],
-                       p.maxk[g], Tl, Tw, Tz);
+    hipLaunchKernelGGL(k_level_P, dim3(blocks), dim3(256), 0, st, T0, p.imax, p.jmax, p.kmax, p.maxi[g], p.maxj[g],
+                       p.maxk[g], face_modes(p), Tnb0, invdT, nT, P);
+}
+
+void launch_pos_from_T(hipStream_t st, const rc_params &p, int g, const float *Tl, const float *Tw, float Tnb0,
+                       float invdT, int nT, float *P) {
+    const size_t n = (size_t)p.maxi[g] * p.maxj[g] * p.maxk[g] + (size_t)2 * p.maxj[g] * p.maxk[g];
+    const int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
+    hipLaunchKernelGGL(k_pos_from_T, dim3(blocks), dim3(256), 0, st, Tl, Tw, p.maxi[g], p.maxj[g], p.maxk[g],
+                       face_modes(p), Tnb0, invdT, nT, P);
 }
 
 void launch_sort(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v, int surface, int nunit,
