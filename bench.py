@@ -258,6 +258,16 @@ def field_legs(args, lib, p, tab, T, bf16: bool, steps: int, warmup: int, plain_
         plain = {"rays_per_s": sp["rays"] / pwall, "ms_per_step": 1e3 * pwall, "eps": eps_p,
                  "time_to_1pct_s": pwall * (eps_p / 0.01) ** 2, "timed_solves": plain_steps}
         log(f"plain:   {1e3 * pwall:.2f} ms/step, {sp['rays'] / pwall / 1e9:.3f} Grays/s, eps {eps_p:.4g}")
+    bands = None
+    if not args.no_plain and args.plain_bands_steps > 0 and p.srt == 1 and not p.grey:
+        from radchan_amd import VARIANT_PLAIN_BANDS
+        s.set_table_precision(False)
+        bwall, sb, _ = timed_leg(s, lib, VARIANT_PLAIN_BANDS, args.plain_bands_steps, 0)
+        eps_b, _ = std_error(s.tallies())
+        bands = {"rays_per_s": sb["rays"] / bwall, "ms_per_step": 1e3 * bwall, "eps": eps_b,
+                 "time_to_1pct_s": bwall * (eps_b / 0.01) ** 2, "timed_solves": args.plain_bands_steps,
+                 "launches_per_step": sb["launches"]}
+        log(f"plain, one launch per band: {1e3 * bwall:.2f} ms/step ({sb['launches']} launches), eps {eps_b:.4g}")
     ncell = s.ncell
     s.close()
 
@@ -281,9 +291,12 @@ def field_legs(args, lib, p, tab, T, bf16: bool, steps: int, warmup: int, plain_
     out = {"workload": f"narrow-band correlated-k OERM {p.imax}x{p.jmax}x{p.kmax}, {p.nB} bands x {p.nQ} g x "
                        f"{p.nT} T, photmax {p.photmax}, nVar {p.nVar}, srt {p.srt}, grid_num {p.grid_num}"
                        if not p.grey else f"grey OERM {p.imax}x{p.jmax}x{p.kmax}, photmax {p.photmax}, nVar {p.nVar}",
-           "cells": ncell, "schedule": sched, "head": head, "batched": legs, "plain": plain, "roofline": roof,
+           "cells": ncell, "schedule": sched, "head": head, "batched": legs, "plain": plain,
+           "plain_one_launch_per_band": bands, "roofline": roof,
            "speedup_vs_plain": (head["rays_per_s"] / plain["rays_per_s"]) if plain else None,
-           "time_to_1pct_speedup_vs_plain": (plain["time_to_1pct_s"] / head["time_to_1pct_s"]) if plain else None}
+           "time_to_1pct_speedup_vs_plain": (plain["time_to_1pct_s"] / head["time_to_1pct_s"]) if plain else None,
+           "time_to_1pct_speedup_vs_plain_per_band": (bands["time_to_1pct_s"] / head["time_to_1pct_s"])
+           if bands else None}
     if scaling:
         sc = eps_scaling(p, tab, T, sched, head["eps"], bf16)
         log(f"eps at photmax {sc['photmax']}: {sc['eps']:.4g} (ratio {sc['eps_ratio']:.3f}, expected 0.5), "
@@ -513,6 +526,8 @@ def main() -> int:
                     help="k-table storage of the headline (batched) leg; both are measured")
     ap.add_argument("--plain-steps", type=int, default=None,
                     help="timed solves of the plain solver (default 2 on fields of 4 M cells or more, else 3)")
+    ap.add_argument("--plain-bands-steps", type=int, default=1,
+                    help="timed solves of the plain solver launched once per sorted band (0: skip)")
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
@@ -648,6 +663,8 @@ def main() -> int:
         "plain": main_leg["plain"],
         "speedup_vs_plain": main_leg["speedup_vs_plain"],
         "time_to_1pct_speedup_vs_plain": main_leg["time_to_1pct_speedup_vs_plain"],
+        "plain_one_launch_per_band": main_leg["plain_one_launch_per_band"],
+        "time_to_1pct_speedup_vs_plain_per_band": main_leg["time_to_1pct_speedup_vs_plain_per_band"],
         "eps_scaling": main_leg.get("eps_scaling"),
         "time_to_1pct_measured": main_leg.get("time_to_1pct_measured"),
         "nb64": second,

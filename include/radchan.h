@@ -92,6 +92,9 @@ typedef struct rc_ray {
 /* Solver variants */
 #define RC_VARIANT_PLAIN   0   /* thread per emitting cell, FP32 per-cell sums  */
 #define RC_VARIANT_BATCHED 1   /* persistent wavefront work queue, FP64 tallies */
+#define RC_VARIANT_PLAIN_BANDS 2 /* the plain variant launched once per sorted  */
+                                 /* band and run (srt=1 narrow band only): the  */
+                                 /* same sums as RC_VARIANT_PLAIN               */
 
 typedef struct rc_stats {
     double   kernel_ms;         /* device time of the last run (HIP events)   */

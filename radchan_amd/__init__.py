@@ -7,12 +7,12 @@ this package binds it and provides parameter sets, synthetic tables and
 fields.
 """
 from . import lib
-from .lib import VARIANT_BATCHED, VARIANT_PLAIN, Params
+from .lib import VARIANT_BATCHED, VARIANT_PLAIN, VARIANT_PLAIN_BANDS, Params
 from .params import base_params, config, reference_params
 from .solver import Group, Solver, get_results, mc_gpu, set_dropin_modes, set_params, set_table_dir
 from .tables import Tables, synthetic_nb
 
 __all__ = [
-    "lib", "Params", "VARIANT_PLAIN", "VARIANT_BATCHED", "base_params", "config", "reference_params",
+    "lib", "Params", "VARIANT_PLAIN", "VARIANT_BATCHED", "VARIANT_PLAIN_BANDS", "base_params", "config", "reference_params",
     "Solver", "Group", "mc_gpu", "get_results", "set_params", "set_table_dir", "set_dropin_modes", "Tables", "synthetic_nb",
 ]

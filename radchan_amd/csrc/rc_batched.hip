@@ -616,7 +616,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                     const bool out = march_core_bulk<TM>(cm, sv, L, xb, xo, xmax, pb, po, pmax, r, e, tol, dacc, kt,
                                                          kc, ws, widx, cw);
                     boundary_defer(cm, L, r, active && out, wall, widx, cw);
-                    counter += ws ? 0 : 1;
+                    counter += 1;           /* a wall step ends the ray: its count is never read */
                     bsteps += __popcll(__ballot(active && !ws));
                     const bool alive = r.tra > tol;
                     const bool done = active && (!alive || (counter >= L.maxs && wall == 0));

@@ -124,6 +124,7 @@ struct rc_ctx {
     DevBuf dsnap;
     DevBuf de, ds, mean, sdev, smean, ssdev;
     DevBuf keys, stats, queue;
+    DevBuf bacc;                            /* per-band plain variant: FP32 cell sums */
     DevBuf tr_rays, tr_unit, tr_ulist, tr_out;
     rc_stats last{};
     bool launched = false;
@@ -678,14 +679,20 @@ static int enqueue_solve(rc_ctx *c, int variant, int *launches, bool timed) {
     launch_zero_u64(c->stream, c->stats.as<unsigned long long>(), 8);
     ++n;
     if (timed) RC_HIP(hipEventRecord(c->evm0, c->stream));
-    if (variant == RC_VARIANT_PLAIN) {
+    if (variant == RC_VARIANT_PLAIN || variant == RC_VARIANT_PLAIN_BANDS) {
         for (int s = 0; s < nsnap; ++s) {
             double *de = c->de.as<double>() + (size_t)s * c->ncell * p.nVar;
             for (int v = 0; v < p.nVar; ++v) {
                 if (cm.srt) { launch_sort(c->stream, cm, ds, s, v, 0, c->ncell, nullptr, c->keys.as<uint16_t>()); ++n; }
-                launch_fluid_plain(c->stream, cm, ds, s, v, cm.srt ? c->keys.as<uint16_t>() : nullptr, de,
-                                   c->stats.as<unsigned long long>());
-                ++n;
+                if (variant == RC_VARIANT_PLAIN_BANDS) {
+                    launch_fluid_bands(c->stream, cm, ds, s, v, c->keys.as<uint16_t>(), c->bacc.as<float>(), de,
+                                       c->stats.as<unsigned long long>());
+                    n += p.nB + 2;
+                } else {
+                    launch_fluid_plain(c->stream, cm, ds, s, v, cm.srt ? c->keys.as<uint16_t>() : nullptr, de,
+                                       c->stats.as<unsigned long long>());
+                    ++n;
+                }
             }
             if (walls) {
                 double *dw = c->ds.as<double>() + (size_t)s * 2 * c->nsurf * p.nVar;
@@ -727,12 +734,21 @@ static int enqueue_solve(rc_ctx *c, int variant, int *launches, bool timed) {
 
 int rc_run(rc_ctx *c, int variant) {
     if (!c || c->snaps.empty()) { set_error("rc_run before rc_set_fields"); return -1; }
-    if (variant != RC_VARIANT_PLAIN && variant != RC_VARIANT_BATCHED) { set_error("unknown variant"); return -2; }
+    if (variant != RC_VARIANT_PLAIN && variant != RC_VARIANT_BATCHED && variant != RC_VARIANT_PLAIN_BANDS) {
+        set_error("unknown variant");
+        return -2;
+    }
     const rc_params &p = c->p;
-    if (variant == RC_VARIANT_PLAIN && p.srt == 1 && !p.grey) {
+    if (variant == RC_VARIANT_PLAIN_BANDS && !(p.srt == 1 && !p.grey)) {
+        set_error("the per-band plain variant needs sorted band keys (srt=1, narrow band)");
+        return -2;
+    }
+    if ((variant == RC_VARIANT_PLAIN || variant == RC_VARIANT_PLAIN_BANDS) && p.srt == 1 && !p.grey) {
         const size_t nk = (size_t)std::max(c->ncell, 2 * c->nsurf) * p.photmax;
         if (int e = c->keys.ensure(nk * 2)) return e;
     }
+    if (variant == RC_VARIANT_PLAIN_BANDS)
+        if (int e = c->bacc.ensure((size_t)c->ncell * 4)) return e;
     int launches = 0;
     const bool graph = c->use_graph && variant == RC_VARIANT_BATCHED;
     if (graph && !c->graph_exec) {
@@ -1250,7 +1266,10 @@ int rc_set_table_dir(const char *dir) {
 }
 
 int rc_set_variant(int variant) {
-    if (variant != RC_VARIANT_PLAIN && variant != RC_VARIANT_BATCHED) { set_error("unknown variant"); return -1; }
+    if (variant != RC_VARIANT_PLAIN && variant != RC_VARIANT_BATCHED && variant != RC_VARIANT_PLAIN_BANDS) {
+        set_error("unknown variant");
+        return -1;
+    }
     std::lock_guard<std::mutex> lk(g_mu);
     g_variant = variant;
     return 0;

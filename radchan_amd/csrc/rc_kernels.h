@@ -113,6 +113,10 @@ void launch_sort(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int 
                  const int32_t *ulist, uint16_t *keys);
 void launch_fluid_plain(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v, const uint16_t *keys,
                         double *de, unsigned long long *stats);
+/* the plain variant in one launch per sorted band (srt keys required):
+ * FP32 cell sums in acc (ncell floats), stored to de[cell][v] */
+void launch_fluid_bands(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v, const uint16_t *keys,
+                        float *acc, double *de, unsigned long long *stats);
 void launch_surface_plain(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v,
                           const uint16_t *keys, double *de, unsigned long long *stats);
 void launch_trace(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int surface, const int32_t *rays,

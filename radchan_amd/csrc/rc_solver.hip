@@ -255,6 +255,56 @@ __global__ void __launch_bounds__(128) k_fluid_plain(DevCommon cm, const DevSnap
     }
 }
 
+/* plain variant, one launch per band: the rays of sorted band `rank` of
+ * every emitting cell (a contiguous range of the cell's sorted keys), added
+ * to the cell's FP32 sum acc[n]; launched for every band in rank order, it
+ * adds each cell's rays in the order k_fluid_plain does, so the sums are
+ * the same to the bit */
+__global__ void __launch_bounds__(128) k_fluid_band(DevCommon cm, const DevSnap *__restrict__ snaps, int s, int v,
+                                                    int rank, const uint16_t *__restrict__ keys,
+                                                    float *__restrict__ acc, unsigned long long *__restrict__ stats) {
+    const DevSnap &sn = snaps[s];
+    const int P = cm.photmax;
+    const uint32_t klo = (uint32_t)rank * (uint32_t)cm.nQ, khi = klo + (uint32_t)cm.nQ;
+    int64_t steps = 0, rays = 0;
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < cm.ncell; n += gridDim.x * blockDim.x) {
+        const uint16_t *kk = keys + (size_t)n * P;
+        /* first ray with key >= klo, and with key >= khi */
+        int a = 0, b = P;
+        while (a < b) { const int m = (a + b) >> 1; if ((uint32_t)kk[m] < klo) a = m + 1; else b = m; }
+        int c = a, d = P;
+        while (c < d) { const int m = (c + d) >> 1; if ((uint32_t)kk[m] < khi) c = m + 1; else d = m; }
+        if (a == c) continue;
+        int i, j, k;
+        cell_ijk(cm, n, i, j, k);
+        const uint32_t cid = (uint32_t)n + (uint32_t)cm.imax * cm.jmax * (cm.zstart - 1);
+        const float Ti = cell_T(cm, sn, i, j, k);
+        float sum = acc[n];
+        int st = 0;
+        for (int h = a; h < c; ++h) sum += fluid_ray<false>(cm, sn, i, j, k, cid, v, h, (uint32_t)kk[h], true, Ti, st, nullptr);
+        acc[n] = sum;
+        steps += st;
+        rays += c - a;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        steps += __shfl_xor(steps, o);
+        rays += __shfl_xor(rays, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], (unsigned long long)rays);
+        atomicAdd(&stats[1], (unsigned long long)steps);
+    }
+}
+
+__global__ void k_acc_zero(float *__restrict__ acc, int n) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x) acc[i] = 0.f;
+}
+
+__global__ void k_acc_store(const float *__restrict__ acc, int n, int nVar, int v, double *__restrict__ de) {
+    for (int i = blockIdx.x * blockDim.x + threadIdx.x; i < n; i += gridDim.x * blockDim.x)
+        de[(size_t)i * nVar + v] = (double)acc[i];
+}
+
 __global__ void __launch_bounds__(128) k_surface_plain(DevCommon cm, const DevSnap *__restrict__ snaps, int s, int v,
                                                        const uint16_t *__restrict__ keys, double *__restrict__ de,
                                                        unsigned long long *__restrict__ stats) {
@@ -361,6 +411,16 @@ void launch_fluid_plain(hipStream_t st, const DevCommon &cm, const DevSnap *snap
                         double *de, unsigned long long *stats) {
     const int blocks = std::max(1, std::min((cm.ncell + 127) / 128, 1 << 20));
     hipLaunchKernelGGL(k_fluid_plain, dim3(blocks), dim3(128), 0, st, cm, snaps, s, v, keys, de, stats);
+}
+
+void launch_fluid_bands(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v, const uint16_t *keys,
+                        float *acc, double *de, unsigned long long *stats) {
+    const int blocks = std::max(1, std::min((cm.ncell + 127) / 128, 1 << 20));
+    const int zb = std::max(1, std::min((cm.ncell + 255) / 256, 65536));
+    hipLaunchKernelGGL(k_acc_zero, dim3(zb), dim3(256), 0, st, acc, cm.ncell);
+    for (int rank = 0; rank < cm.nB; ++rank)
+        hipLaunchKernelGGL(k_fluid_band, dim3(blocks), dim3(128), 0, st, cm, snaps, s, v, rank, keys, acc, stats);
+    hipLaunchKernelGGL(k_acc_store, dim3(zb), dim3(256), 0, st, acc, cm.ncell, cm.nVar, v, de);
 }
 
 void launch_surface_plain(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v,

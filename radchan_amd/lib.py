@@ -17,7 +17,7 @@ MAXLEV = 9
 FACES = ("W", "E", "S", "N", "B", "T")
 
 RAY_ABSORBED, RAY_WALL, RAY_LOST, RAY_CAPPED, RAY_STOPPED = range(5)
-VARIANT_PLAIN, VARIANT_BATCHED = 0, 1
+VARIANT_PLAIN, VARIANT_BATCHED, VARIANT_PLAIN_BANDS = 0, 1, 2
 
 
 class Params(C.Structure):

@@ -285,3 +285,21 @@ def test_group_with_wall_fluxes(nb_tables):
     g.close()
     s1.close()
     s2.close()
+
+
+def test_plain_one_launch_per_band_equals_plain(nb_tables):
+    """The plain variant launched once per sorted band (and run) adds every
+    cell's rays in the same order as the one-launch plain variant: equal
+    tallies bit for bit (single grid, and three levels with walls)."""
+    from radchan_amd import VARIANT_PLAIN_BANDS
+    for p, tab, T in (cases.nb_case(nb_tables, n=12, photmax=48), cases.mg_case(nb_tables)):
+        print(f"per-band plain: {p.imax}x{p.jmax}x{p.kmax} grid_num={p.grid_num}", flush=True)
+        s = Solver(p, tab)
+        s.set_fields([T])
+        st0 = s.solve(VARIANT_PLAIN)
+        a, aw = s.tallies(), s.surface()[2]
+        st1 = s.solve(VARIANT_PLAIN_BANDS)
+        assert np.array_equal(s.tallies(), a) and np.array_equal(s.surface()[2], aw)
+        assert st1["rays"] == st0["rays"] and st1["steps"] == st0["steps"]
+        assert st1["launches"] > p.nB * p.nVar
+        s.close()
