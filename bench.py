@@ -250,6 +250,17 @@ def field_legs(args, lib, p, tab, T, bf16: bool, steps: int, warmup: int, plain_
     _, st, march = timed_leg(s, lib, VARIANT_BATCHED, max(3, steps), 1)
     rays, nsteps = st["rays"], st["steps"]
 
+    det = None
+    if not args.no_deterministic:
+        # the deterministic reduction: per-ray tallies in HBM, summed in a fixed order
+        s.set_reduction(1)
+        dwall, sd, _ = timed_leg(s, lib, VARIANT_BATCHED, 2, 1)
+        s.set_reduction(0)
+        det = {"ms_per_step": 1e3 * dwall, "rays_per_s": sd["rays"] / dwall,
+               "overhead_vs_atomic": dwall * 1e3 / head["ms_per_step"] - 1.0, "timed_solves": 2,
+               "ray_tally_bytes": int(sd["rays"]) * 4}
+        log(f"batched, deterministic reduction: {1e3 * dwall:.2f} ms/step")
+
     plain = None
     if not args.no_plain and plain_steps > 0:
         s.set_table_precision(False)
@@ -292,7 +303,7 @@ def field_legs(args, lib, p, tab, T, bf16: bool, steps: int, warmup: int, plain_
                        f"{p.nT} T, photmax {p.photmax}, nVar {p.nVar}, srt {p.srt}, grid_num {p.grid_num}"
                        if not p.grey else f"grey OERM {p.imax}x{p.jmax}x{p.kmax}, photmax {p.photmax}, nVar {p.nVar}",
            "cells": ncell, "schedule": sched, "head": head, "batched": legs, "plain": plain,
-           "plain_one_launch_per_band": bands, "roofline": roof,
+           "plain_one_launch_per_band": bands, "deterministic_reduction": det, "roofline": roof,
            "speedup_vs_plain": (head["rays_per_s"] / plain["rays_per_s"]) if plain else None,
            "time_to_1pct_speedup_vs_plain": (plain["time_to_1pct_s"] / head["time_to_1pct_s"]) if plain else None,
            "time_to_1pct_speedup_vs_plain_per_band": (bands["time_to_1pct_s"] / head["time_to_1pct_s"])
@@ -531,6 +542,8 @@ def main() -> int:
     ap.add_argument("--cpu-seconds", type=float, default=15.0)
     ap.add_argument("--no-cpu", action="store_true")
     ap.add_argument("--no-plain", action="store_true")
+    ap.add_argument("--no-deterministic", action="store_true",
+                    help="skip the batched solve with the deterministic reduction")
     ap.add_argument("--no-autotune", action="store_true")
     ap.add_argument("--no-graph", action="store_true")
     ap.add_argument("--no-ckdist", action="store_true", help="skip the correlated-k generator leg")
@@ -664,6 +677,7 @@ def main() -> int:
         "speedup_vs_plain": main_leg["speedup_vs_plain"],
         "time_to_1pct_speedup_vs_plain": main_leg["time_to_1pct_speedup_vs_plain"],
         "plain_one_launch_per_band": main_leg["plain_one_launch_per_band"],
+        "deterministic_reduction": main_leg["deterministic_reduction"],
         "time_to_1pct_speedup_vs_plain_per_band": main_leg["time_to_1pct_speedup_vs_plain_per_band"],
         "eps_scaling": main_leg.get("eps_scaling"),
         "time_to_1pct_measured": main_leg.get("time_to_1pct_measured"),

@@ -277,6 +277,16 @@ int rc_set_restriction(rc_ctx *c, int mode);
  * Narrow-band runs only (grey runs are unaffected); applies to fields loaded
  * by later rc_set_fields calls. */
 int rc_set_estimator(rc_ctx *c, int mode);
+/* summation of the batched variant's tallies (no counterpart in the
+ * reference): 0 = each ray's FP32 tally added to its unit's FP64 total as
+ * the ray ends (LDS and FP64 atomics; the order, and so the last bits,
+ * vary from run to run; default); 1 = deterministic: every ray's tally
+ * stored at its own entry (4 bytes per ray in HBM: photmax nVar per
+ * emitting cell), then summed per (cell, run) in a fixed order with
+ * compensated FP64 arithmetic -- the same bits every run and for every
+ * schedule.  photmax <= 65535.  The plain variant is always deterministic;
+ * groups use mode 0. */
+int rc_set_reduction(rc_ctx *c, int mode);
 /* time the batched solve of the loaded snapshots over schedules (steps per
  * iteration, then workgroups per CU, then refill threshold, each at the
  * best of the previous) and keep the fastest */

@@ -234,17 +234,18 @@ __device__ __forceinline__ Slots slot_view(uint32_t *lds) {
 }
 
 __device__ __forceinline__ void q_put(uint32_t *q, int qs, float sx, float sy, float sz, const En &e, uint32_t key,
-                                      int unit, int tid) {
+                                      int unit, int tid, int h) {
     uint4 *rec = reinterpret_cast<uint4 *>(q + qs * QREC);
     rec[0] = make_uint4(__float_as_uint(sx), __float_as_uint(sy), __float_as_uint(sz), __float_as_uint(1.0f / sx));
     rec[1] = make_uint4(__float_as_uint(1.0f / sy), __float_as_uint(1.0f / sz), __float_as_uint(e.W),
                         __float_as_uint(e.Bi));
-    rec[2] = make_uint4(__float_as_uint(e.invBi), key | ((uint32_t)e.rank << 16), (uint32_t)unit, (uint32_t)tid);
+    rec[2] = make_uint4(__float_as_uint(e.invBi), key | ((uint32_t)e.rank << 16),
+                        ((uint32_t)unit & 0xFFFFu) | ((uint32_t)h << 16), (uint32_t)tid);
 }
 
 template <bool SURF>
 __device__ __forceinline__ void q_get(const uint32_t *q, int qs, const Slots &sl, const DevCommon &cm,
-                                      const DevSnap &sn, Ray &r, En &e, int &unit, int &tid) {
+                                      const DevSnap &sn, Ray &r, En &e, int &unit, int &tid, int &h) {
     const uint4 *rec = reinterpret_cast<const uint4 *>(q + qs * QREC);
     const uint4 a = rec[0], b = rec[1], c = rec[2];
     r.sx = __uint_as_float(a.x);
@@ -260,7 +261,8 @@ __device__ __forceinline__ void q_get(const uint32_t *q, int qs, const Slots &sl
     e.invBi = __uint_as_float(c.x);
     e.row = c.y & 0xFFFFu;
     e.rank = (int)(c.y >> 16);
-    unit = (int)c.z;
+    unit = (int)(c.z & 0xFFFFu);
+    h = (int)(c.z >> 16);
     tid = (int)c.w;
     const uint4 *pr = reinterpret_cast<const uint4 *>(&sl.pl[unit & (NSLOT - 1)]);
     const uint4 p0 = pr[0], p1 = pr[1];
@@ -269,11 +271,18 @@ __device__ __forceinline__ void q_get(const uint32_t *q, int qs, const Slots &sl
     if (SURF) surface_adjust(sn, r);
 }
 
+/* the distance from unit number u (16 bits, as the ray records hold it)
+ * to the wavefront's unit count seq */
+__device__ __forceinline__ int unit_age(int seq, int u) { return (int)((uint32_t)(seq - u) & 0xFFFFu); }
+
 /* a ray's end: its tally and count go to its unit's slot, or to HBM once
- * the slot has been taken over (seq: units the wavefront has started) */
-__device__ __forceinline__ void finish_ray(const Slots &sl, int seq, int runit, int rtid, float dacc,
-                                           double *__restrict__ de) {
-    if (seq - runit <= NSLOT) {
+ * the slot has been taken over (seq: units the wavefront has started); in
+ * the deterministic reduction, the tally goes to the ray's own entry */
+__device__ __forceinline__ void finish_ray(const DevCommon &cm, const Slots &sl, int seq, int runit, int rtid, int h,
+                                           float dacc, double *__restrict__ de) {
+    if (cm.ray_de) {
+        cm.ray_de[(size_t)rtid * (size_t)cm.photmax + (size_t)h] = dacc;
+    } else if (unit_age(seq, runit) <= NSLOT) {
         SlotTally &t = sl.tal[runit & (NSLOT - 1)];
         atomicAdd(&t.acc, (double)dacc);
         atomicAdd(&t.done, 1);
@@ -382,7 +391,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
     e.Bi = 1.f;
     e.invBi = 1.f;
     float dacc = 0.f;
-    int runit = 0, rtid = 0, lev = 0, counter = 0;
+    int runit = 0, rtid = 0, rh = 0, lev = 0, counter = 0;
     int kt = -1;                  /* the lane's cached table pair (t, values) */
     KbRaw kc;
     kc.q = make_float4(0.f, 0.f, 0.f, 0.f);
@@ -485,7 +494,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
         }
         const unsigned long long live = __ballot(active);
         if (cur_h >= P && nxt >= 0 && (nxt_s == snap || (live == 0ull && qcount == 0)) &&
-            (qcount == 0 || seq - uni((int)rq[qhead * QREC + 10]) < NSLOT)) {
+            (qcount == 0 || unit_age(seq, uni((int)(rq[qhead * QREC + 10] & 0xFFFFu))) < NSLOT)) {
             /* (b) start the prepared unit (a new snapshot only once every
              * lane and the queue are empty; never over the slot of a ray
              * still queued) */
@@ -506,7 +515,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                 if (lane == 0) {
                     const DevSnap &sn = snaps[U.s];
                     SlotTally &t = sl.tal[cslot];
-                    if (seq >= NSLOT) {
+                    if (seq >= NSLOT && !cm.ray_de) {
                         if (t.done == P) de[t.tid] = t.acc;
                         else atomicAdd(&de[t.tid], t.acc);
                     }
@@ -557,7 +566,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                     else fluid_dir(a, b, sx, sy, sz);
                     En ge;
                     energy<TM>(cm, sn, key, U.Ti, SURF, U.bound, ge);
-                    q_put(rq, qwrap(qhead + qcount + lane), sx, sy, sz, ge, key, cunit, ctid);
+                    q_put(rq, qwrap(qhead + qcount + lane), sx, sy, sz, ge, key, cunit, ctid, h);
                 }
                 __syncthreads();
                 qcount += ng;
@@ -570,7 +579,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                 const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
                 const int npop = min(nidle, qcount);
                 if (!active && rank < npop) {
-                    q_get<SURF>(rq, qwrap(qhead + rank), sl, cm, snaps[snap], r, e, runit, rtid);
+                    q_get<SURF>(rq, qwrap(qhead + rank), sl, cm, snaps[snap], r, e, runit, rtid, rh);
                     active = true;
                     dacc = 0.f;
                     lev = 0;
@@ -623,7 +632,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                     const bool fin = done && (!MULTI || !alive || lev == cm.grid_num - 1);
                     const bool crs = MULTI && done && !fin;
                     if (fin) {
-                        finish_ray(sl, seq, runit, rtid, dacc, de);
+                        finish_ray(cm, sl, seq, runit, rtid, rh, dacc, de);
                         nrays += 1;
                         active = false;
                         wall = 0;
@@ -665,7 +674,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
         float Kw, Bw;
         kb_pos<TM>(sv, e.row, e.rank, cm.nT, gld(lp->P, widx), Kw, Bw);
         wall_hit(e, r, Bw, dacc);
-        finish_ray(sl, seq, runit, rtid, dacc, de);
+        finish_ray(cm, sl, seq, runit, rtid, rh, dacc, de);
         nrays += 1;
         active = false;
         wall = 0;
@@ -701,7 +710,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                     lev += 1;
                     counter = 0;
                 }
-                finish_ray(sl, seq, runit, rtid, dacc, de);
+                finish_ray(cm, sl, seq, runit, rtid, rh, dacc, de);
                 nrays += 1;
                 active = false;
             }
@@ -714,7 +723,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                         march_ahead<TM, true>(cm, sv, l0, r, e, tol, dacc, counter, nsteps, xul,
                                               min(l0.maxs, counter + DRAIN_BLOCK));
                         if (!(r.tra > tol) || counter >= l0.maxs) {
-                            finish_ray(sl, seq, runit, rtid, dacc, de);
+                            finish_ray(cm, sl, seq, runit, rtid, rh, dacc, de);
                             nrays += 1;
                             active = false;
                         }
@@ -728,7 +737,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                     int cc = bcast(counter, src), cs = 0;
                     march_coop<TM>(cm, sv, l0, cr, ce, tol, cd, cc, cs, xul, lane);
                     if (lane == src) {
-                        finish_ray(sl, seq, runit, rtid, cd, de);
+                        finish_ray(cm, sl, seq, runit, rtid, rh, cd, de);
                         nsteps += cs;
                         nrays += 1;
                         active = false;
@@ -739,7 +748,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
     }
     /* the last units' totals (every ray has finished) */
     __syncthreads();
-    if (lane < NSLOT && lane < seq) {
+    if (lane < NSLOT && lane < seq && !cm.ray_de) {
         const SlotTally &t = sl.tal[lane];
         if (t.done == P) de[t.tid] = t.acc;
         else atomicAdd(&de[t.tid], t.acc);
@@ -790,6 +799,41 @@ __global__ void k_zero_f64(double *p, size_t n) {
 __global__ void k_zero_u64(unsigned long long *p, int n) {
     const int i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i < n) p[i] = 0ull;
+}
+
+/* one wavefront per unit: every lane sums its rays h = lane, lane + 64, ...
+ * in order (Neumaier), then the lanes' (sum, error) pairs are combined by
+ * a butterfly of TwoSums; the same operations every run, whatever the
+ * order the rays ended in */
+__global__ void __launch_bounds__(256) k_reduce_rays(const float *__restrict__ ray, size_t nunit, int P,
+                                                     double *__restrict__ de) {
+    const int lane = threadIdx.x & 63;
+    const size_t w0 = (blockIdx.x * (size_t)blockDim.x + threadIdx.x) >> 6;
+    const size_t nw = ((size_t)gridDim.x * blockDim.x) >> 6;
+    for (size_t u = w0; u < nunit; u += nw) {
+        const float *x = ray + u * (size_t)P;
+        double s = 0.0, c = 0.0;
+        for (int h = lane; h < P; h += 64) {
+            const double v = (double)x[h];
+            const double t = s + v;
+            c += fabs(s) >= fabs(v) ? (s - t) + v : (v - t) + s;
+            s = t;
+        }
+        for (int o = 1; o < 64; o <<= 1) {
+            const double s2 = __shfl_xor(s, o), c2 = __shfl_xor(c, o);
+            const double t = s + s2;
+            const double bp = t - s;
+            const double e = (s - (t - bp)) + (s2 - bp);
+            c = c + c2 + e;
+            s = t;
+        }
+        if (lane == 0) de[u] = s + c;
+    }
+}
+
+void launch_reduce_rays(hipStream_t st, const float *ray, size_t nunit, int photmax, double *de) {
+    const int blocks = (int)std::max<size_t>(1, std::min<size_t>((nunit + 3) / 4, 65536));
+    hipLaunchKernelGGL(k_reduce_rays, dim3(blocks), dim3(256), 0, st, ray, nunit, photmax, de);
 }
 
 void launch_zero_u64(hipStream_t st, unsigned long long *p, int n) {

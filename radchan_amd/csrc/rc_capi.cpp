@@ -125,6 +125,8 @@ struct rc_ctx {
     DevBuf de, ds, mean, sdev, smean, ssdev;
     DevBuf keys, stats, queue;
     DevBuf bacc;                            /* per-band plain variant: FP32 cell sums */
+    int reduction = 0;                      /* batched tallies: 0 FP64 atomics, 1 deterministic */
+    DevBuf rayf, rayw;                      /* deterministic reduction: per-ray tallies */
     DevBuf tr_rays, tr_unit, tr_ulist, tr_out;
     rc_stats last{};
     bool launched = false;
@@ -351,6 +353,7 @@ DevCommon make_common(const rc_ctx *c) {
     for (int f = 0; f < 6; ++f) cm.bd[f] = p.bd[f];
     cm.adiab[0] = p.adiabw == 1; cm.adiab[1] = p.adiabe == 1;
     cm.k0 = (uint32_t)p.seed; cm.k1 = (uint32_t)(p.seed >> 32);
+    cm.ray_de = nullptr;
     return cm;
 }
 
@@ -705,14 +708,28 @@ static int enqueue_solve(rc_ctx *c, int variant, int *launches, bool timed) {
             }
         }
     } else {
-        launch_batched(c->stream, cm, ds, nsnap, nsnap * c->ncell, 0, c->table_bf16, c->de.as<double>(),
+        DevCommon cf = cm, cw = cm;
+        if (c->reduction == 1) {
+            cf.ray_de = c->rayf.as<float>();
+            cw.ray_de = c->rayw.as<float>();
+        }
+        launch_batched(c->stream, cf, ds, nsnap, nsnap * c->ncell, 0, c->table_bf16, c->de.as<double>(),
                        c->stats.as<unsigned long long>(), c->queue.as<int>(), c->wg_per_cu, c->threshold, c->spi);
         n += 3;
+        if (c->reduction == 1) {
+            launch_reduce_rays(c->stream, cf.ray_de, (size_t)nsnap * c->ncell * p.nVar, p.photmax, c->de.as<double>());
+            ++n;
+        }
         if (walls) {
-            launch_batched(c->stream, cm, ds, nsnap, nsnap * 2 * c->nsurf, 1, c->table_bf16, c->ds.as<double>(),
+            launch_batched(c->stream, cw, ds, nsnap, nsnap * 2 * c->nsurf, 1, c->table_bf16, c->ds.as<double>(),
                            c->stats.as<unsigned long long>(), c->queue.as<int>() + 16, c->wg_per_cu, c->threshold,
                            c->spi);
             n += 3;
+            if (c->reduction == 1) {
+                launch_reduce_rays(c->stream, cw.ray_de, (size_t)nsnap * 2 * c->nsurf * p.nVar, p.photmax,
+                                   c->ds.as<double>());
+                ++n;
+            }
         }
     }
     if (timed) RC_HIP(hipEventRecord(c->evm1, c->stream));
@@ -749,6 +766,12 @@ int rc_run(rc_ctx *c, int variant) {
     }
     if (variant == RC_VARIANT_PLAIN_BANDS)
         if (int e = c->bacc.ensure((size_t)c->ncell * 4)) return e;
+    if (variant == RC_VARIANT_BATCHED && c->reduction == 1) {
+        const size_t ns = c->snaps.size();
+        if (int e = c->rayf.ensure(ns * c->ncell * p.nVar * (size_t)p.photmax * 4)) return e;
+        if (p.calcwalls == 1)
+            if (int e = c->rayw.ensure(ns * 2 * c->nsurf * p.nVar * (size_t)p.photmax * 4)) return e;
+    }
     int launches = 0;
     const bool graph = c->use_graph && variant == RC_VARIANT_BATCHED;
     if (graph && !c->graph_exec) {
@@ -803,6 +826,15 @@ int rc_set_restriction(rc_ctx *c, int mode) {
         return -2;
     }
     c->restriction = mode;
+    return 0;
+}
+
+int rc_set_reduction(rc_ctx *c, int mode) {
+    if (!c || (mode != 0 && mode != 1)) { set_error("reduction: 0 (FP64 atomics) or 1 (deterministic)"); return -1; }
+    if (mode == 1 && c->p.photmax > 65535) { set_error("deterministic reduction: photmax above 65535"); return -2; }
+    if (c->launched) (void)rc_sync(c, nullptr);
+    c->reduction = mode;
+    c->drop_graph();
     return 0;
 }
 

@@ -24,6 +24,10 @@ struct DevCommon {
     float Lx, Ly, Lz, toll, tollS;
     int bd[6], adiab[2];
     uint32_t k0, k1;               /* Philox key */
+    /* deterministic reduction (batched variant): every ray's FP32 tally at
+     * ray_de[unit photmax + h], summed per unit afterwards in a fixed order
+     * with compensation; nullptr: FP64 atomics, the default */
+    float *ray_de;
 };
 
 /* Table rows: nT temperatures, one pad entry (the clamped top row needs
@@ -122,6 +126,9 @@ void launch_surface_plain(hipStream_t st, const DevCommon &cm, const DevSnap *sn
 void launch_trace(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int surface, const int32_t *rays,
                   const int32_t *unit, int n, const uint16_t *keys, rc_ray *out);
 void launch_finalize(hipStream_t st, const double *de, int n, int nVar, float *mean, float *sdev);
+/* de[u] = the compensated FP64 sum of ray[u photmax + h], h = 0..photmax-1,
+ * in a fixed order (one wavefront per unit) */
+void launch_reduce_rays(hipStream_t st, const float *ray, size_t nunit, int photmax, double *de);
 void launch_zero(hipStream_t st, double *p, size_t n);
 void launch_zero_u64(hipStream_t st, unsigned long long *p, int n);
 

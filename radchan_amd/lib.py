@@ -124,6 +124,7 @@ def load() -> C.CDLL:
         "rc_spline_restrict_noalloc": (C.c_int, [P(Params), C.c_int, vp, vp, vp, vp]),
         "rc_set_restriction": (C.c_int, [C.c_void_p, C.c_int]),
         "rc_set_estimator": (C.c_int, [C.c_void_p, C.c_int]),
+        "rc_set_reduction": (C.c_int, [C.c_void_p, C.c_int]),
         "rc_set_dropin_modes": (C.c_int, [C.c_int, C.c_int]),
         "rc_ckdist_device": (C.c_int, [C.c_int, C.c_void_p] + [C.c_void_p] * 4 + [C.c_int, C.c_double, C.c_int]
                              + [C.c_void_p] * 5 + [C.c_int, C.c_void_p]),

@@ -115,6 +115,11 @@ class Solver:
         at Tmax; applies to fields set afterwards."""
         L.check(self.lib.rc_set_estimator(self.ctx, mode), "rc_set_estimator")
 
+    def set_reduction(self, mode: int) -> None:
+        """Batched tallies: 0 FP64 atomics (default), 1 deterministic
+        (per-ray tallies summed in a fixed order with compensation)."""
+        L.check(self.lib.rc_set_reduction(self.ctx, mode), "rc_set_reduction")
+
     def set_graph(self, on: bool = True) -> None:
         L.check(self.lib.rc_set_graph(self.ctx, int(on)), "rc_set_graph")
 
