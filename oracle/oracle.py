@@ -13,7 +13,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "liboracle.so")
+LIB_PATH = os.environ.get("RADCHAN_ORACLE_LIB") or os.path.join(HERE, "liboracle.so")
 
 _lib = None
 

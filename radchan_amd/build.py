@@ -72,6 +72,38 @@ def build_library(verbose: bool = False) -> str:
     return LIB
 
 
+SAN_FLAGS = ["-fsanitize=address,undefined", "-fno-omit-frame-pointer", "-fno-sanitize-recover=undefined"]
+
+
+def build_sanitized(outdir: str) -> tuple:
+    """The library with its host C++ (rc_tables.cpp, rc_capi.cpp) built by
+    g++ with AddressSanitizer and UndefinedBehaviorSanitizer, linked with
+    the HIP objects of build_library (device code is not instrumented),
+    and the oracle built the same way, both into `outdir`: for the CPU
+    tests of the host code.  Load them with the sanitizers' runtimes
+    preloaded and RADCHAN_LIB / RADCHAN_ORACLE_LIB pointing at them.
+    Returns (library, oracle library)."""
+    os.makedirs(outdir, exist_ok=True)
+    objdir = os.path.join(HERE, "_build")
+    objs = []
+    for src in SOURCES:
+        if src.endswith(".hip"):
+            obj = os.path.join(objdir, os.path.splitext(src)[0] + ".o")
+            if not os.path.exists(obj):
+                build_library()
+        else:
+            obj = os.path.join(outdir, os.path.splitext(src)[0] + ".o")
+            _run(["g++", "-std=c++17", "-fPIC", "-O1", "-g", *SAN_FLAGS, "-D__HIP_PLATFORM_AMD__",
+                  "-I/opt/rocm/include", "-c", os.path.join(CSRC, src), "-o", obj])
+        objs.append(obj)
+    lib = os.path.join(outdir, "libradchan_san.so")
+    _run(["g++", "-shared", *SAN_FLAGS, "-o", lib, *objs, "-L/opt/rocm/lib", "-lamdhip64",
+          "-Wl,-rpath,/opt/rocm/lib"])
+    olib = os.path.join(outdir, "liboracle_san.so")
+    _run(["make", "-C", ORACLE_DIR, "-s", "san", f"SAN_OUT={olib}"])
+    return lib, olib
+
+
 def build_oracle(with_reference: bool = True) -> None:
     """Compile the CPU checker and, if the RadChan tree is present, its
     reference drivers (outputs only under oracle/_ref)."""

@@ -11,7 +11,7 @@ import os
 import numpy as np
 
 HERE = os.path.dirname(os.path.abspath(__file__))
-LIB_PATH = os.path.join(HERE, "libradchan.so")
+LIB_PATH = os.environ.get("RADCHAN_LIB") or os.path.join(HERE, "libradchan.so")
 
 MAXLEV = 9
 FACES = ("W", "E", "S", "N", "B", "T")
