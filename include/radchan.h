@@ -56,6 +56,10 @@ typedef struct rc_params {
     int32_t nT, nQ, nB;              /* NarrowBand.h:10-12 (grey :35-37)       */
     float   abscoeff, Tmins, wvmax;  /* NarrowBand.h:38-40                     */
     uint64_t seed;                   /* Philox key (no reference counterpart)  */
+    int32_t sampler;                 /* fluid directions, param.h:62-63 random: */
+                                     /* 1 Sobol points (digitally shifted per   */
+                                     /* cell and run), 0 Philox                 */
+    int32_t pad_;
 } rc_params;
 
 /* Spectral tables in the reference's original band order, as read by readT

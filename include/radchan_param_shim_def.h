@@ -24,8 +24,14 @@ __attribute__((constructor)) static void radchan_params_from_macros(void) {
     const int b6[6] = {bdw, bde, bds, bdn, bdb, bdt};
     if (rc_params_compiled(&prm, imax, jmax, kmax, p_col, Lx, Ly, Lz, fact_mesh, nVar, photmax, chanHeight,
                            grid_num, maxi, maxj, maxk, maxs, srt, grey, calcwalls, e6, t6, adiabw, adiabe, toll,
-                           tollS, b6, nT, nQ, nB, RADCHAN_GREY_VALUES) != 0 ||
-        rc_set_params(&prm) != 0)
+                           tollS, b6, nT, nQ, nB, RADCHAN_GREY_VALUES) != 0) {
+        fprintf(stderr, "radchan: compiled-in parameters rejected: %s\n", rc_last_error());
+        return;
+    }
+#if defined(random)
+    prm.sampler = (random == 1) ? 1 : 0;   /* param.h:62-63: 1 Sobol, other pseudo-random */
+#endif
+    if (rc_set_params(&prm) != 0)
         fprintf(stderr, "radchan: compiled-in parameters rejected: %s\n", rc_last_error());
 }
 

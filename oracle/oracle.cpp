@@ -97,6 +97,38 @@ inline void draw4(uint64_t seed, uint32_t h, uint32_t v, uint32_t cell, uint32_t
     for (int q = 0; q < 4; ++q) u[q] = u01f(c[q]);
 }
 
+/* Sobol directions (the reference's random == 1, emiss_angSobol,
+ * mc_gpu_noint.cu:1085-1092, with cuRAND's Sobol32 generators of
+ * :84-88 and :813-817): point h of the first two dimensions of the Sobol
+ * sequence -- dimension 1 the van der Corput sequence (direction numbers
+ * 2^(31-k)), dimension 2 Joe and Kuo's first primitive polynomial x + 1
+ * with m_1 = 1 (v_1 = 2^31, v_k = v_(k-1) xor v_(k-1) >> 1) -- digitally
+ * shifted (xor) by two Philox words of counter (2^32 - 1, v, cell, tag),
+ * one shift per (cell, run), so that the runs are independent estimates
+ * (the reference starts every thread's sequence at its thread index,
+ * quirk A9). */
+inline void sobol2_point(uint32_t h, uint32_t *x0, uint32_t *x1) {
+    uint32_t r = 0u, y = 0u, v = 0x80000000u;
+    for (int k = 0; k < 32; ++k)
+        if (h >> k & 1u) r |= 0x80000000u >> k;
+    for (uint32_t g = h; g; g >>= 1) {
+        if (g & 1u) y ^= v;
+        v ^= v >> 1;
+    }
+    *x0 = r;
+    *x1 = y;
+}
+
+inline void sobol_dir_uniforms(uint64_t seed, uint32_t h, uint32_t v, uint32_t cell, uint32_t tag, float *a,
+                               float *b) {
+    uint32_t c[4] = {0xFFFFFFFFu, v, cell, tag};
+    philox10(c, (uint32_t)seed, (uint32_t)(seed >> 32));
+    uint32_t x0, x1;
+    sobol2_point(h, &x0, &x1);
+    *a = u01f(x0 ^ c[0]);
+    *b = u01f(x1 ^ c[1]);
+}
+
 /* ---------------- project math: identical op sequence on GPU -------------- */
 inline float bits2f(uint32_t b) { float f; std::memcpy(&f, &b, 4); return f; }
 
@@ -639,6 +671,9 @@ void fluid_ray(const orc_ctx &c, int i, int j, int k, int v, int h, int rank, in
     const Level &L0 = c.lev[0];
     float u[4];
     draw4(p.seed, (uint32_t)h, (uint32_t)v, fluid_cell_id(p, i, j, k), TAG_FLUID + c.tagoff, u);
+    if (p.sampler == 1)
+        sobol_dir_uniforms(p.seed, (uint32_t)h, (uint32_t)v, fluid_cell_id(p, i, j, k), TAG_FLUID + c.tagoff, &u[0],
+                           &u[1]);
     Ray<G> r;
     r.xp = (G)L0.x[i]; r.yp = (G)L0.y[j]; r.zp = (G)L0.z[k];
     r.ic = i; r.jc = j; r.kc = k;
@@ -758,6 +793,15 @@ void orc_sort_idx(const float *kavg, int nB, int32_t *idx) {
 /* isotropic direction (emiss_ang, mc_gpu_noint.cu:1077-1084):
  * phi = 2 pi u0, cos(theta) = 1 - 2 u1, s = (cos t, sin t cos p, sin t sin p);
  * sin(theta) = 2 sqrt(u1 (1-u1)) exactly from u1. */
+/* point h of the two-dimensional Sobol sequence (32-bit fractions) */
+void orc_sobol2(uint32_t h, uint32_t x[2]) { sobol2_point(h, &x[0], &x[1]); }
+
+/* the direction uniforms (phi, theta) of fluid ray h of run v of cell id
+ * `cell` under the Sobol sampler */
+void orc_sobol_uniforms(uint64_t seed, uint32_t h, uint32_t v, uint32_t cell, uint32_t tag, float u[2]) {
+    sobol_dir_uniforms(seed, h, v, cell, tag, &u[0], &u[1]);
+}
+
 void orc_iso_dir(float u0, float u1, float s[3]) {
     const float ct = 1.0f - 2.0f * u1;
     const float st = 2.0f * std::sqrt(u1 * (1.0f - u1));

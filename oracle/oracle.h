@@ -52,6 +52,8 @@ typedef struct orc_params {
     int32_t nT, nQ, nB;              /* NarrowBand.h:10-12 / 35-37             */
     float   abscoeff, Tmins, wvmax;  /* NarrowBand.h:38-40 (grey block)        */
     uint64_t seed;                   /* Philox key                             */
+    int32_t sampler;                 /* 1: Sobol fluid directions, 0: Philox   */
+    int32_t pad_;
 } orc_params;
 
 /* Spectral tables as read by readT (read_tables.cpp:21-95).  Arrays are in
@@ -104,6 +106,8 @@ void     orc_expm1_exp(float x, float *em1, float *e);
 float    orc_planck(float T, float nu);
 int      orc_upper_bound(const float *cdf, int n, float R);
 void     orc_sort_idx(const float *kavg, int nB, int32_t *idx);
+void     orc_sobol2(uint32_t h, uint32_t x[2]);
+void     orc_sobol_uniforms(uint64_t seed, uint32_t h, uint32_t v, uint32_t cell, uint32_t tag, float u[2]);
 void     orc_iso_dir(float u0, float u1, float s[3]);
 void     orc_lamb_dir(float u0, float u1, int bound, float s[3]);
 int      orc_grid(const orc_params *p, int level, float *x, float *xu,

@@ -38,6 +38,8 @@ def load():
         "orc_upper_bound": (C.c_int, [P(f), C.c_int, f]),
         "orc_sort_idx": (None, [P(f), C.c_int, P(C.c_int32)]),
         "orc_iso_dir": (None, [f, f, P(f)]),
+        "orc_sobol2": (None, [C.c_uint32, P(C.c_uint32)]),
+        "orc_sobol_uniforms": (None, [C.c_uint64, C.c_uint32, C.c_uint32, C.c_uint32, C.c_uint32, P(f)]),
         "orc_lamb_dir": (None, [f, f, C.c_int, P(f)]),
         "orc_grid": (C.c_int, [P(L.Params), C.c_int] + [P(f)] * 6),
         "orc_tmax": (f, [P(L.Params), P(f)]),
@@ -129,6 +131,21 @@ def sort_idx(kavg: np.ndarray) -> np.ndarray:
     out = np.zeros(len(kavg), np.int32)
     load().orc_sort_idx(_f(kavg), len(kavg), out.ctypes.data_as(C.POINTER(C.c_int32)))
     return out
+
+
+def sobol2(h: int):
+    """Point h of the two-dimensional Sobol sequence as 32-bit fractions."""
+    x = (C.c_uint32 * 2)()
+    load().orc_sobol2(h, x)
+    return int(x[0]), int(x[1])
+
+
+def sobol_uniforms(seed: int, h: int, v: int, cell: int, tag: int = 0):
+    """(phi, theta) uniforms of fluid ray h of run v of cell id `cell` under
+    the Sobol sampler."""
+    u = (C.c_float * 2)()
+    load().orc_sobol_uniforms(seed, h, v, cell, tag, u)
+    return float(u[0]), float(u[1])
 
 
 def iso_dir(u0: float, u1: float):

@@ -67,7 +67,8 @@ def build_library(verbose: bool = False) -> str:
             print(out)
         objs.append(obj)
     tmp = LIB + ".tmp"
-    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs])
+    _run([hipcc, "--offload-arch=gfx950", "-shared", "-fPIC", "-o", tmp, *objs, "-L/opt/rocm/lib",
+          "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib"])
     os.replace(tmp, LIB)
     return LIB
 
@@ -98,7 +99,7 @@ def build_sanitized(outdir: str) -> tuple:
         objs.append(obj)
     lib = os.path.join(outdir, "libradchan_san.so")
     _run(["g++", "-shared", *SAN_FLAGS, "-o", lib, *objs, "-L/opt/rocm/lib", "-lamdhip64",
-          "-Wl,-rpath,/opt/rocm/lib"])
+          "-lrocprofiler-sdk-roctx", "-Wl,-rpath,/opt/rocm/lib"])
     olib = os.path.join(outdir, "liboracle_san.so")
     _run(["make", "-C", ORACLE_DIR, "-s", "san", f"SAN_OUT={olib}"])
     return lib, olib

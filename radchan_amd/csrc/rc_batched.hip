@@ -101,6 +101,7 @@ struct Unit {
     uint32_t cid, tag;    /* Philox cell id and stream tag               */
     float Ti;             /* emitting temperature                        */
     int skip;             /* wall unit of a non-adiabatic wall           */
+    uint32_t sh0, sh1;    /* Sobol sampler: the (cell, run)'s shift words */
 };
 
 /* unit u of the launch: (cell, run) of the snapshot whose range of
@@ -117,12 +118,18 @@ __device__ __forceinline__ Unit decode(const DevCommon &cm, const DevSnap *__res
     U.s = s;
     U.n = cs - (SURF ? sn.sbase : sn.ubase);
     U.skip = 0;
+    U.sh0 = U.sh1 = 0u;
     if (!SURF) {
         cell_ijk(sn, U.n, U.i, U.j, U.k);
         U.bound = 0;
         U.cid = (uint32_t)U.n + (uint32_t)sn.imax * sn.jmax * (sn.zstart - 1);
         U.tag = TAG_FLUID;
         U.Ti = cell_T(sn, sn, U.i, U.j, U.k);
+        if (cm.sampler) {
+            const uint4 sh = draw4_bits(cm.k0, cm.k1, SOBOL_SHIFT_H, (uint32_t)U.v, U.cid, TAG_FLUID + 16u * (uint32_t)sn.snap);
+            U.sh0 = sh.x;
+            U.sh1 = sh.y;
+        }
     } else {
         U.bound = U.n / sn.nsurf;
         const int m = U.n % sn.nsurf;
@@ -179,6 +186,7 @@ __device__ __forceinline__ Unit uni(const Unit &a) {
     b.s = uni(a.s); b.n = uni(a.n); b.v = uni(a.v);
     b.i = uni(a.i); b.j = uni(a.j); b.k = uni(a.k); b.bound = uni(a.bound);
     b.cid = uni(a.cid); b.tag = uni(a.tag); b.Ti = uni(a.Ti); b.skip = uni(a.skip);
+    b.sh0 = uni(a.sh0); b.sh1 = uni(a.sh1);
     return b;
 }
 
@@ -561,6 +569,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                         a = q.a;
                         b = q.b;
                     }
+                    if (!SURF && cm.sampler) sobol_uniforms((uint32_t)h, U.sh0, U.sh1, a, b);
                     float sx, sy, sz;
                     if (SURF) surface_dir(a, b, U.bound, sx, sy, sz);
                     else fluid_dir(a, b, sx, sy, sz);

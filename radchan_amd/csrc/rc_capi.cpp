@@ -19,6 +19,8 @@
 #include <string>
 #include <vector>
 
+#include <rocprofiler-sdk-roctx/roctx.h>
+
 #include "rc_internal.h"
 #include "rc_kernels.h"
 
@@ -33,6 +35,16 @@ using namespace rc;
 namespace {
 
 constexpr int kGuideM = 2048;   /* guide-table resolution for the band CDFs */
+
+/* a roctx range over the enclosing scope (set-up, launch and harvest
+ * phases show up on a rocprofv3 --marker-trace timeline; no cost without
+ * a profiler attached) */
+struct Range {
+    explicit Range(const char *what) { roctxRangePushA(what); }
+    ~Range() { roctxRangePop(); }
+    Range(const Range &) = delete;
+    Range &operator=(const Range &) = delete;
+};
 
 #define RC_HIP(call)                                                                         \
     do {                                                                                     \
@@ -354,6 +366,7 @@ DevCommon make_common(const rc_ctx *c) {
     cm.adiab[0] = p.adiabw == 1; cm.adiab[1] = p.adiabe == 1;
     cm.k0 = (uint32_t)p.seed; cm.k1 = (uint32_t)(p.seed >> 32);
     cm.ray_de = nullptr;
+    cm.sampler = p.sampler == 1 ? 1 : 0;
     return cm;
 }
 
@@ -525,6 +538,7 @@ int rc_device_synchronize(void) {
 }
 
 rc_ctx *rc_create(const rc_params *pp) {
+    const Range range_("radchan: set-up (rc_create)");
     rc_params p = *pp;
     if (p.grey) p.srt = 0;                              /* NarrowBand.h:220-221 */
     if (check_params(p)) return nullptr;
@@ -596,6 +610,7 @@ int rc_load_tables(rc_ctx *c, const rc_tables *t) {
 }
 
 int rc_set_fields(rc_ctx *c, int nsnap, const float *const *T, int zstart) {
+    const Range range_("radchan: fields (rc_set_fields)");
     if (!c || nsnap < 1 || !T) { set_error("bad arguments to rc_set_fields"); return -1; }
     const rc_params &p = c->p;
     if (zstart < 1 || zstart + c->kloc - 1 > p.kmax) { set_error("zstart outside 1..kmax-kmax/p_col+1"); return -2; }
@@ -750,6 +765,7 @@ static int enqueue_solve(rc_ctx *c, int variant, int *launches, bool timed) {
 }
 
 int rc_run(rc_ctx *c, int variant) {
+    const Range range_("radchan: launch (rc_run)");
     if (!c || c->snaps.empty()) { set_error("rc_run before rc_set_fields"); return -1; }
     if (variant != RC_VARIANT_PLAIN && variant != RC_VARIANT_BATCHED && variant != RC_VARIANT_PLAIN_BANDS) {
         set_error("unknown variant");
@@ -900,6 +916,7 @@ int rc_set_graph(rc_ctx *c, int on) {
  * variant, device-event timing of the march): steps per iteration at the
  * default occupancy, then workgroups per CU, then the refill threshold */
 int rc_autotune(rc_ctx *c, int32_t *best_wg, int32_t *best_th, int32_t *best_spi, float *best_ms) {
+    const Range range_("radchan: autotune");
     if (!c || c->snaps.empty()) { set_error("rc_autotune before rc_set_fields"); return -1; }
     static const int spis[] = {1, 2, 3, 4};
     static const int wgs[] = {8, 12, 16, 20, 24, 32, 40};
@@ -933,6 +950,7 @@ int rc_autotune(rc_ctx *c, int32_t *best_wg, int32_t *best_th, int32_t *best_spi
 }
 
 int rc_sync(rc_ctx *c, rc_stats *st) {
+    const Range range_("radchan: harvest (rc_sync)");
     if (!c) { set_error("null context"); return -1; }
     if (c->launched) {
         RC_HIP(hipStreamSynchronize(c->stream));
@@ -1127,6 +1145,7 @@ int rc_group_set_schedule(rc_group *g, int wg_per_cu, int threshold, int steps_p
 }
 
 int rc_group_run(rc_group *g) {
+    const Range range_("radchan: group launch");
     if (!g) { set_error("null group"); return -1; }
     /* the sessions' own queued work on the buffers the group writes */
     for (rc_ctx *c : g->cs) {
@@ -1211,6 +1230,7 @@ int rc_group_run(rc_group *g) {
 }
 
 int rc_group_sync(rc_group *g, rc_stats *st) {
+    const Range range_("radchan: group harvest");
     if (!g) { set_error("null group"); return -1; }
     if (g->launched) {
         RC_HIP(hipStreamSynchronize(g->stream));
@@ -1321,6 +1341,7 @@ int rc_set_dropin_modes(int estimator, int restriction) {
 static void report(const char *where) { std::fprintf(stderr, "radchan %s: %s\n", where, last_error()); }
 
 void mc_gpu_(float *Tfort, int *zstart, int *rank) {
+    const Range range_("radchan: mc_gpu_");
     (void)rank;
     std::lock_guard<std::mutex> lk(g_mu);
     if (!Tfort || !zstart) { set_error("null argument"); report("mc_gpu_"); return; }
@@ -1353,6 +1374,7 @@ void mc_gpu_(float *Tfort, int *zstart, int *rank) {
 }
 
 void get_results_(float *resfort, float *varfort, int *rank) {
+    const Range range_("radchan: get_results_");
     (void)rank;
     std::lock_guard<std::mutex> lk(g_mu);
     if (!g_pending || !g_ctx) { set_error("get_results_ called without a preceding mc_gpu_"); report("get_results_"); return; }

@@ -99,6 +99,38 @@ __device__ __forceinline__ U4 draw4(uint32_t k0, uint32_t k1, uint32_t h, uint32
     return u;
 }
 
+/* the four raw 32-bit Philox outputs of counter (h, v, cell, tag) */
+__device__ __forceinline__ uint4 draw4_bits(uint32_t k0, uint32_t k1, uint32_t h, uint32_t v, uint32_t cell,
+                                            uint32_t tag) {
+    uint32_t c0 = h, c1 = v, c2 = cell, c3 = tag;
+    philox10(c0, c1, c2, c3, k0, k1);
+    return make_uint4(c0, c1, c2, c3);
+}
+
+/* Point h of the first two dimensions of the Sobol sequence, as 32-bit
+ * fractions: dimension 1 has the direction numbers v_k = 2^(31-k) (the van
+ * der Corput sequence, the bit reversal of h), dimension 2 those of the
+ * primitive polynomial x + 1 with m_1 = 1 (Joe and Kuo's first entry):
+ * v_1 = 2^31, v_k = v_(k-1) xor (v_(k-1) >> 1). */
+__device__ __forceinline__ void sobol2(uint32_t h, uint32_t &x0, uint32_t &x1) {
+    x0 = __builtin_bitreverse32(h);
+    uint32_t y = 0u, v = 0x80000000u;
+    for (; h; h >>= 1) {
+        y ^= (h & 1u) ? v : 0u;
+        v ^= v >> 1;
+    }
+    x1 = y;
+}
+
+/* the direction uniforms of ray h under the Sobol sampler: the Sobol
+ * point h digitally shifted (xor) by its cell and run's shift words */
+__device__ __forceinline__ void sobol_uniforms(uint32_t h, uint32_t s0, uint32_t s1, float &a, float &b) {
+    uint32_t x0, x1;
+    sobol2(h, x0, x1);
+    a = u01(x0 ^ s0);
+    b = u01(x1 ^ s1);
+}
+
 /* sin and cos of 2 pi u, u in (0,1]: exact quadrant split of 4u, octant
  * fold, Taylor polynomials (degree 9 / 10) on [0, pi/4]. */
 __device__ __forceinline__ void sincos2pi(float u, float &s, float &c) {

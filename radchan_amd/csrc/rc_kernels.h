@@ -28,7 +28,12 @@ struct DevCommon {
      * ray_de[unit photmax + h], summed per unit afterwards in a fixed order
      * with compensation; nullptr: FP64 atomics, the default */
     float *ray_de;
+    int sampler;                   /* fluid directions: 1 Sobol points shifted per (cell, run), 0 Philox */
+    int pad_s;
 };
+
+/* the counter word h of a (cell, run)'s Sobol shift: no ray has it */
+constexpr uint32_t SOBOL_SHIFT_H = 0xFFFFFFFFu;
 
 /* Table rows: nT temperatures, one pad entry (the clamped top row needs
  * no branch), then two entries per face (W,E,S,N,B,T) holding the Ib of a

@@ -901,8 +901,14 @@ __device__ __forceinline__ void surface_adjust(const DevSnap &sn, Ray &r) {
 __device__ __forceinline__ uint32_t fluid_setup(const DevCommon &cm, const DevSnap &sn, int i, int j, int k,
                                                 uint32_t cid, int v, int h, uint32_t key, bool have_key, float Ti,
                                                 Ray &r, En &e) {
-    const U4 u = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)v, cid, TAG_FLUID + 16u * (uint32_t)sn.snap);
+    const uint32_t tag = TAG_FLUID + 16u * (uint32_t)sn.snap;
+    U4 u = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)v, cid, tag);
     if (!have_key) key = band_key(cm, sn, false, u.c, u.d);
+    if (cm.sampler) {
+        /* the reference's random == 1 (emiss_angSobol, mc_gpu_noint.cu:1085-1092) */
+        const uint4 sh = draw4_bits(cm.k0, cm.k1, SOBOL_SHIFT_H, (uint32_t)v, cid, tag);
+        sobol_uniforms((uint32_t)h, sh.x, sh.y, u.a, u.b);
+    }
     float sx, sy, sz;
     fluid_dir(u.a, u.b, sx, sy, sz);
     set_dir(r, sx, sy, sz);

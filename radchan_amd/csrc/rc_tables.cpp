@@ -103,6 +103,7 @@ void default_params(rc_params &p) {
     p.nT = 135; p.nQ = 16; p.nB = 999;
     p.abscoeff = 1.f; p.Tmins = 490.f; p.wvmax = 5000.f;
     p.seed = 1234;
+    p.sampler = 1;      /* param.h:62-63: random 1 (Sobol directions) */
 }
 
 int check_params(const rc_params &p) {
@@ -139,6 +140,7 @@ int check_params(const rc_params &p) {
     if (!(p.toll > 0) || !(p.tollS > 0) || !(p.toll < 1) || !(p.tollS < 1))
         return fail("toll and tollS must lie in (0, 1)");
     if (p.calcwalls == 1 && !(p.adiabw == 1 || p.adiabe == 1)) return fail("calcwalls needs an adiabatic x wall");
+    if (p.sampler != 0 && p.sampler != 1) return fail("sampler: 0 (Philox) or 1 (Sobol)");
     return 0;
 }
 

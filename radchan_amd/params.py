@@ -57,6 +57,7 @@ def base_params(imax: int, jmax: int, kmax: int, *, photmax: int = 100, nVar: in
     else:
         p.nB, p.nT, p.nQ = 360, 20, 16
     p.seed = seed
+    p.sampler = 0                 # Philox directions (the workloads' stated streams)
     return p
 
 
