@@ -85,6 +85,18 @@ def config(name: str) -> L.Params:
         return p
     if name == "big256":          # the 256 x 128 x 256 narrow-band field
         return base_params(256, 128, 256, photmax=100, fact_mesh=1.5)
+    if name == "prod28":          # param.h's solver settings on 28x96x128 (its coarsening pattern scaled)
+        r = reference_params()
+        sizes = [(28 // a, 96 // b, 128 // c) for a, b, c in
+                 zip((1, 1, 1, 1, 1, 7, 7, 7, 14), (1, 3, 6, 12, 24, 48, 96, 96, 96),
+                     (1, 2, 4, 8, 16, 32, 64, 128, 128))]
+        p = base_params(28, 96, 128, photmax=r.photmax, fact_mesh=r.fact_mesh, levels=sizes,
+                        maxs=[r.maxs[g] for g in range(9)])
+        p.bd[:] = list(r.bd)
+        p.Lx, p.Ly, p.Lz = r.Lx, r.Ly, r.Lz
+        p.nB, p.nT, p.nQ = r.nB, r.nT, r.nQ   # NarrowBand.h's table sizes
+        p.sampler = r.sampler                 # param.h random 1: Sobol directions
+        return p
     if name == "ens64x96":        # the small snapshots of the ensemble
         return base_params(64, 96, 64, photmax=100, fact_mesh=1.5)
     raise KeyError(name)
