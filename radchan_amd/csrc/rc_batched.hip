@@ -7,8 +7,9 @@
  * units in chunks of nVar from per-XCD ranges of a global atomic queue.  For
  * each unit it draws the photmax (band, g) samples and sorts them (the srt
  * enumeration of mc_gpu_noint.cu:1234-1267): in registers up to 256 keys,
- * in LDS above; the previous unit's rays keep marching from the LDS ray
- * queue meanwhile.
+ * above that by two 8-bit radix passes through the wavefront's own HBM
+ * scratch; the previous unit's rays keep marching from the LDS ray queue
+ * meanwhile.
  *
  * The main loop generates rays in bulk (all 64 lanes set up one ray each),
  * refills idle lanes from the queue once enough of them are idle, and
@@ -39,21 +40,64 @@ using namespace rcd;
 
 namespace rc {
 
-/* bitonic sort of n2 (power of two) keys in LDS by one 64-lane workgroup */
-template <class K>
-__device__ __forceinline__ void lds_sort(K *lds, int n2) {
-    for (int k = 2; k <= n2; k <<= 1)
-        for (int j = k >> 1; j > 0; j >>= 1) {
-            for (int i = threadIdx.x; i < n2; i += 64) {
-                const int ixj = i ^ j;
-                if (ixj > i) {
-                    const K a = lds[i], b = lds[ixj];
-                    const bool up = (i & k) == 0;
-                    if ((a > b) == up) { lds[i] = b; lds[ixj] = a; }
-                }
-            }
-            __syncthreads();
+/* the bits of m below this lane */
+__device__ __forceinline__ uint32_t lanes_below(unsigned long long m) {
+    return __builtin_amdgcn_mbcnt_hi((uint32_t)(m >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)m, 0u));
+}
+
+/* The two digit histograms (hist[0..255]: bits 0-7 of the keys,
+ * hist[256..511]: bits 8-15) turned into exclusive prefix sums, each lane
+ * holding 4 consecutive buckets of a histogram. */
+__device__ __forceinline__ void radix_offsets(uint32_t *hist, int lane) {
+#pragma unroll 1
+    for (int half = 0; half < 2; ++half) {
+        uint32_t *hh = hist + 256 * half + 4 * lane;
+        const uint32_t c0 = hh[0], c1 = hh[1], c2 = hh[2], c3 = hh[3];
+        const uint32_t s = c0 + c1 + c2 + c3;
+        uint32_t inc = s;
+#pragma unroll
+        for (int d = 1; d < 64; d <<= 1) {
+            const uint32_t y = (uint32_t)__shfl_up((int)inc, d);
+            if (lane >= d) inc += y;
         }
+        const uint32_t ex = inc - s;
+        hh[0] = ex;
+        hh[1] = ex + c0;
+        hh[2] = ex + c0 + c1;
+        hh[3] = ex + c0 + c1 + c2;
+    }
+    __syncthreads();
+}
+
+/* One stable counting pass of an LSD radix sort over n 16-bit keys: the
+ * digit (key >> sh) & 255 of key h goes to dst[off[digit] + (keys of that
+ * digit before h)], off being the digit's running offset in LDS.  Keys are
+ * taken 64 at a time in order; the lanes holding the same digit are found
+ * by eight ballots, so the pass keeps the order of equal digits.  src and
+ * dst are the wavefront's own HBM scratch. */
+__device__ __forceinline__ void radix_pass(const uint16_t *src, uint16_t *dst, uint32_t *off, int n, int sh,
+                                           int lane) {
+#pragma unroll 1
+    for (int base = 0; base < n; base += 64) {
+        const bool ok = base + lane < n;
+        const uint32_t key = ok ? (uint32_t)src[base + lane] : 0u;
+        {
+            const uint32_t dig = (key >> sh) & 255u;
+            unsigned long long m = __ballot(ok);
+#pragma unroll
+            for (int b = 0; b < 8; ++b) {
+                const unsigned long long bb = __ballot((dig >> b) & 1u);
+                m &= ((dig >> b) & 1u) ? bb : ~bb;
+            }
+            if (ok) {
+                const uint32_t o = off[dig];
+                const uint32_t below = lanes_below(m);
+                dst[o + below] = (uint16_t)key;
+                if (below == 0) off[dig] = o + (uint32_t)__popcll(m);
+            }
+        }
+    }
+    __syncthreads();
 }
 
 /* bitonic sort of 64 R keys held in registers, element R lane + r in x[r]:
@@ -300,9 +344,10 @@ __device__ __forceinline__ void finish_ray(const DevCommon &cm, const Slots &sl,
 }
 
 /* LDS words of one wavefront: unit slots, ray queue, (multigrid) the grid
- * levels of the current snapshot, then (srt) its sorted keys (16-bit: keys
- * are below nB nQ <= 65535; at least 256, the register sort's width),
- * preceded, up to 256 keys, by the direction uniforms of the unit's rays */
+ * levels of the current snapshot, then (srt) up to 256 keys the unit's
+ * sorted keys (16-bit: keys are below nB nQ <= 65535; 256 of them, the
+ * register sort's width) preceded by the direction uniforms of its rays,
+ * above 256 keys the radix sort's two digit histograms (512 words) */
 constexpr int LEV_WORDS = (int)(sizeof(DevLevel) / 4);
 
 /* drain of the last rays: at most COOP_RAYS live rays are marched one at a
@@ -337,10 +382,10 @@ __device__ __forceinline__ En bcast_en(const En &a, int src) {
 }
 __host__ __device__ __forceinline__ int batched_lds_words(int n2, int srt, int multi) {
     return SLOT_WORDS * NSLOT + QREC * QCAP + (multi ? LEV_WORDS * RC_MAXLEV : 0) +
-           (srt ? (n2 <= 256 ? 2 * n2 + 128 : n2 / 2) : 0);
+           (srt ? (n2 <= 256 ? 2 * n2 + 128 : 512) : 0);
 }
 
-template <bool SURF, bool MULTI, int TM>
+template <bool SURF, bool MULTI, int TM, bool RADIX>
 __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap *__restrict__ snaps, int nsnap,
                                              double *__restrict__ de, unsigned long long *__restrict__ stats,
                                              int *__restrict__ queue, int threshold, int spi) {
@@ -352,12 +397,17 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
     int n2 = 1;
     while (n2 < P) n2 <<= 1;
     /* srt: up to 256 keys the direction uniforms of the unit's rays are
-     * kept from the key draws; above that they are drawn again at the
-     * rays' generation, and only the keys take LDS */
-    const int nkeep = n2 <= 256 ? n2 : 0;
+     * kept from the key draws and the sorted keys sit in LDS; above that
+     * the uniforms are drawn again at the rays' generation, and the keys
+     * are sorted in the wavefront's HBM scratch (raw and sorted keys in
+     * kr, the first radix pass in ka) */
+    const int nkeep = RADIX ? 0 : n2;
     float *dira = reinterpret_cast<float *>(rq + QREC * QCAP + (MULTI ? LEV_WORDS * RC_MAXLEV : 0));
     float *dirb = dira + nkeep;
     uint16_t *keys = reinterpret_cast<uint16_t *>(dirb + nkeep);
+    const int Pr = (P + 63) & ~63;
+    uint16_t *const kr = RADIX ? cm.srt_keys + (size_t)blockIdx.x * 2 * Pr : nullptr;
+    uint16_t *const ka = kr ? kr + Pr : nullptr;
     const int lane = threadIdx.x;
     const DevSnap &slast = snaps[nsnap - 1];
     const int total = (SURF ? slast.sbase + 2 * slast.nsurf : slast.ubase + slast.ncell) * cm.nVar;
@@ -451,7 +501,7 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                 if (cm.srt && !N.skip) {
                     const DevSnap &sn = snaps[N.s];
                     const uint32_t tagx = N.tag + 16u * (uint32_t)sn.snap;
-                    if (n2 <= 256) {
+                    if constexpr (!RADIX) {
                         /* up to 256 keys: drawn one round of 64 at a time
                          * up to photmax, padded to 128 or 256, then sorted
                          * in registers, 2 or 4 keys per lane */
@@ -486,16 +536,24 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                         }
                         __syncthreads();
                     } else {
-                        for (int h = lane; h < n2; h += 64) {
-                            uint32_t key = 0xFFFFFFFFu;
-                            if (h < P) {
-                                const U4 q = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)N.v, N.cid, tagx);
-                                key = band_key(cm, sn, SURF, q.c, q.d);
-                            }
-                            keys[h] = (uint16_t)key;
+                        /* above 256 keys: drawn into HBM with both digit
+                         * histograms counted in LDS, then two radix passes
+                         * (low byte into ka, high byte back into kr) */
+                        uint32_t *hist = reinterpret_cast<uint32_t *>(keys);
+                        for (int w = lane; w < 512; w += 64) hist[w] = 0u;
+                        __syncthreads();
+#pragma unroll 1
+                        for (int h = lane; h < P; h += 64) {
+                            const U4 q = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)N.v, N.cid, tagx);
+                            const uint32_t key = band_key(cm, sn, SURF, q.c, q.d);
+                            kr[h] = (uint16_t)key;
+                            atomicAdd(&hist[key & 255u], 1u);
+                            atomicAdd(&hist[256u + (key >> 8)], 1u);
                         }
                         __syncthreads();
-                        lds_sort(keys, n2);
+                        radix_offsets(hist, lane);
+                        radix_pass(kr, ka, hist, P, 0, lane);
+                        radix_pass(ka, kr, hist + 256, P, 8, lane);
                     }
                 }
             }
@@ -554,12 +612,12 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                     const int h = cur_h + lane;
                     float a, b;
                     uint32_t key;
-                    if (cm.srt && nkeep) {
+                    if (!RADIX && cm.srt) {
                         key = (uint32_t)keys[h];
                         a = dira[h];
                         b = dirb[h];
-                    } else if (cm.srt) {
-                        key = (uint32_t)keys[h];
+                    } else if (RADIX) {
+                        key = (uint32_t)kr[h];
                         const U4 q = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)U.v, U.cid, U.tag + 16u * (uint32_t)sn.snap);
                         a = q.a;
                         b = q.b;
@@ -780,20 +838,22 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
 
 /* single grid (registers held to 5 wavefronts per SIMD: the march is
  * issue-bound, and a fifth wavefront outweighs the few values the
- * compiler then keeps in scratch) */
-template <bool SURF, int TM>
+ * compiler then keeps in scratch).  RADIX: srt above 256 keys, sorted by
+ * the radix passes; otherwise (no srt, or up to 256 keys) the register
+ * sort, with no radix code in the kernel. */
+template <bool SURF, int TM, bool RADIX>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_batched(DevCommon cm, const DevSnap *__restrict__ snaps, int nsnap, double *__restrict__ de,
           unsigned long long *__restrict__ stats, int *__restrict__ queue, int threshold, int spi) {
-    batched_body<SURF, false, TM>(cm, snaps, nsnap, de, stats, queue, threshold, spi);
+    batched_body<SURF, false, TM, RADIX>(cm, snaps, nsnap, de, stats, queue, threshold, spi);
 }
 
-/* multigrid (registers held to 4 wavefronts per SIMD) */
-template <bool SURF, int TM>
+/* multigrid (registers held to 5 wavefronts per SIMD) */
+template <bool SURF, int TM, bool RADIX>
 __global__ void __launch_bounds__(64) __attribute__((amdgpu_waves_per_eu(5, 8)))
 k_batched_mg(DevCommon cm, const DevSnap *__restrict__ snaps, int nsnap, double *__restrict__ de,
              unsigned long long *__restrict__ stats, int *__restrict__ queue, int threshold, int spi) {
-    batched_body<SURF, true, TM>(cm, snaps, nsnap, de, stats, queue, threshold, spi);
+    batched_body<SURF, true, TM, RADIX>(cm, snaps, nsnap, de, stats, queue, threshold, spi);
 }
 
 __global__ void k_zero_int(int *p, int n) {
@@ -854,9 +914,14 @@ void launch_zero(hipStream_t st, double *p, size_t n) {
     hipLaunchKernelGGL(k_zero_f64, dim3(blocks), dim3(256), 0, st, p, n);
 }
 
-void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int nsnap, int nunit_cells, int surface,
-                    int snaps_bf16, double *de, unsigned long long *stats, int *queue, int wg_per_cu, int threshold,
-                    int spi) {
+/* defaults: what the autotuner picks on 64^3 (single grid) and on the
+ * three-level 192x128x192 case (multigrid: fewer, longer-stepping
+ * wavefronts) */
+static int default_wg_per_cu(const DevCommon &cm, int wg_per_cu) {
+    return wg_per_cu > 0 ? wg_per_cu : (cm.grid_num > 1 ? 20 : 32);
+}
+
+int batched_grid(const DevCommon &cm, long nunit_cells, int wg_per_cu) {
     static int ncu = 0;       /* compute units of the (single) device */
     if (ncu <= 0) {
         int dev = 0;
@@ -864,32 +929,43 @@ void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, i
         (void)hipDeviceGetAttribute(&ncu, hipDeviceAttributeMultiprocessorCount, dev);
         if (ncu <= 0) ncu = 256;
     }
-    /* defaults: what the autotuner picks on 64^3 (single grid) and on the
-     * three-level 192x128x192 case (multigrid: fewer, longer-stepping
-     * wavefronts) */
+    const long total = nunit_cells * (long)cm.nVar;
+    return (int)std::max<long>(1, std::min<long>((long)ncu * default_wg_per_cu(cm, wg_per_cu), total));
+}
+
+size_t batched_srt_keys(const DevCommon &cm, long nunit_cells, int wg_per_cu) {
+    if (!cm.srt || cm.photmax <= 256) return 0;
+    return (size_t)batched_grid(cm, nunit_cells, wg_per_cu) * 2 * (size_t)((cm.photmax + 63) & ~63);
+}
+
+void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int nsnap, int nunit_cells, int surface,
+                    int snaps_bf16, double *de, unsigned long long *stats, int *queue, int wg_per_cu, int threshold,
+                    int spi) {
     const bool mg = cm.grid_num > 1;
-    if (wg_per_cu <= 0) wg_per_cu = mg ? 20 : 32;
     if (threshold <= 0) threshold = 8;
     if (spi <= 0) spi = mg ? 4 : 3;
     int n2 = 1;
     while (n2 < cm.photmax) n2 <<= 1;
     const size_t lds = (size_t)batched_lds_words(n2, cm.srt, cm.grid_num > 1) * 4;
     const size_t nde = (size_t)nunit_cells * cm.nVar;
-    const long total = (long)nde;
-    const int grid = (int)std::max<long>(1, std::min<long>((long)ncu * wg_per_cu, total));
+    const int grid = batched_grid(cm, nunit_cells, wg_per_cu);
     hipLaunchKernelGGL(k_zero_int, dim3(1), dim3(64), 0, st, queue, QPARTS);
     launch_zero(st, de, nde);
     const bool multi = cm.grid_num > 1;
     const bool bf = snaps_bf16;
-#define RC_LAUNCH(K, S, T) hipLaunchKernelGGL((K<S, T>), dim3(grid), dim3(64), lds, st, cm, snaps, nsnap, de, stats, \
-                                           queue, threshold, spi)
+    const bool radix = cm.srt && n2 > 256;
+    if (radix && !cm.srt_keys) return;     /* the callers size it (batched_srt_keys); never launched without */
+#define RC_LAUNCH(K, S, T, R) hipLaunchKernelGGL((K<S, T, R>), dim3(grid), dim3(64), lds, st, cm, snaps, nsnap, de, \
+                                              stats, queue, threshold, spi)
+#define RC_LAUNCH2(K, S, T) do { if (radix) RC_LAUNCH(K, S, T, true); else RC_LAUNCH(K, S, T, false); } while (0)
     if (!surface) {
-        if (multi) { if (bf) RC_LAUNCH(k_batched_mg, false, TAB_BF16); else RC_LAUNCH(k_batched_mg, false, TAB_F32); }
-        else { if (bf) RC_LAUNCH(k_batched, false, TAB_BF16); else RC_LAUNCH(k_batched, false, TAB_F32); }
+        if (multi) { if (bf) RC_LAUNCH2(k_batched_mg, false, TAB_BF16); else RC_LAUNCH2(k_batched_mg, false, TAB_F32); }
+        else { if (bf) RC_LAUNCH2(k_batched, false, TAB_BF16); else RC_LAUNCH2(k_batched, false, TAB_F32); }
     } else {
-        if (multi) { if (bf) RC_LAUNCH(k_batched_mg, true, TAB_BF16); else RC_LAUNCH(k_batched_mg, true, TAB_F32); }
-        else { if (bf) RC_LAUNCH(k_batched, true, TAB_BF16); else RC_LAUNCH(k_batched, true, TAB_F32); }
+        if (multi) { if (bf) RC_LAUNCH2(k_batched_mg, true, TAB_BF16); else RC_LAUNCH2(k_batched_mg, true, TAB_F32); }
+        else { if (bf) RC_LAUNCH2(k_batched, true, TAB_BF16); else RC_LAUNCH2(k_batched, true, TAB_F32); }
     }
+#undef RC_LAUNCH2
 #undef RC_LAUNCH
 }
 

@@ -139,6 +139,7 @@ struct rc_ctx {
     DevBuf bacc;                            /* per-band plain variant: FP32 cell sums */
     int reduction = 0;                      /* batched tallies: 0 FP64 atomics, 1 deterministic */
     DevBuf rayf, rayw;                      /* deterministic reduction: per-ray tallies */
+    DevBuf srtk;                            /* batched, srt above 256 keys: radix-sort scratch */
     DevBuf tr_rays, tr_unit, tr_ulist, tr_out;
     rc_stats last{};
     bool launched = false;
@@ -724,6 +725,7 @@ static int enqueue_solve(rc_ctx *c, int variant, int *launches, bool timed) {
         }
     } else {
         DevCommon cf = cm, cw = cm;
+        cf.srt_keys = cw.srt_keys = c->srtk.as<uint16_t>();
         if (c->reduction == 1) {
             cf.ray_de = c->rayf.as<float>();
             cw.ray_de = c->rayw.as<float>();
@@ -782,6 +784,19 @@ int rc_run(rc_ctx *c, int variant) {
     }
     if (variant == RC_VARIANT_PLAIN_BANDS)
         if (int e = c->bacc.ensure((size_t)c->ncell * 4)) return e;
+    if (variant == RC_VARIANT_BATCHED) {
+        /* radix-sort scratch for the larger of the fluid and wall launches;
+         * a new buffer invalidates a captured graph */
+        const DevCommon cm = make_common(c);
+        const long ns = (long)c->snaps.size();
+        const size_t nk = std::max(batched_srt_keys(cm, ns * c->ncell, c->wg_per_cu),
+                                   p.calcwalls == 1 ? batched_srt_keys(cm, ns * 2 * c->nsurf, c->wg_per_cu) : 0);
+        if (nk) {
+            void *old = c->srtk.p;
+            if (int e = c->srtk.ensure(nk * 2)) return e;
+            if (c->srtk.p != old) c->drop_graph();
+        }
+    }
     if (variant == RC_VARIANT_BATCHED && c->reduction == 1) {
         const size_t ns = c->snaps.size();
         if (int e = c->rayf.ensure(ns * c->ncell * p.nVar * (size_t)p.photmax * 4)) return e;
@@ -1084,7 +1099,7 @@ struct rc_group {
     hipStream_t stream = nullptr;
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evm0 = nullptr, evm1 = nullptr;
     std::vector<DevSnap> h;
-    DevBuf dsnap, de, ds, stats, queue;
+    DevBuf dsnap, de, ds, stats, queue, srtk;
     int wg_per_cu = 0, threshold = 0, spi = 0;
     rc_stats last{};
     bool launched = false;
@@ -1096,7 +1111,7 @@ static int group_compatible(const rc_ctx *a, const rc_ctx *b) {
     const DevCommon x = make_common(a), y = make_common(b);
     bool ok = x.nVar == y.nVar && x.photmax == y.photmax && x.srt == y.srt && x.nB == y.nB && x.nQ == y.nQ &&
               x.nT == y.nT && x.grid_num == y.grid_num && x.Lx == y.Lx && x.Ly == y.Ly && x.Lz == y.Lz &&
-              x.toll == y.toll && x.tollS == y.tollS && x.k0 == y.k0 && x.k1 == y.k1 &&
+              x.toll == y.toll && x.tollS == y.tollS && x.k0 == y.k0 && x.k1 == y.k1 && x.sampler == y.sampler &&
               x.adiab[0] == y.adiab[0] && x.adiab[1] == y.adiab[1] && a->p.grey == b->p.grey &&
               a->table_bf16 == b->table_bf16 && a->p.calcwalls == b->p.calcwalls;
     for (int f = 0; f < 6; ++f) ok = ok && x.bd[f] == y.bd[f];
@@ -1182,7 +1197,12 @@ int rc_group_run(rc_group *g) {
     if (walls) { if (int e = g->ds.ensure((size_t)wcells * nVar * 8)) return e; }
     if (int e = g->stats.ensure(8 * sizeof(unsigned long long))) return e;
     if (int e = g->queue.ensure(32 * sizeof(int))) return e;
-    const DevCommon cm = make_common(c0);
+    DevCommon cm = make_common(c0);
+    const size_t nk = std::max(batched_srt_keys(cm, cells, g->wg_per_cu),
+                               walls ? batched_srt_keys(cm, wcells, g->wg_per_cu) : 0);
+    if (nk)
+        if (int e = g->srtk.ensure(nk * 2)) return e;
+    cm.srt_keys = g->srtk.as<uint16_t>();
     int n = 0;
     RC_HIP(hipEventRecord(g->ev0, g->stream));
     launch_zero_u64(g->stream, g->stats.as<unsigned long long>(), 8);

@@ -30,6 +30,10 @@ struct DevCommon {
     float *ray_de;
     int sampler;                   /* fluid directions: 1 Sobol points shifted per (cell, run), 0 Philox */
     int pad_s;
+    /* batched variant, srt above 256 keys: the radix sort's scratch, 2
+     * rows of photmax (rounded up to 64) keys per workgroup of the launch
+     * (batched_srt_keys) */
+    uint16_t *srt_keys;
 };
 
 /* the counter word h of a (cell, run)'s Sobol shift: no ray has it */
@@ -134,6 +138,10 @@ void launch_finalize(hipStream_t st, const double *de, int n, int nVar, float *m
 /* de[u] = the compensated FP64 sum of ray[u photmax + h], h = 0..photmax-1,
  * in a fixed order (one wavefront per unit) */
 void launch_reduce_rays(hipStream_t st, const float *ray, size_t nunit, int photmax, double *de);
+/* workgroups of a batched launch over nunit_cells emitting (wall) cells,
+ * and the 16-bit keys of radix-sort scratch it needs (0: none) */
+int batched_grid(const DevCommon &cm, long nunit_cells, int wg_per_cu);
+size_t batched_srt_keys(const DevCommon &cm, long nunit_cells, int wg_per_cu);
 void launch_zero(hipStream_t st, double *p, size_t n);
 void launch_zero_u64(hipStream_t st, unsigned long long *p, int n);
 

@@ -116,8 +116,18 @@ def test_field_narrow_band(nb_tables, variant):
 @pytest.mark.parametrize("variant", [VARIANT_PLAIN, VARIANT_BATCHED])
 def test_field_narrow_band_sort_widths(nb_tables, variant, photmax):
     """photmax above 64: the batched kernel's band keys sorted in registers
-    2 per lane (<= 128), 4 per lane (<= 256) and in LDS (above)."""
+    2 per lane (<= 128), 4 per lane (<= 256) and by radix passes through
+    HBM (above)."""
     _field_parity(*cases.nb_case(nb_tables, n=8, photmax=photmax), variant)
+
+
+@pytest.mark.parametrize("photmax", [257, 1000])
+def test_radix_sorted_keys_multigrid_and_walls(nb_tables, photmax):
+    """The radix-sorted band keys (photmax above 256, not a multiple of 64)
+    in the multigrid kernel, fluid and wall units."""
+    p, tab, T = cases.mg_case(nb_tables, photmax=photmax)
+    _field_parity(p, tab, T, VARIANT_BATCHED)
+    _field_parity(p, tab, T, VARIANT_BATCHED, surface=True)
 
 
 @pytest.mark.parametrize("variant", [VARIANT_PLAIN, VARIANT_BATCHED])
