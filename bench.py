@@ -298,8 +298,9 @@ def field_legs(args, lib, p, tab, T, bf16: bool, steps: int, warmup: int, plain_
             traffic = json.load(f).get("hbm_bytes_per_launch")
     roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": traffic,
-            "kernel": "%s<false,%d> (persistent march, %s k-tables)" % (
-                "k_batched_mg" if p.grid_num > 1 else "k_batched", 1 if bf16 else 0, "BF16" if bf16 else "FP32"),
+            "kernel": "%s<false, %d, %s> (persistent march, %s k-tables)" % (
+                "k_batched_mg" if p.grid_num > 1 else "k_batched", 1 if bf16 else 0,
+                "true" if (p.srt == 1 and not p.grey and p.photmax > 256) else "false", "BF16" if bf16 else "FP32"),
             "kernel_ms": round(march_ms, 4), "kernel_ms_each": [round(float(m), 4) for m in march],
             "algorithmic_bytes": int(alg_bytes), "steps_per_launch": int(nsteps), "rays_per_launch": int(rays),
             "bytes_per_step": b_step, "bytes_per_ray": b_ray, "bytes_per_cell": B_CELL}
