@@ -221,7 +221,8 @@ def eps_scaling(p, tab, T, sched, eps, bf16: bool):
             "march_ms": st["march_ms"]}
 
 
-def field_legs(args, lib, p, tab, T, bf16: bool, steps: int, warmup: int, plain_steps: int, scaling: bool = True):
+def field_legs(args, lib, p, tab, T, bf16: bool, steps: int, warmup: int, plain_steps: int, scaling: bool = True,
+               fp64_steps: int = 0):
     """Every leg of one field: autotune, the batched solver with FP32 and
     BF16 k-tables (hipGraph replays, `steps` timed), the march kernel alone
     (HIP events on the solver's stream, no graph) for the roofline, the
@@ -285,6 +286,18 @@ def field_legs(args, lib, p, tab, T, bf16: bool, steps: int, warmup: int, plain_
         log(f"plain, one launch per band: {1e3 * bwall:.2f} ms/step ({sb['launches']} launches), eps {eps_b:.4g}")
     ncell = s.ncell
     s.close()
+    fp64 = None
+    if not args.no_plain and fp64_steps > 0 and plain:
+        # floatpoint 1: the full-FP64 kernels in the plain launch shape
+        q = p.copy()
+        q.fp64 = 1
+        s64 = new_session(q, tab, [T])
+        fwall, sf, _ = timed_leg(s64, lib, VARIANT_PLAIN, fp64_steps, 1)
+        eps_f, _ = std_error(s64.tallies())
+        s64.close()
+        fp64 = {"rays_per_s": sf["rays"] / fwall, "ms_per_step": 1e3 * fwall, "eps": eps_f, "timed_solves": fp64_steps,
+                "time_vs_fp32_plain": fwall * 1e3 / plain["ms_per_step"]}
+        log(f"plain, FP64 (floatpoint 1): {1e3 * fwall:.2f} ms/step, {fp64['time_vs_fp32_plain']:.2f}x the FP32 plain")
 
     b_step, b_ray = (B_STEP_BF16, B_RAY_BF16) if bf16 else (B_STEP, B_RAY)
     alg_bytes = nsteps * b_step + rays * b_ray + ncell * B_CELL
@@ -308,7 +321,7 @@ def field_legs(args, lib, p, tab, T, bf16: bool, steps: int, warmup: int, plain_
                        f"{p.nT} T, photmax {p.photmax}, nVar {p.nVar}, srt {p.srt}, grid_num {p.grid_num}"
                        if not p.grey else f"grey OERM {p.imax}x{p.jmax}x{p.kmax}, photmax {p.photmax}, nVar {p.nVar}",
            "cells": ncell, "schedule": sched, "head": head, "batched": legs, "plain": plain,
-           "plain_one_launch_per_band": bands, "deterministic_reduction": det, "roofline": roof,
+           "plain_one_launch_per_band": bands, "plain_fp64": fp64, "deterministic_reduction": det, "roofline": roof,
            "speedup_vs_plain": (head["rays_per_s"] / plain["rays_per_s"]) if plain else None,
            "time_to_1pct_speedup_vs_plain": (plain["time_to_1pct_s"] / head["time_to_1pct_s"]) if plain else None,
            "time_to_1pct_speedup_vs_plain_per_band": (bands["time_to_1pct_s"] / head["time_to_1pct_s"])
@@ -629,7 +642,8 @@ def main() -> int:
     steps = args.steps
     big = p.imax * p.jmax * p.kmax >= 4 << 20
     main_leg = field_legs(args, lib, p, ptab, T, bf16, steps, args.warmup,
-                          args.plain_steps if args.plain_steps is not None else (2 if big else 3))
+                          args.plain_steps if args.plain_steps is not None else (2 if big else 3),
+                          fp64_steps=0 if big else 3)
     head = main_leg["head"]
 
     # the 64^3 field beside the headline, with the measured time to 1%
@@ -637,7 +651,7 @@ def main() -> int:
     if not args.no_secondary and args.workload != "nb64" and not p.grey:
         q, qtab, qT = field_of("nb64")
         log("bench: 64x64x64 photmax 200 beside the headline")
-        second = field_legs(args, lib, q, qtab, qT, bf16, 10, 2, 3, scaling=False)
+        second = field_legs(args, lib, q, qtab, qT, bf16, 10, 2, 3, scaling=False, fp64_steps=3)
         nb = second["head"]
         second["time_to_1pct_measured"] = confirm_leg(lib, q, qtab, qT, second["schedule"], nb["eps"],
                                                       nb["ms_per_step"])

@@ -59,7 +59,9 @@ typedef struct rc_params {
     int32_t sampler;                 /* fluid directions, param.h:62-63 random: */
                                      /* 1 Sobol points (digitally shifted per   */
                                      /* cell and run), 0 Philox                 */
-    int32_t pad_;
+    int32_t fp64;                    /* param.h:64-65 floatpoint: 1 double      */
+                                     /* precision (the full-FP64 kernels, plain */
+                                     /* variant only), 0 single precision       */
 } rc_params;
 
 /* Spectral tables in the reference's original band order, as read by readT

@@ -31,6 +31,9 @@ __attribute__((constructor)) static void radchan_params_from_macros(void) {
 #if defined(random)
     prm.sampler = (random == 1) ? 1 : 0;   /* param.h:62-63: 1 Sobol, other pseudo-random */
 #endif
+#if defined(floatpoint)
+    prm.fp64 = (floatpoint == 1) ? 1 : 0;   /* param.h:64-65: 1 double, 0 float */
+#endif
     if (rc_set_params(&prm) != 0)
         fprintf(stderr, "radchan: compiled-in parameters rejected: %s\n", rc_last_error());
 }

@@ -183,6 +183,78 @@ inline void expm1_exp_f(float x, float *em1, float *e) {
     *em1 = (ni == 0) ? p : fmaf(sc, p, sc - 1.0f);
 }
 
+/* ---------------- FP64 project math (the full-FP64 path, floatpoint 1) ----
+ * The same operation sequence compiles on the GPU, so the FP64 paths agree
+ * bit for bit; accuracy against libm is a test (a few ulp). */
+inline double bits2d(uint64_t b) { double d; std::memcpy(&d, &b, 8); return d; }
+
+/* expm1(x) and exp(x) for x <= 0: x = n ln2 + r (Cody-Waite, two fma
+ * steps), |r| <= ln2/2, expm1(r) by its Taylor series to r^13/13! (Horner
+ * with fma), then scaled by 2^n (two factors below the normal range). */
+inline void expm1_exp_d(double x, double *em1, double *e) {
+    if (!(x < 0.0)) { *em1 = x; *e = 1.0; return; }          /* x = 0 (or -0) */
+    if (x < -745.5) { *em1 = -1.0; *e = 0.0; return; }
+    const double n = std::floor(x * 1.4426950408889634 + 0.5);
+    double r = std::fma(-n, 6.93147180369123816490e-01, x);
+    r = std::fma(-n, 1.90821492927058770002e-10, r);
+    double p = std::fma(r, 1.0 / 6227020800.0, 1.0 / 479001600.0);
+    p = std::fma(r, p, 1.0 / 39916800.0);
+    p = std::fma(r, p, 1.0 / 3628800.0);
+    p = std::fma(r, p, 1.0 / 362880.0);
+    p = std::fma(r, p, 1.0 / 40320.0);
+    p = std::fma(r, p, 1.0 / 5040.0);
+    p = std::fma(r, p, 1.0 / 720.0);
+    p = std::fma(r, p, 1.0 / 120.0);
+    p = std::fma(r, p, 1.0 / 24.0);
+    p = std::fma(r, p, 1.0 / 6.0);
+    p = std::fma(r, p, 0.5);
+    p = std::fma(r * r, p, r);                                /* expm1(r) */
+    const int ni = (int)n;                                    /* -1075 .. 0 */
+    if (ni < -1000) {                                         /* exp(x) < 2^-1000: scaled twice */
+        const double s1 = bits2d((uint64_t)(ni + 600 + 1023) << 52);
+        *e = std::fma(s1, p, s1) * bits2d((uint64_t)(1023 - 600) << 52);
+        *em1 = -1.0;
+        return;
+    }
+    const double s1 = bits2d((uint64_t)(ni + 1023) << 52);
+    *e = std::fma(s1, p, s1);
+    *em1 = (ni == 0) ? p : std::fma(s1, p, s1 - 1.0);
+}
+
+/* sin and cos of 2 pi u for u in [0,1] given in single precision: 4u and
+ * its quadrant split are exact; the quadrant-local angle f pi/2, |f| <= 1/2,
+ * by Taylor polynomials to degree 17 / 18. */
+inline void sincos2pi_d(float u, double *s, double *c) {
+    const double x = 4.0 * (double)u;
+    const double m = std::floor(x + 0.5);
+    const double f = x - m;                                   /* exact, in [-1/2, 1/2] */
+    const double y = f * 1.5707963267948966;
+    const double y2 = y * y;
+    double ps = std::fma(y2, 1.0 / 355687428096000.0, -1.0 / 1307674368000.0);
+    ps = std::fma(y2, ps, 1.0 / 6227020800.0);
+    ps = std::fma(y2, ps, -1.0 / 39916800.0);
+    ps = std::fma(y2, ps, 1.0 / 362880.0);
+    ps = std::fma(y2, ps, -1.0 / 5040.0);
+    ps = std::fma(y2, ps, 1.0 / 120.0);
+    ps = std::fma(y2, ps, -1.0 / 6.0);
+    const double sp = std::fma(y * y2, ps, y);
+    double pc = std::fma(y2, 1.0 / 6402373705728000.0, -1.0 / 20922789888000.0);
+    pc = std::fma(y2, pc, 1.0 / 87178291200.0);
+    pc = std::fma(y2, pc, -1.0 / 479001600.0);
+    pc = std::fma(y2, pc, 1.0 / 3628800.0);
+    pc = std::fma(y2, pc, -1.0 / 40320.0);
+    pc = std::fma(y2, pc, 1.0 / 720.0);
+    pc = std::fma(y2, pc, -1.0 / 24.0);
+    pc = std::fma(y2, pc, 0.5);
+    const double cp = std::fma(-y2, pc, 1.0);
+    switch ((int)m & 3) {
+        case 0: *s = sp;  *c = cp;  break;
+        case 1: *s = cp;  *c = -sp; break;
+        case 2: *s = -sp; *c = -cp; break;
+        default: *s = -cp; *c = sp; break;
+    }
+}
+
 /* Planck spectral intensity per unit wavenumber (nu in cm^-1), computed in
  * FP64 and rounded once; I_blackC, memory_copy_noint.cu:288-297. */
 inline float planck_f(float T, float nu) {
@@ -461,12 +533,14 @@ int march_level(const orc_ctx &c, int lev, Ray<G> &r, const Energy &e, float tol
             kb_d(c, e, (double)Tc, &K64, &B64);
             const double x64 = K64 * (double)ds;
             const double fac64 = (B64 - e.Bi64) / e.Bi64;
-            const double wta64 = e.W64 * r.tra64 * (-std::expm1(-x64));
+            double em64, ex64;
+            expm1_exp_d(-x64, &em64, &ex64);
+            const double wta64 = e.W64 * r.tra64 * (-em64);
             const double term64 = wta64 * fac64;
             t.de64 -= term64;
             t.sumabs += std::fabs(wta64) * (std::fabs(B64) / e.Bi64 + 1.0);
             r.tra = (G)((float)r.tra * ex);
-            r.tra64 *= std::exp(-x64);
+            r.tra64 *= ex64;
             if ((float)r.tra < tol_dump) {
                 t.de -= (e.W * (float)r.tra) * fac;
                 const double d64 = e.W64 * r.tra64 * fac64;
@@ -480,11 +554,13 @@ int march_level(const orc_ctx &c, int lev, Ray<G> &r, const Energy &e, float tol
             kb_d(c, e, (double)Tc, &K, &B);
             const double x = K * (double)ds;
             const double fac = (B - e.Bi64) / e.Bi64;
-            const double wta = e.W64 * (double)r.tra * (-std::expm1(-x));
+            double em, ex;
+            expm1_exp_d(-x, &em, &ex);
+            const double wta = e.W64 * (double)r.tra * (-em);
             const double term = wta * fac;
             t.de64 -= term;
             t.sumabs += std::fabs(wta) * (std::fabs(B) / e.Bi64 + 1.0);
-            r.tra = (G)((double)r.tra * std::exp(-x));
+            r.tra = (G)((double)r.tra * ex);
             if ((double)r.tra < (double)tol_dump) {
                 const double d = e.W64 * (double)r.tra * fac;
                 t.de64 -= d;
@@ -685,8 +761,9 @@ void fluid_ray(const orc_ctx &c, int i, int j, int k, int v, int h, int rank, in
     } else {
         const double ct = 1.0 - 2.0 * (double)u[1];
         const double st = 2.0 * std::sqrt((double)u[1] * (1.0 - (double)u[1]));
-        const double ph = 2.0 * kPi * (double)u[0];
-        sx = (G)ct; sy = (G)(st * std::cos(ph)); sz = (G)(st * std::sin(ph));
+        double sn, cs;
+        sincos2pi_d(u[0], &sn, &cs);
+        sx = (G)ct; sy = (G)(st * cs); sz = (G)(st * sn);
     }
     /* zero components replaced (:132-137) */
     if (sx == 0) sx = (G)1e-10f;
@@ -727,10 +804,11 @@ void surface_ray(const orc_ctx &c, int bound, int j, int k, int v, int h, int ra
         sx = (G)s[0]; sy = (G)s[1]; sz = (G)s[2];
     } else {
         const double a = std::sqrt(1.0 - (double)u[1]), b = std::sqrt((double)u[1]);
-        const double ph = 2.0 * kPi * (double)u[0];
+        double sn, cs;
+        sincos2pi_d(u[0], &sn, &cs);
         const double st = bound ? a : b;
         sx = bound ? (G)(-b) : (G)a;
-        sy = (G)(st * std::cos(ph)); sz = (G)(st * std::sin(ph));
+        sy = (G)(st * cs); sz = (G)(st * sn);
     }
     /* near-zero components (:1395-1400) */
     if (std::fabs((double)sx) <= 1e-10) sx = (G)((float)(1 - 2 * bound) * 1e-10f);
@@ -778,6 +856,8 @@ void orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_t ou
 float orc_u01(uint32_t x) { return u01f(x); }
 void orc_sincos2pi(float u, float *s, float *c) { sincos2pi_f(u, s, c); }
 void orc_expm1_exp(float x, float *em1, float *e) { expm1_exp_f(x, em1, e); }
+void orc_sincos2pi_d(float u, double *s, double *c) { sincos2pi_d(u, s, c); }
+void orc_expm1_exp_d(double x, double *em1, double *e) { expm1_exp_d(x, em1, e); }
 float orc_planck(float T, float nu) { return planck_f(T, nu); }
 int orc_upper_bound(const float *cdf, int n, float R) { return upper_bound_f(cdf, n, R); }
 

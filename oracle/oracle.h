@@ -103,6 +103,10 @@ void     orc_philox4x32_10(const uint32_t ctr[4], const uint32_t key[2], uint32_
 float    orc_u01(uint32_t x);
 void     orc_sincos2pi(float u, float *s, float *c);
 void     orc_expm1_exp(float x, float *em1, float *e);
+/* the full-FP64 path's project math: sin/cos of 2 pi u (u given in single
+ * precision) and expm1/exp for x <= 0 */
+void     orc_sincos2pi_d(float u, double *s, double *c);
+void     orc_expm1_exp_d(double x, double *em1, double *e);
 float    orc_planck(float T, float nu);
 int      orc_upper_bound(const float *cdf, int n, float R);
 void     orc_sort_idx(const float *kavg, int nB, int32_t *idx);

@@ -34,6 +34,8 @@ def load():
         "orc_u01": (f, [C.c_uint32]),
         "orc_sincos2pi": (None, [f, P(f), P(f)]),
         "orc_expm1_exp": (None, [f, P(f), P(f)]),
+        "orc_sincos2pi_d": (None, [f, P(C.c_double), P(C.c_double)]),
+        "orc_expm1_exp_d": (None, [C.c_double, P(C.c_double), P(C.c_double)]),
         "orc_planck": (f, [f, f]),
         "orc_upper_bound": (C.c_int, [P(f), C.c_int, f]),
         "orc_sort_idx": (None, [P(f), C.c_int, P(C.c_int32)]),
@@ -114,6 +116,18 @@ def sincos2pi(u: float):
 def expm1_exp(x: float):
     a, b = C.c_float(), C.c_float()
     load().orc_expm1_exp(x, C.byref(a), C.byref(b))
+    return a.value, b.value
+
+
+def sincos2pi_d(u: float):
+    s, c = C.c_double(), C.c_double()
+    load().orc_sincos2pi_d(u, C.byref(s), C.byref(c))
+    return s.value, c.value
+
+
+def expm1_exp_d(x: float):
+    a, b = C.c_double(), C.c_double()
+    load().orc_expm1_exp_d(x, C.byref(a), C.byref(b))
     return a.value, b.value
 
 

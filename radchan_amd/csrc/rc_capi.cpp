@@ -105,9 +105,11 @@ inline uint16_t to_bf16(float x) {
 
 struct SnapState {
     float Tmax = 0.f, kappamax = 0.f, Ibmax = 0.f, Ibmax_w = 0.f, Ibmax_e = 0.f;
+    double kappamax64 = 0.0, Ibmax64 = 0.0, Ibmax_w64 = 0.0, Ibmax_e64 = 0.0;   /* the FP64 path's */
     int tm = 0, tstar = 0;
     DevBuf T0, cdf_b, cdf_g, cdf_bs, guide_b, guide_bs;
     DevBuf Pall;                          /* table positions of every level (DevLevel.P) */
+    DevBuf Tall;                          /* FP64 path: the temperatures behind Pall */
     std::unique_ptr<TableSet> grey;     /* grey runs: tables follow Tmax */
     Tables greyhost;
 };
@@ -250,12 +252,19 @@ int snapshot_scalars(const rc_params &p, const Tables &t, SnapState &s) {
         return -5;
     }
     const int tm = s.tm;
-    if (p.grey) s.kappamax = p.abscoeff;
-    else s.kappamax = (t.kP[tm + 1] - t.kP[tm]) / (t.Tnb[tm + 1] - t.Tnb[tm]) * (s.Tmax - t.Tnb[tm]) + t.kP[tm];
+    if (p.grey) { s.kappamax = p.abscoeff; s.kappamax64 = p.abscoeff; }
+    else {
+        s.kappamax = (t.kP[tm + 1] - t.kP[tm]) / (t.Tnb[tm + 1] - t.Tnb[tm]) * (s.Tmax - t.Tnb[tm]) + t.kP[tm];
+        s.kappamax64 = ((double)t.kP[tm + 1] - t.kP[tm]) / ((double)t.Tnb[tm + 1] - t.Tnb[tm]) *
+                       ((double)s.Tmax - t.Tnb[tm]) + (double)t.kP[tm];
+    }
     const double T4 = (double)s.Tmax * s.Tmax * s.Tmax * s.Tmax;
     s.Ibmax = (float)(4.0 * (double)s.kappamax * T4 * kStefan / p.photmax);
-    s.Ibmax_w = (float)((double)p.eps[FW] * kStefan * T4 / p.photmax);
-    s.Ibmax_e = (float)((double)p.eps[FE] * kStefan * T4 / p.photmax);
+    s.Ibmax64 = 4.0 * s.kappamax64 * T4 * kStefan / p.photmax;
+    s.Ibmax_w64 = (double)p.eps[FW] * kStefan * T4 / p.photmax;
+    s.Ibmax_e64 = (double)p.eps[FE] * kStefan * T4 / p.photmax;
+    s.Ibmax_w = (float)s.Ibmax_w64;
+    s.Ibmax_e = (float)s.Ibmax_e64;
     s.tstar = ((s.Tmax - t.Tnb[tm]) < (t.Tnb[tm + 1] - s.Tmax)) ? tm : tm + 1;
     return 0;
 }
@@ -312,10 +321,14 @@ int snapshot_exact(const rc_params &p, const Tables &t, SnapState &s, std::vecto
     cb[nB - 1] = 1.0f;
     cs[nB - 1] = 1.0f;
     const double T4 = (double)s.Tmax * s.Tmax * s.Tmax * s.Tmax;
-    s.kappamax = (float)(kPi * Z / (kStefan * T4));
-    s.Ibmax = (float)(4.0 * kPi * Z / p.photmax);
-    s.Ibmax_w = (float)((double)p.eps[FW] * kPi * Zs / p.photmax);
-    s.Ibmax_e = (float)((double)p.eps[FE] * kPi * Zs / p.photmax);
+    s.kappamax64 = kPi * Z / (kStefan * T4);
+    s.kappamax = (float)s.kappamax64;
+    s.Ibmax64 = 4.0 * kPi * Z / p.photmax;
+    s.Ibmax = (float)s.Ibmax64;
+    s.Ibmax_w64 = (double)p.eps[FW] * kPi * Zs / p.photmax;
+    s.Ibmax_e64 = (double)p.eps[FE] * kPi * Zs / p.photmax;
+    s.Ibmax_w = (float)s.Ibmax_w64;
+    s.Ibmax_e = (float)s.Ibmax_e64;
     return 0;
 }
 
@@ -390,6 +403,10 @@ int push_snaps(rc_ctx *c) {
         d.T0 = S.T0.as<float>();
         d.xu_all = c->xuall.as<float>();
         d.P_all = S.Pall.as<float>();
+        d.T_all = S.Tall.as<float>();
+        d.Ibmax64 = S.Ibmax64; d.Ibmax_w64 = S.Ibmax_w64; d.Ibmax_e64 = S.Ibmax_e64;
+        d.Tnb0d = (double)ts.Tnb[0];
+        d.invdTd = 1.0 / (double)(ts.Tnb[1] - ts.Tnb[0]);
         d.nxu = c->nxu;
         d.npos = (int)c->npos;
         d.x0 = c->x0.as<float>(); d.y0 = c->y0.as<float>(); d.z0 = c->z0.as<float>();
@@ -650,10 +667,14 @@ int rc_set_fields(rc_ctx *c, int nsnap, const float *const *T, int zstart) {
         RC_HIP(hipMemcpyAsync(S.T0.p, T[s], nT0 * 4, hipMemcpyHostToDevice, c->stream));
         const TableSet &tset = S.grey ? *S.grey : c->nbset;
         if (int e = S.Pall.ensure(c->npos * 4)) return e;
+        const bool f64 = p.fp64 == 1;
+        if (f64)
+            if (int e = S.Tall.ensure(c->npos * 4)) return e;
         for (int g = 0; g < p.grid_num; ++g) {
             const size_t nl = (size_t)p.maxi[g] * p.maxj[g] * p.maxk[g];
             float *Pg = S.Pall.as<float>() + c->poff[g];
-            launch_level_P(c->stream, S.T0.as<float>(), p, g, tset.Tnb0, tset.invdT, p.nT, Pg);
+            float *Tg = f64 ? S.Tall.as<float>() + c->poff[g] : nullptr;
+            launch_level_P(c->stream, S.T0.as<float>(), p, g, tset.Tnb0, tset.invdT, p.nT, Pg, Tg);
             if (c->restriction == 1 && g > 0) {
                 /* cells and adiabatic x-wall faces by cubic splines (then to
                  * positions); bd=4 z walls keep the texture mean */
@@ -666,7 +687,7 @@ int rc_set_fields(rc_ctx *c, int nsnap, const float *const *T, int zstart) {
                                     c->cz[0].as<float>(), c->cx[g].as<float>(), c->cy[g].as<float>(),
                                     c->cz[g].as<float>(), p.imax, p.jmax, p.kmax, p.maxi[g], p.maxj[g], p.maxk[g],
                                     tl, wx ? tw : nullptr, c->spl.as<float>());
-                launch_pos_from_T(c->stream, p, g, tl, wx ? tw : nullptr, tset.Tnb0, tset.invdT, p.nT, Pg);
+                launch_pos_from_T(c->stream, p, g, tl, wx ? tw : nullptr, tset.Tnb0, tset.invdT, p.nT, Pg, Tg);
             }
         }
         RC_HIP(hipGetLastError());
@@ -707,6 +728,10 @@ static int enqueue_solve(rc_ctx *c, int variant, int *launches, bool timed) {
                     launch_fluid_bands(c->stream, cm, ds, s, v, c->keys.as<uint16_t>(), c->bacc.as<float>(), de,
                                        c->stats.as<unsigned long long>());
                     n += p.nB + 2;
+                } else if (p.fp64 == 1) {
+                    launch_fluid_f64(c->stream, cm, ds, s, v, cm.srt ? c->keys.as<uint16_t>() : nullptr, de,
+                                     c->stats.as<unsigned long long>());
+                    ++n;
                 } else {
                     launch_fluid_plain(c->stream, cm, ds, s, v, cm.srt ? c->keys.as<uint16_t>() : nullptr, de,
                                        c->stats.as<unsigned long long>());
@@ -717,8 +742,12 @@ static int enqueue_solve(rc_ctx *c, int variant, int *launches, bool timed) {
                 double *dw = c->ds.as<double>() + (size_t)s * 2 * c->nsurf * p.nVar;
                 for (int v = 0; v < p.nVar; ++v) {
                     if (cm.srt) { launch_sort(c->stream, cm, ds, s, v, 1, 2 * c->nsurf, nullptr, c->keys.as<uint16_t>()); ++n; }
-                    launch_surface_plain(c->stream, cm, ds, s, v, cm.srt ? c->keys.as<uint16_t>() : nullptr, dw,
-                                         c->stats.as<unsigned long long>());
+                    if (p.fp64 == 1)
+                        launch_surface_f64(c->stream, cm, ds, s, v, cm.srt ? c->keys.as<uint16_t>() : nullptr, dw,
+                                           c->stats.as<unsigned long long>());
+                    else
+                        launch_surface_plain(c->stream, cm, ds, s, v, cm.srt ? c->keys.as<uint16_t>() : nullptr,
+                                             dw, c->stats.as<unsigned long long>());
                     ++n;
                 }
             }
@@ -774,6 +803,10 @@ int rc_run(rc_ctx *c, int variant) {
         return -2;
     }
     const rc_params &p = c->p;
+    if (p.fp64 == 1 && variant != RC_VARIANT_PLAIN) {
+        set_error("floatpoint 1 (double precision) runs in the plain variant (RC_VARIANT_PLAIN)");
+        return -2;
+    }
     if (variant == RC_VARIANT_PLAIN_BANDS && !(p.srt == 1 && !p.grey)) {
         set_error("the per-band plain variant needs sorted band keys (srt=1, narrow band)");
         return -2;
@@ -1122,6 +1155,11 @@ rc_group *rc_group_create(rc_ctx *const *cs, int n) {
     if (!cs || n < 1) { set_error("rc_group_create: no sessions"); return nullptr; }
     for (int i = 0; i < n; ++i) {
         if (!cs[i]) { set_error("rc_group_create: null session"); return nullptr; }
+        if (cs[i]->p.fp64 == 1) {
+            set_error("rc_group_create: a group runs the batched kernel, and floatpoint 1 (double precision) runs "
+                      "in the plain variant");
+            return nullptr;
+        }
         if (group_compatible(cs[0], cs[i])) {
             set_error("rc_group_create: the sessions differ in a parameter other than the grid (nVar, photmax, "
                       "srt, table sizes, tolerances, boundary codes, domain lengths, seed or table precision)");
@@ -1389,7 +1427,8 @@ void mc_gpu_(float *Tfort, int *zstart, int *rank) {
     if (rc_set_estimator(g_ctx, g_estimator) || rc_set_restriction(g_ctx, g_restriction)) { report("mc_gpu_"); return; }
     const float *T = Tfort;
     if (rc_set_fields(g_ctx, 1, &T, *zstart)) { report("mc_gpu_"); return; }
-    if (rc_run(g_ctx, g_variant)) { report("mc_gpu_"); return; }
+    /* floatpoint 1: the full-FP64 kernels, which run in the plain launch shape */
+    if (rc_run(g_ctx, p.fp64 == 1 ? RC_VARIANT_PLAIN : g_variant)) { report("mc_gpu_"); return; }
     g_pending = true;
 }
 

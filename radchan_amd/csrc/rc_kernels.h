@@ -97,6 +97,10 @@ struct DevSnap {
     const float *P_all;            /* table positions of every level, g at lev[g].poff */
     int nxu, npos;                 /* their lengths                       */
     const float *x0, *y0, *z0;     /* level-0 cell centres                */
+    /* the full-FP64 path (floatpoint 1): the temperatures behind P_all (same
+     * layout), the weights in double and the table's temperature map */
+    const float *T_all;
+    double Ibmax64, Ibmax_w64, Ibmax_e64, Tnb0d, invdTd;
     /* the snapshot's grid (snapshots of one batched launch may differ in
      * it): global sizes, first z plane and counts of emitting cells and
      * wall cells per wall; ubase / sbase: the launch's emitting (wall)
@@ -111,11 +115,11 @@ struct DevSnap {
  * from the ghost-padded field T0; the table's first temperature, inverse
  * spacing and row count map temperatures to positions */
 void launch_level_P(hipStream_t st, const float *T0, const rc_params &p, int g, float Tnb0, float invdT, int nT,
-                    float *P);
+                    float *P, float *Tout = nullptr);
 /* cell and x-wall temperatures of level g (Tl [km][jm][im], Tw [2][km][jm]
  * or NULL) converted to positions in P (the spline restriction's output) */
 void launch_pos_from_T(hipStream_t st, const rc_params &p, int g, const float *Tl, const float *Tw, float Tnb0,
-                       float invdT, int nT, float *P);
+                       float invdT, int nT, float *P, float *Tout = nullptr);
 /* cubic-spline restriction of level g (interp3D): cells Tl [km][jm][im] and,
  * if Tw, x-wall faces [2][km][jm]; scratch of spline_scratch_floats floats */
 size_t spline_scratch_floats(int imax, int jmax, int kmax, int im, int jm, int km);
@@ -132,6 +136,12 @@ void launch_fluid_bands(hipStream_t st, const DevCommon &cm, const DevSnap *snap
                         float *acc, double *de, unsigned long long *stats);
 void launch_surface_plain(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v,
                           const uint16_t *keys, double *de, unsigned long long *stats);
+/* the full-FP64 variant (floatpoint 1) in the plain launch shape: FP64
+ * geometry, energy and cell sums; the snapshot's T_all must be set */
+void launch_fluid_f64(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v, const uint16_t *keys,
+                      double *de, unsigned long long *stats);
+void launch_surface_f64(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v,
+                        const uint16_t *keys, double *de, unsigned long long *stats);
 void launch_trace(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int surface, const int32_t *rays,
                   const int32_t *unit, int n, const uint16_t *keys, rc_ray *out);
 void launch_finalize(hipStream_t st, const double *de, int n, int nVar, float *mean, float *sdev);

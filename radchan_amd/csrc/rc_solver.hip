@@ -39,6 +39,7 @@
 #include "rc_internal.h"
 #include "rc_kernels.h"
 #include "rc_march.h"
+#include "rc_march64.h"
 
 using namespace rcd;
 
@@ -104,7 +105,7 @@ FaceModes face_modes(const rc_params &p) {
  * their face's fixed position */
 __global__ void __launch_bounds__(256) k_level_P(const float *__restrict__ T0, int imax, int jmax, int kmax, int im,
                                                  int jm, int km, FaceModes fm, float Tnb0, float invdT, int nT,
-                                                 float *__restrict__ P) {
+                                                 float *__restrict__ P, float *__restrict__ Tout) {
     const int rx = imax / im, ry = jmax / jm, rz = kmax / km;
     const int nx = imax + 2, ny = jmax + 2, nz = kmax + 2;
     const size_t ncell = (size_t)im * jm * km;
@@ -113,11 +114,13 @@ __global__ void __launch_bounds__(256) k_level_P(const float *__restrict__ T0, i
     for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total; id += (size_t)gridDim.x * blockDim.x) {
         if (id < ncell) {
             const int i = (int)(id % im) + 1, j = (int)((id / im) % jm) + 1, k = (int)(id / ((size_t)im * jm)) + 1;
-            P[id] = table_pos(tex3_T(T0, nx, ny, nz, tmap(i, rx), tmap(j, ry), tmap(k, rz)), Tnb0, invdT, nT);
+            const float Tc = tex3_T(T0, nx, ny, nz, tmap(i, rx), tmap(j, ry), tmap(k, rz));
+            P[id] = table_pos(Tc, Tnb0, invdT, nT);
+            if (Tout) Tout[id] = Tc;
             continue;
         }
         size_t q = id - ncell;
-        float v = 0.f;
+        float v = 0.f, tw = 0.f;
         if (q < 2 * nfx) {
             const int b = (int)(q / nfx);
             const size_t r = q % nfx;
@@ -125,7 +128,8 @@ __global__ void __launch_bounds__(256) k_level_P(const float *__restrict__ T0, i
             const int mode = fm.m[FW + b];
             if (mode == 1) {
                 const float tx = b ? (float)imax + 1.0f : 1.0f;
-                v = table_pos(tex3_T(T0, nx, ny, nz, tx, tmap(j, ry), tmap(k, rz)), Tnb0, invdT, nT);
+                tw = tex3_T(T0, nx, ny, nz, tx, tmap(j, ry), tmap(k, rz));
+                v = table_pos(tw, Tnb0, invdT, nT);
             } else if (mode == 2) {
                 v = fixed_wall_pos(nT, FW + b);
             }
@@ -140,28 +144,35 @@ __global__ void __launch_bounds__(256) k_level_P(const float *__restrict__ T0, i
             const int mode = fm.m[FB + b];
             if (mode == 1) {
                 const float tz = b ? (float)kmax + 0.5f : 1.5f;
-                v = table_pos(tex3_T(T0, nx, ny, nz, tmap(i, rx), tmap(j, ry), tz), Tnb0, invdT, nT);
+                tw = tex3_T(T0, nx, ny, nz, tmap(i, rx), tmap(j, ry), tz);
+                v = table_pos(tw, Tnb0, invdT, nT);
             } else if (mode == 2) {
                 v = fixed_wall_pos(nT, FB + b);
             }
         }
         P[id] = v;
+        if (Tout) Tout[id] = tw;
     }
 }
 
 /* cell (and adiabatic x-wall) temperatures of a level, e.g. the spline
- * restriction's, to table positions in P */
+ * restriction's, to table positions in P (and, for the FP64 path, kept in
+ * Tout beside them) */
 __global__ void __launch_bounds__(256) k_pos_from_T(const float *__restrict__ Tl, const float *__restrict__ Tw,
                                                     int im, int jm, int km, FaceModes fm, float Tnb0, float invdT,
-                                                    int nT, float *__restrict__ P) {
+                                                    int nT, float *__restrict__ P, float *__restrict__ Tout) {
     const size_t ncell = (size_t)im * jm * km, nfx = (size_t)jm * km;
     const size_t total = ncell + (Tw ? 2 * nfx : 0);
     for (size_t id = blockIdx.x * (size_t)blockDim.x + threadIdx.x; id < total; id += (size_t)gridDim.x * blockDim.x) {
         if (id < ncell) {
             P[id] = table_pos(Tl[id], Tnb0, invdT, nT);
+            if (Tout) Tout[id] = Tl[id];
         } else {
             const size_t q = id - ncell;
-            if (fm.m[FW + (int)(q / nfx)] == 1) P[id] = table_pos(Tw[q], Tnb0, invdT, nT);
+            if (fm.m[FW + (int)(q / nfx)] == 1) {
+                P[id] = table_pos(Tw[q], Tnb0, invdT, nT);
+                if (Tout) Tout[id] = Tw[q];
+            }
         }
     }
 }
@@ -337,6 +348,70 @@ __global__ void __launch_bounds__(128) k_surface_plain(DevCommon cm, const DevSn
     }
 }
 
+/* ------------------------------------------------------------------------ */
+/* full-FP64 variant (floatpoint 1): the reference's launch shape, a thread  */
+/* per emitting (wall) cell, rays in sequence, FP64 per-cell sums             */
+__global__ void __launch_bounds__(128) k_fluid_f64(DevCommon cm, const DevSnap *__restrict__ snaps, int s, int v,
+                                                   const uint16_t *__restrict__ keys, double *__restrict__ de,
+                                                   unsigned long long *__restrict__ stats) {
+    const DevSnap &sn = snaps[s];
+    int64_t steps = 0, rays = 0;
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < cm.ncell; n += gridDim.x * blockDim.x) {
+        int i, j, k;
+        cell_ijk(cm, n, i, j, k);
+        const uint32_t cid = (uint32_t)n + (uint32_t)cm.imax * cm.jmax * (cm.zstart - 1);
+        const float Ti = cell_T(cm, sn, i, j, k);
+        double acc = 0.0;
+        int st = 0;
+        for (int h = 0; h < cm.photmax; ++h) {
+            const uint32_t key = keys ? (uint32_t)keys[(size_t)n * cm.photmax + h] : 0u;
+            acc += fluid_ray64(cm, sn, i, j, k, cid, v, h, key, keys != nullptr, Ti, st);
+        }
+        steps += st;
+        rays += cm.photmax;
+        de[(size_t)n * cm.nVar + v] = acc;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        steps += __shfl_xor(steps, o);
+        rays += __shfl_xor(rays, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], (unsigned long long)rays);
+        atomicAdd(&stats[1], (unsigned long long)steps);
+    }
+}
+
+__global__ void __launch_bounds__(128) k_surface_f64(DevCommon cm, const DevSnap *__restrict__ snaps, int s, int v,
+                                                     const uint16_t *__restrict__ keys, double *__restrict__ de,
+                                                     unsigned long long *__restrict__ stats) {
+    const DevSnap &sn = snaps[s];
+    int64_t steps = 0, rays = 0;
+    for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < 2 * cm.nsurf; n += gridDim.x * blockDim.x) {
+        const int bound = n / cm.nsurf, w = n % cm.nsurf;
+        if (!cm.adiab[bound]) continue;
+        const int j = w % cm.jmax + 1, k = w / cm.jmax + cm.zstart;
+        const uint32_t cid = (uint32_t)(w + cm.jmax * (cm.zstart - 1));
+        const float Tw = wall_T(cm, sn, bound, j, k);
+        double acc = 0.0;
+        int st = 0;
+        for (int h = 0; h < cm.photmax; ++h) {
+            const uint32_t key = keys ? (uint32_t)keys[(size_t)n * cm.photmax + h] : 0u;
+            acc += surface_ray64(cm, sn, bound, j, k, cid, v, h, key, keys != nullptr, Tw, st);
+        }
+        steps += st;
+        rays += cm.photmax;
+        de[(size_t)n * cm.nVar + v] = acc;
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        steps += __shfl_xor(steps, o);
+        rays += __shfl_xor(rays, o);
+    }
+    if ((threadIdx.x & 63) == 0) {
+        atomicAdd(&stats[0], (unsigned long long)rays);
+        atomicAdd(&stats[1], (unsigned long long)steps);
+    }
+}
+
 /* traced rays: rays[5*q] = (i or bound, j, k, v, h); unit[q] indexes keys */
 __global__ void __launch_bounds__(64) k_trace(DevCommon cm, const DevSnap *__restrict__ snaps, int surface,
                                               const int32_t *__restrict__ rays, const int32_t *__restrict__ unit,
@@ -383,19 +458,19 @@ __global__ void k_finalize(const double *__restrict__ de, int n, int nVar, float
 /* launchers used by the session code                                        */
 
 void launch_level_P(hipStream_t st, const float *T0, const rc_params &p, int g, float Tnb0, float invdT, int nT,
-                    float *P) {
+                    float *P, float *Tout) {
     const size_t n = level_pos_count(p.maxi[g], p.maxj[g], p.maxk[g]);
     const int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
     hipLaunchKernelGGL(k_level_P, dim3(blocks), dim3(256), 0, st, T0, p.imax, p.jmax, p.kmax, p.maxi[g], p.maxj[g],
-                       p.maxk[g], face_modes(p), Tnb0, invdT, nT, P);
+                       p.maxk[g], face_modes(p), Tnb0, invdT, nT, P, Tout);
 }
 
 void launch_pos_from_T(hipStream_t st, const rc_params &p, int g, const float *Tl, const float *Tw, float Tnb0,
-                       float invdT, int nT, float *P) {
+                       float invdT, int nT, float *P, float *Tout) {
     const size_t n = (size_t)p.maxi[g] * p.maxj[g] * p.maxk[g] + (size_t)2 * p.maxj[g] * p.maxk[g];
     const int blocks = (int)std::min<size_t>((n + 255) / 256, 65536);
     hipLaunchKernelGGL(k_pos_from_T, dim3(blocks), dim3(256), 0, st, Tl, Tw, p.maxi[g], p.maxj[g], p.maxk[g],
-                       face_modes(p), Tnb0, invdT, nT, P);
+                       face_modes(p), Tnb0, invdT, nT, P, Tout);
 }
 
 void launch_sort(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v, int surface, int nunit,
@@ -427,6 +502,18 @@ void launch_surface_plain(hipStream_t st, const DevCommon &cm, const DevSnap *sn
                           const uint16_t *keys, double *de, unsigned long long *stats) {
     const int blocks = std::max(1, std::min((2 * cm.nsurf + 127) / 128, 1 << 20));
     hipLaunchKernelGGL(k_surface_plain, dim3(blocks), dim3(128), 0, st, cm, snaps, s, v, keys, de, stats);
+}
+
+void launch_fluid_f64(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v, const uint16_t *keys,
+                      double *de, unsigned long long *stats) {
+    const int blocks = std::max(1, std::min((cm.ncell + 127) / 128, 1 << 20));
+    hipLaunchKernelGGL(k_fluid_f64, dim3(blocks), dim3(128), 0, st, cm, snaps, s, v, keys, de, stats);
+}
+
+void launch_surface_f64(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v,
+                        const uint16_t *keys, double *de, unsigned long long *stats) {
+    const int blocks = std::max(1, std::min((2 * cm.nsurf + 127) / 128, 1 << 20));
+    hipLaunchKernelGGL(k_surface_f64, dim3(blocks), dim3(128), 0, st, cm, snaps, s, v, keys, de, stats);
 }
 
 void launch_trace(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int surface, const int32_t *rays,

@@ -104,6 +104,7 @@ void default_params(rc_params &p) {
     p.abscoeff = 1.f; p.Tmins = 490.f; p.wvmax = 5000.f;
     p.seed = 1234;
     p.sampler = 1;      /* param.h:62-63: random 1 (Sobol directions) */
+    p.fp64 = 0;   /* param.h:64-65: floatpoint 0 (single precision) */
 }
 
 int check_params(const rc_params &p) {
@@ -141,6 +142,7 @@ int check_params(const rc_params &p) {
         return fail("toll and tollS must lie in (0, 1)");
     if (p.calcwalls == 1 && !(p.adiabw == 1 || p.adiabe == 1)) return fail("calcwalls needs an adiabatic x wall");
     if (p.sampler != 0 && p.sampler != 1) return fail("sampler: 0 (Philox) or 1 (Sobol)");
+    if (p.fp64 != 0 && p.fp64 != 1) return fail("floatpoint: 0 (single) or 1 (double precision)");
     return 0;
 }
 

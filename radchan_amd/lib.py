@@ -37,7 +37,7 @@ class Params(C.Structure):
         ("nT", C.c_int32), ("nQ", C.c_int32), ("nB", C.c_int32),
         ("abscoeff", C.c_float), ("Tmins", C.c_float), ("wvmax", C.c_float),
         ("seed", C.c_uint64),
-        ("sampler", C.c_int32), ("pad_", C.c_int32),
+        ("sampler", C.c_int32), ("fp64", C.c_int32),
     ]
 
     def copy(self) -> "Params":

@@ -9,9 +9,9 @@
 extern "C" int shim_check_values(int32_t *out);
 
 int main() {
-    int32_t v[16];
+    int32_t v[18];
     shim_check_values(v);
-    for (int i = 0; i < 16; ++i) printf("%d ", v[i]);
+    for (int i = 0; i < 18; ++i) printf("%d ", v[i]);
     printf("\n");
     return 0;
 }

@@ -114,4 +114,5 @@ def test_param_shim_against_reference_headers(tmp_path):
                     "-Wl,-rpath," + os.path.dirname(L.LIB_PATH), "-o", str(exe)], check=True)
     v = [int(x) for x in subprocess.run([str(exe)], capture_output=True, text=True, check=True).stdout.split()]
     # imax jmax kmax p_col photmax grid_num maxi[5] maxj[8] maxk[8] maxs[8] nB nT nQ bdb srt toll*1e6
-    assert v == [406, 576, 2304, 48, 6000, 9, 14, 3, 9, 100000, 999, 135, 16, 4, 1, 1000]
+    # sampler (random 1: Sobol) fp64 (floatpoint 0: single precision)
+    assert v == [406, 576, 2304, 48, 6000, 9, 14, 3, 9, 100000, 999, 135, 16, 4, 1, 1000, 1, 0]

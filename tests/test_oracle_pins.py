@@ -4,6 +4,8 @@ compiles (oracle/_ref)."""
 import os
 import subprocess
 
+import math
+
 import numpy as np
 import pytest
 from scipy.special import expn
@@ -61,6 +63,44 @@ def test_expm1_exp_accuracy():
     assert worst_m < 4e-7 and worst_e < 4e-7
     assert O.expm1_exp(0.0) == (0.0, 1.0)
     assert O.expm1_exp(-100.0) == (-1.0, 0.0)
+
+
+def test_fp64_sincos2pi_accuracy_and_quadrants():
+    """The full-FP64 path's sin/cos of 2 pi u, against long-double libm (the
+    quadrant split is exact, so the error stays within 2 ulp of 1)."""
+    rng = np.random.default_rng(3)
+    us = np.concatenate([np.arange(0, 4097) / 4096.0, rng.integers(0, 2 ** 24, 4000) / 2.0 ** 24]).astype(np.float32)
+    two_pi = np.longdouble(2) * np.arccos(np.longdouble(-1))
+    err = 0.0
+    for u in us:
+        s, c = O.sincos2pi_d(float(u))
+        ref = two_pi * np.longdouble(u)
+        err = max(err, float(abs(np.longdouble(s) - np.sin(ref))), float(abs(np.longdouble(c) - np.cos(ref))))
+    assert err < 2.5e-16
+    assert O.sincos2pi_d(0.0) == (0.0, 1.0) and O.sincos2pi_d(1.0) == (0.0, 1.0)
+    assert O.sincos2pi_d(0.25)[0] == 1.0 and O.sincos2pi_d(0.25)[1] == 0.0
+    assert O.sincos2pi_d(0.5) == (0.0, -1.0) and O.sincos2pi_d(0.75)[0] == -1.0
+
+
+def test_fp64_expm1_exp_accuracy():
+    """The full-FP64 path's expm1/exp on x <= 0, against libm: within one
+    ulp over [-745, 0] and down to 1e-20, exact special values, subnormal
+    results at the bottom of the range."""
+    rng = np.random.default_rng(4)
+    xs = np.concatenate([-rng.uniform(0.0, 745.0, 20000), -10.0 ** rng.uniform(-20.0, 0.0, 20000),
+                         [-0.34657359027997264, -0.3465735902799727, -1.0, -708.3, -709.0]])
+    wm, we = 0.0, 0.0
+    for x in xs:
+        em1, e = O.expm1_exp_d(float(x))
+        rm, re = math.expm1(x), math.exp(x)
+        wm = max(wm, abs(em1 - rm) / abs(rm) / 2.0 ** -52)
+        if re > 2.3e-308:
+            we = max(we, abs(e - re) / re / 2.0 ** -52)
+    assert wm <= 1.0 and we <= 1.0
+    assert O.expm1_exp_d(0.0) == (0.0, 1.0) and math.copysign(1.0, O.expm1_exp_d(-0.0)[0]) == -1.0
+    assert O.expm1_exp_d(-746.0) == (-1.0, 0.0)
+    em1, e = O.expm1_exp_d(-740.0)
+    assert em1 == -1.0 and abs(e - math.exp(-740.0)) <= 2 * 5e-324
 
 
 def test_planck_closed_form_and_integral():
