@@ -187,6 +187,31 @@ int rc_ckdist(int nspec, const int64_t *off, const double *wvn, const double *kv
 int rc_ck_tables(int nB, int nT, int nQ, const double *T, const double *wvl, const double *wvr,
                  const double *kq, const double *kavg, const double *wq, double chanHeight,
                  rc_tables *t);
+/* The generator's front end on a line-by-line spectrum in file order (wv
+ * in cm^-1 increasing, kv in cm^-1), in place of absWideBand
+ * (nbcorrk/read_files.cpp:22-73) and the narrow-band split of its main
+ * (nbcorrk/ckdist_hitran_part.cpp:55-114): wide bands where kv > toll, with
+ * weak stretches of up to wvtoll (counted dwv per weak point) kept inside,
+ * each split into (int)(width / deltawv[min(i/3, 2)]) (>= 1) narrow bands by
+ * point count.  Narrow band b is points first[b] .. first[b]+count[b]-1 of
+ * the input, with edges wv_left/wv_right[b] and centre wvc[b] (those three
+ * may be NULL); at most max_bands are written.  Returns the number of
+ * narrow bands (< 0 on error); nwide (may be NULL) gets the wide ones.
+ * The reference's constants: toll 1e-6, wvtoll 25000, dwv 0.01,
+ * deltawv {10, 10, 10} (nbcorrk/param.h:18-23). */
+int rc_lbl_bands(int64_t n, const double *wv, const double *kv, double toll, double wvtoll, double dwv,
+                 const double *deltawv, int max_bands, int64_t *first, int64_t *count, double *wv_left,
+                 double *wv_right, double *wvc, int32_t *nwide);
+/* readFillT (nbcorrk/read_files.cpp:76-114) for one temperature T [K]: the
+ * spectrum (wv, kv, n points in file order) fills kvn[ntot], the narrow
+ * bands' absorption coefficients in m^-1 (kv times 100), at the points whose
+ * wavenumber equals the next one awaited in wvn[ntot] (the bands' points in
+ * order, e.g. the reference spectrum's wv over the bands' ranges); *kPr gets
+ * the line-by-line Planck mean in m^-1 accumulated over the points read
+ * until the last band is full.  Returns the points filled (ntot when every
+ * awaited wavenumber was found; < 0 on error); kvn past them is untouched. */
+int64_t rc_lbl_fill(int64_t n, const double *wv, const double *kv, double T, double dwv, int64_t ntot,
+                    const double *wvn, double *kvn, double *kPr);
 /* interp3D (src/montecarlo/spline.cpp:36-177; not called by the
  * reference, mc_gpu_noint.cu:694): the field T ((imax+2)(jmax+2)(kmax+2)
  * host floats) restricted to grid level `level` (1..grid_num-1) by natural

@@ -514,6 +514,41 @@ int rc_ck_tables(int nB, int nT, int nQ, const double *T, const double *wvl, con
     return 0;
 }
 
+int rc_lbl_bands(int64_t n, const double *wv, const double *kv, double toll, double wvtoll, double dwv,
+                 const double *deltawv, int max_bands, int64_t *first, int64_t *count, double *wv_left,
+                 double *wv_right, double *wvc, int32_t *nwide) {
+    if (n < 1 || !wv || !kv || !deltawv || max_bands < 0 || (max_bands > 0 && (!first || !count))) {
+        set_error("rc_lbl_bands: empty spectrum, null array or negative max_bands");
+        return -1;
+    }
+    for (int d = 0; d < 3; ++d)
+        if (!(deltawv[d] > 0.0)) { set_error("rc_lbl_bands: deltawv must be positive"); return -1; }
+    std::vector<int64_t> f, c;
+    std::vector<double> l, r, m;
+    int nw = 0;
+    const int nb = lbl_bands(n, wv, kv, toll, wvtoll, dwv, deltawv, f, c, l, r, m, nw);
+    if (nb < 0) return nb;
+    if (nwide) *nwide = nw;
+    const int k = std::min(nb, max_bands);
+    for (int b = 0; b < k; ++b) {
+        first[b] = f[b];
+        count[b] = c[b];
+        if (wv_left) wv_left[b] = l[b];
+        if (wv_right) wv_right[b] = r[b];
+        if (wvc) wvc[b] = m[b];
+    }
+    return nb;
+}
+
+int64_t rc_lbl_fill(int64_t n, const double *wv, const double *kv, double T, double dwv, int64_t ntot,
+                    const double *wvn, double *kvn, double *kPr) {
+    if (n < 1 || ntot < 1 || !wv || !kv || !wvn || !kvn || !kPr || !(T > 0.0)) {
+        set_error("rc_lbl_fill: empty spectrum or bands, null array or T <= 0");
+        return -1;
+    }
+    return lbl_fill(n, wv, kv, T, dwv, ntot, wvn, kvn, *kPr);
+}
+
 int rc_grey_tables(const rc_params *p, float Tmax, rc_tables *t) {
     Tables tt;
     tt.resize(t->nB, t->nT, t->nQ);

@@ -43,6 +43,13 @@ float field_tmax(const rc_params &p, const float *T);
 int read_tables(const std::string &dir, double chanHeight, Tables &t);
 int grey_tables(const rc_params &p, float Tmax, Tables &t);
 int write_tables(const std::string &dir, const Tables &t, double chanHeight);
+/* the generator's front end: wide and narrow bands of a line-by-line
+ * spectrum, and the fill of the bands at one temperature */
+int lbl_bands(int64_t n, const double *wv, const double *kv, double toll, double wvtoll, double dwv,
+              const double *deltawv, std::vector<int64_t> &first, std::vector<int64_t> &count,
+              std::vector<double> &wvl, std::vector<double> &wvr, std::vector<double> &wvc, int &nwide);
+int64_t lbl_fill(int64_t n, const double *wv, const double *kv, double T, double dwv, int64_t ntot,
+                 const double *wvn, double *kvn, double &kPr);
 int ck_tables(int nB, int nT, int nQ, const double *T, const double *wvl, const double *wvr, const double *kq,
               const double *kavg, const double *wq, double chanHeight, Tables &t);
 void sort_idx(const float *kavg, int nB, int32_t *idx);

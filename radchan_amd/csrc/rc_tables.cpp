@@ -320,6 +320,97 @@ int write_tables(const std::string &dir, const Tables &t, double chanHeight) {
     return 0;
 }
 
+/* The generator's front end on a line-by-line spectrum read in file order
+ * (wavenumber wv in cm^-1, absorption coefficient kv in cm^-1).
+ *
+ * Wide bands (absWideBand, nbcorrk/read_files.cpp:22-73): a point with
+ * kv > toll opens a new wide band when the previous point had kv <= toll
+ * and at least wvtoll of "weak" wavenumber (dwv per point with kv <= toll)
+ * has passed since the last strong point; otherwise it joins the current
+ * band; a weak point joins the current band while that weak wavenumber is
+ * below wvtoll.  Every wide band is thus a contiguous run of points.
+ * Narrow bands (ckdist_hitran_part.cpp:55-114): wide band i is split into
+ * numbands = (int)((wv_last - wv_first) / deltawv[min(i / 3, 2)]) (at least
+ * one) bands by point count: with comp = points / numbands, point j goes to
+ * band h, and h moves on after every point j > 0 with j % comp == 0 until
+ * the last band, which takes the rest (so the first band holds comp + 1
+ * points).  A band's edges are its first and last wavenumber, wvc their
+ * mean.  Outputs per narrow band: its first point and point count in the
+ * input, wv_left, wv_right, wvc; returns the number of narrow bands (or
+ * < 0 on error), nwide the number of wide bands. */
+int lbl_bands(int64_t n, const double *wv, const double *kv, double toll, double wvtoll, double dwv,
+              const double *deltawv, std::vector<int64_t> &first, std::vector<int64_t> &count,
+              std::vector<double> &wvl, std::vector<double> &wvr, std::vector<double> &wvc, int &nwide) {
+    std::vector<int64_t> ws, we;                 /* wide bands [ws, we) */
+    double kvprev = 0.0, wvcu = wvtoll + 1.0;
+    for (int64_t i = 0; i < n; ++i) {
+        if (kv[i] > toll) {
+            if (kvprev <= toll && wvcu >= wvtoll) { ws.push_back(i); we.push_back(i + 1); }
+            else if (!ws.empty()) we.back() = i + 1;
+            wvcu = 0.0;
+        } else if (wvcu < wvtoll) {
+            if (!ws.empty()) we.back() = i + 1;
+            wvcu = wvcu + dwv;
+        }
+        kvprev = kv[i];
+    }
+    nwide = (int)ws.size();
+    first.clear(); count.clear(); wvl.clear(); wvr.clear(); wvc.clear();
+    for (size_t b = 0; b < ws.size(); ++b) {
+        const int64_t a = ws[b], m = we[b] - ws[b];
+        const double range = wv[we[b] - 1] - wv[a];
+        int numbands = (int)(range / deltawv[std::min<size_t>(b / 3, 2)]);
+        if (numbands == 0) numbands = 1;
+        const int64_t comp = m / numbands;
+        if (comp < 1) {
+            set_error("wide band " + std::to_string(b) + " holds fewer points than narrow bands");
+            return -3;
+        }
+        /* band h holds points [lo_h, hi_h): band 0 ends after point comp,
+         * band h < numbands - 1 after point (h + 1) comp, the last at m */
+        for (int64_t h = 0; h < numbands; ++h) {
+            const int64_t lo = (h == 0) ? 0 : h * comp + 1;
+            const int64_t hi = (h == numbands - 1) ? m : (h + 1) * comp + 1;
+            if (hi <= lo) { set_error("narrow band " + std::to_string(h) + " of wide band " + std::to_string(b) +
+                                      " is empty"); return -3; }
+            first.push_back(a + lo);
+            count.push_back(hi - lo);
+            const double l = wv[a + lo], r = wv[a + hi - 1];
+            wvl.push_back(l);
+            wvr.push_back(r);
+            wvc.push_back((l + r) / 2);
+        }
+    }
+    return (int)first.size();
+}
+
+/* readFillT (nbcorrk/read_files.cpp:76-114) for one temperature T: the
+ * spectrum (wv, kv, n points in file order) fills the narrow bands'
+ * absorption coefficients kvn (times 100: m^-1) wherever its wavenumber
+ * equals the next wavenumber awaited (wvn, the bands' points in order),
+ * and accumulates the line-by-line Planck mean
+ * kPr = sum pi kv 100 I_b(T, wv) dwv 100 / (sigma T^4) over the points read
+ * until the last band is full (the reading stops there).  Returns the
+ * number of band points filled. */
+int64_t lbl_fill(int64_t n, const double *wv, const double *kv, double T, double dwv, int64_t ntot,
+                 const double *wvn, double *kvn, double &kPr) {
+    const double pi = kPi, stefan = kStefan;
+    const double C1 = kC1, C2 = kC2;
+    auto iblack = [&](double Tr, double nu) {     /* nbcorrk/ckdist_hitran_part.cpp:229-242 */
+        return 1.0 / pi * C1 * std::pow((nu * 100), 3) / (std::exp((float)(C2 * nu * 100 / Tr)) - 1);
+    };
+    kPr = 0.0;
+    int64_t q = 0;
+    for (int64_t i = 0; i < n && q < ntot; ++i) {
+        kPr = kPr + pi * kv[i] * 100 * iblack(T, wv[i]) * dwv * 100 / (stefan * std::pow(T, 4));
+        if (wv[i] == wvn[q]) {
+            kvn[q] = kv[i] * 100;
+            ++q;
+        }
+    }
+    return q;
+}
+
 /* calcProbT (nbcorrk/calc_prob.cpp:17-106) from the distributions of every
  * (temperature row i, band j): kq[i][j][nQ] and kavg[i][j] in FP64.  Per
  * row, the correlated-k Planck mean kP and the band CDF come from the band

@@ -130,6 +130,11 @@ def load() -> C.CDLL:
         "rc_ckdist_device": (C.c_int, [C.c_int, C.c_void_p] + [C.c_void_p] * 4 + [C.c_int, C.c_double, C.c_int]
                              + [C.c_void_p] * 5 + [C.c_int, C.c_void_p]),
         "rc_ck_tables": (C.c_int, [C.c_int, C.c_int, C.c_int] + [P(C.c_double)] * 6 + [C.c_double, P(TablesC)]),
+        "rc_lbl_bands": (C.c_int, [C.c_int64, P(C.c_double), P(C.c_double), C.c_double, C.c_double, C.c_double,
+                                   P(C.c_double), C.c_int, P(C.c_int64), P(C.c_int64), P(C.c_double),
+                                   P(C.c_double), P(C.c_double), P(C.c_int32)]),
+        "rc_lbl_fill": (C.c_int64, [C.c_int64, P(C.c_double), P(C.c_double), C.c_double, C.c_double, C.c_int64,
+                                    P(C.c_double), P(C.c_double), P(C.c_double)]),
         "rc_ckdist": (C.c_int, [C.c_int, P(C.c_int64), P(C.c_double), P(C.c_double), P(C.c_double),
                                 P(C.c_double), C.c_int, C.c_double, C.c_int, P(C.c_double), P(C.c_double),
                                 P(C.c_double), P(C.c_double), P(C.c_double), P(C.c_double)]),

@@ -100,3 +100,91 @@ def generate_tables(spec: dict, gq, wq, chanHeight: float = 1.0, n_pwrk: int = 2
                                   g["kavg"].reshape(nT, nB), wq, chanHeight)
     return t, g
 
+
+
+# ---------------------------------------------------------------------------
+# the generator's front end on line-by-line spectrum files
+
+REF_TOLL, REF_WVTOLL, REF_DWV, REF_DELTAWV = 1e-6, 25000.0, 0.01, (10.0, 10.0, 10.0)   # nbcorrk/param.h:18-23
+
+
+def read_lbl(path: str):
+    """A line-by-line spectrum file as the generator reads it
+    (nbcorrk/read_files.cpp:29: "%lf %lf" pairs, wavenumber in cm^-1 and
+    absorption coefficient in cm^-1): returns (wv, kv) float64 arrays."""
+    a = np.loadtxt(path, dtype=np.float64, ndmin=2)
+    return np.ascontiguousarray(a[:, 0]), np.ascontiguousarray(a[:, 1])
+
+
+def lbl_bands(wv, kv, toll: float = REF_TOLL, wvtoll: float = REF_WVTOLL, dwv: float = REF_DWV,
+              deltawv=REF_DELTAWV) -> dict:
+    """Wide and narrow bands of the reference spectrum (rc_lbl_bands):
+    returns first, count (point ranges), wvl, wvr, wvc per narrow band and
+    nwide."""
+    import ctypes as C
+
+    from . import lib as L
+    wv = np.ascontiguousarray(wv, np.float64)
+    kv = np.ascontiguousarray(kv, np.float64)
+    dw = np.ascontiguousarray(deltawv, np.float64)
+    lib = L.load()
+    pd = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    pi = lambda a: a.ctypes.data_as(C.POINTER(C.c_int64))
+    nw = C.c_int32()
+    nb = lib.rc_lbl_bands(len(wv), pd(wv), pd(kv), toll, wvtoll, dwv, pd(dw), 0, None, None, None, None, None,
+                          C.byref(nw))
+    if nb < 0:
+        raise RuntimeError("rc_lbl_bands: " + L.last_error())
+    out = dict(first=np.zeros(nb, np.int64), count=np.zeros(nb, np.int64), wvl=np.zeros(nb), wvr=np.zeros(nb),
+               wvc=np.zeros(nb))
+    nb2 = lib.rc_lbl_bands(len(wv), pd(wv), pd(kv), toll, wvtoll, dwv, pd(dw), nb, pi(out["first"]),
+                           pi(out["count"]), pd(out["wvl"]), pd(out["wvr"]), pd(out["wvc"]), C.byref(nw))
+    assert nb2 == nb
+    out["nwide"] = nw.value
+    return out
+
+
+def lbl_fill(wv, kv, T: float, wvn, dwv: float = REF_DWV):
+    """One temperature's spectrum into the narrow bands' points wvn
+    (rc_lbl_fill): returns (kvn in m^-1, kPr in m^-1, points filled)."""
+    import ctypes as C
+
+    from . import lib as L
+    wv = np.ascontiguousarray(wv, np.float64)
+    kv = np.ascontiguousarray(kv, np.float64)
+    wvn = np.ascontiguousarray(wvn, np.float64)
+    kvn = np.zeros(len(wvn))
+    kpr = C.c_double()
+    pd = lambda a: a.ctypes.data_as(C.POINTER(C.c_double))
+    q = L.load().rc_lbl_fill(len(wv), pd(wv), pd(kv), T, dwv, len(wvn), pd(wvn), pd(kvn), C.byref(kpr))
+    if q < 0:
+        raise RuntimeError("rc_lbl_fill: " + L.last_error())
+    return kvn, kpr.value, int(q)
+
+
+def spectra_from_lbl(ref, temps, spectra, dwv: float = REF_DWV, **band_args) -> dict:
+    """The generator's front end end to end: the narrow bands of the
+    reference spectrum ref = (wv, kv) (ckdist_hitran_part.cpp:31-114), then
+    every temperature's spectrum (temps[i], spectra[i] = (wv, kv)) filled
+    into them (allocT + readFillT, alloc_bands.cpp:17-51,
+    read_files.cpp:76-114).  Returns the layout rc_ckdist and
+    generate_tables take (spectrum i nB + j), plus kPr per temperature."""
+    wv0, kv0 = ref
+    b = lbl_bands(wv0, kv0, dwv=dwv, **band_args)
+    nB = len(b["first"])
+    idx = np.concatenate([np.arange(f, f + c) for f, c in zip(b["first"], b["count"])])
+    wvn1 = np.asarray(wv0, np.float64)[idx]
+    temps = np.asarray(temps, np.float64)
+    kvn, kpr = [], []
+    for T, (wv, kv) in zip(temps, spectra):
+        k, pr, q = lbl_fill(wv, kv, T, wvn1, dwv)
+        if q != len(wvn1):
+            raise RuntimeError(f"spectrum at {T} K lacks wavenumber {wvn1[q]} of the bands (filled {q} of {len(wvn1)})")
+        kvn.append(k)
+        kpr.append(pr)
+    per_off = np.concatenate([[0], np.cumsum(b["count"])]).astype(np.int64)
+    nT = len(temps)
+    off = np.concatenate([per_off[:-1] + t * per_off[-1] for t in range(nT)] + [[nT * per_off[-1]]]).astype(np.int64)
+    return dict(off=off, wvn=np.tile(wvn1, nT), kvn=np.concatenate(kvn), wvl=np.tile(b["wvl"], nT),
+                wvr=np.tile(b["wvr"], nT), T=np.repeat(temps, nB), nB=nB, temps=temps, kPr=np.array(kpr),
+                bands=b)
