@@ -493,7 +493,7 @@ def bench_c4(args) -> int:
     from radchan_amd import Group
     group = Group([s for _, _, _, s in shapes])
     # the schedule the autotuner picks for a single 256x128x256 field
-    group.set_schedule(32, 8, 4)
+    group.set_schedule(32, 4, 4)
 
     def step(variant):
         if variant == VARIANT_BATCHED:
@@ -516,6 +516,20 @@ def bench_c4(args) -> int:
     rays = sum(st["rays"] for st in sts)
     eps_l = [std_error(shapes[0][3].tallies(i))[0] for i in range(shapes[0][2])]
     log(f"c4 batched: {1e3 * wall:.1f} ms/step, {rays / wall / 1e9:.3f} Grays/s, eps (large) {max(eps_l):.4g}")
+    # the mixed-precision form: BF16 k tables in both sessions, one group launch
+    group.close()
+    for _, _, _, s in shapes:
+        s.set_table_precision(True)
+    group = Group([s for _, _, _, s in shapes])
+    group.set_schedule(32, 4, 4)
+    bwall, _ = timed(VARIANT_BATCHED, args.steps, args.warmup)
+    beps = [std_error(shapes[0][3].tallies(i))[0] for i in range(shapes[0][2])]
+    bf16 = {"ms_per_step": 1e3 * bwall, "rays_per_s": rays / bwall, "eps_large_max": max(beps),
+            "time_to_1pct_s": bwall * (max(beps) / 0.01) ** 2}
+    log(f"c4 batched, BF16 tables: {1e3 * bwall:.1f} ms/step")
+    group.close()
+    for _, _, _, s in shapes:
+        s.set_table_precision(False)
     plain = None
     if not args.no_plain:
         pwall, psts = timed(VARIANT_PLAIN, 1, 0)
@@ -532,8 +546,9 @@ def bench_c4(args) -> int:
         "data": "synthetic (H2O-like tables, laminar 300-1800 K channel T fields with 5% smoothed noise)",
         "config": {"workload": "snapshot ensemble: 8 x 64x96x64 + 4 x 256x128x256, photmax 100, nVar %d" % args.nvar,
                    "rays_per_step": int(rays), "batched_launches_per_step": 1,
-                   "schedule": {"wg_per_cu": 32, "threshold": 8, "steps_per_iter": 4}},
+                   "schedule": {"wg_per_cu": 32, "threshold": 4, "steps_per_iter": 4}},
         "eps_large": eps_l, "time_to_1pct_s": t1,
+        "batched_bf16_tables": bf16,
         "plain": plain,
         "time_to_1pct_speedup_vs_plain": (plain["time_to_1pct_s"] / t1) if plain else None,
     }

@@ -618,9 +618,15 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                         b = dirb[h];
                     } else if (RADIX) {
                         key = (uint32_t)kr[h];
-                        const U4 q = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)U.v, U.cid, U.tag + 16u * (uint32_t)sn.snap);
-                        a = q.a;
-                        b = q.b;
+                        /* the direction uniforms drawn again, unless the
+                         * Sobol sampler replaces them below */
+                        a = b = 0.f;
+                        if (SURF || !cm.sampler) {
+                            const U4 q = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)U.v, U.cid,
+                                               U.tag + 16u * (uint32_t)sn.snap);
+                            a = q.a;
+                            b = q.b;
+                        }
                     } else {
                         const U4 q = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)U.v, U.cid, U.tag + 16u * (uint32_t)sn.snap);
                         key = band_key(cm, sn, SURF, q.c, q.d);

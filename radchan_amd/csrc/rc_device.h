@@ -114,12 +114,19 @@ __device__ __forceinline__ uint4 draw4_bits(uint32_t k0, uint32_t k1, uint32_t h
  * v_1 = 2^31, v_k = v_(k-1) xor (v_(k-1) >> 1). */
 __device__ __forceinline__ void sobol2(uint32_t h, uint32_t &x0, uint32_t &x1) {
     x0 = __builtin_bitreverse32(h);
-    uint32_t y = 0u, v = 0x80000000u;
-    for (; h; h >>= 1) {
-        y ^= (h & 1u) ? v : 0u;
-        v ^= v >> 1;
-    }
-    x1 = y;
+    /* x1 = xor of v_k over the set bits k of h.  v_k (v_0 = 2^31, v_k =
+     * v_(k-1) ^ v_(k-1) >> 1) holds bit 31 - j exactly when the binomial
+     * C(k, j) is odd, i.e. when j's bits are a subset of k's (Lucas), so
+     * bit 31 - j of x1 is the xor of h's bits at the supersets k of j: a
+     * superset-sum over the five index bits, five shift-mask-xor steps,
+     * then reversed -- branch-free, the same value as the recurrence. */
+    uint32_t f = h;
+    f ^= (f >> 1) & 0x55555555u;
+    f ^= (f >> 2) & 0x33333333u;
+    f ^= (f >> 4) & 0x0F0F0F0Fu;
+    f ^= (f >> 8) & 0x00FF00FFu;
+    f ^= (f >> 16) & 0x0000FFFFu;
+    x1 = __builtin_bitreverse32(f);
 }
 
 /* the direction uniforms of ray h under the Sobol sampler: the Sobol

@@ -902,7 +902,10 @@ __device__ __forceinline__ uint32_t fluid_setup(const DevCommon &cm, const DevSn
                                                 uint32_t cid, int v, int h, uint32_t key, bool have_key, float Ti,
                                                 Ray &r, En &e) {
     const uint32_t tag = TAG_FLUID + 16u * (uint32_t)sn.snap;
-    U4 u = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)v, cid, tag);
+    /* the ray's four uniforms: band and g (unless the sorted key is given)
+     * and the direction (unless the Sobol sampler gives it) */
+    U4 u{0.f, 0.f, 0.f, 0.f};
+    if (!have_key || !cm.sampler) u = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)v, cid, tag);
     if (!have_key) key = band_key(cm, sn, false, u.c, u.d);
     if (cm.sampler) {
         /* the reference's random == 1 (emiss_angSobol, mc_gpu_noint.cu:1085-1092) */

@@ -253,7 +253,8 @@ __device__ __forceinline__ void energy64(const DevCommon &cm, const DevSnap &sn,
 __device__ inline double fluid_ray64(const DevCommon &cm, const DevSnap &sn, int i, int j, int k, uint32_t cid, int v,
                                      int h, uint32_t key, bool have_key, float Ti, int &steps) {
     const uint32_t tag = TAG_FLUID + 16u * (uint32_t)sn.snap;
-    U4 u = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)v, cid, tag);
+    U4 u{0.f, 0.f, 0.f, 0.f};
+    if (!have_key || !cm.sampler) u = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)v, cid, tag);
     if (!have_key) key = band_key(cm, sn, false, u.c, u.d);
     if (cm.sampler) {
         const uint4 sh = draw4_bits(cm.k0, cm.k1, SOBOL_SHIFT_H, (uint32_t)v, cid, tag);
