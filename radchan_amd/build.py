@@ -105,6 +105,18 @@ def build_sanitized(outdir: str) -> tuple:
     return lib, olib
 
 
+def build_dropin_driver() -> str:
+    """The C caller of mc_gpu_ / get_results_ used by the drop-in GPU test
+    (tests/dropin/dropin_driver.c), linked against the built library with
+    a relative run path so that it loads the in-tree libradchan.so."""
+    root = os.path.dirname(HERE)
+    src = os.path.join(root, "tests", "dropin", "dropin_driver.c")
+    exe = os.path.join(root, "tests", "dropin", "dropin_driver")
+    _run(["gcc", "-O2", "-std=c11", "-Wall", "-I", os.path.join(root, "include"), src, "-L", HERE, "-lradchan",
+          "-Wl,-rpath,$ORIGIN/../../radchan_amd", "-o", exe])
+    return exe
+
+
 def build_oracle(with_reference: bool = True) -> None:
     """Compile the CPU checker and, if the RadChan tree is present, its
     reference drivers (outputs only under oracle/_ref)."""

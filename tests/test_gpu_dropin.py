@@ -89,3 +89,43 @@ def test_mc_gpu_with_exact_estimator_and_spline_restriction(tmp_path, nb_tables)
     s.close()
     r3 = res.reshape(p.kmax + 2, p.jmax + 2, p.imax + 2)
     assert np.array_equal(r3[1:-1, 1:-1, 1:-1], mean)
+
+
+DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dropin", "dropin_driver")
+
+
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="C driver not built (python -c 'import __graft_entry__ as g; g.build()')")
+def test_c_driver_as_the_dns_calls(tmp_path):
+    """A C program (tests/dropin/dropin_driver.c) calls get_results_ before
+    any launch (outputs untouched, error reported), then mc_gpu_ /
+    get_results_ for two radiative steps, by reference, on one slab of
+    two: its harvest equals the session API's solve of the same slab."""
+    import subprocess
+    tab = synthetic_nb()
+    tab.write(str(tmp_path), 0.5)
+    p = P.base_params(16, 12, 16, photmax=32, nVar=3, fact_mesh=1.5, seed=9)
+    p.p_col = 2
+    p.calcwalls = 1
+    p.chanHeight = 0.5
+    T = F.channel_field(p, 300.0, 1500.0, noise=0.03, seed=6)
+    T.astype(np.float32).tofile(tmp_path / "field.bin")
+    kloc = p.kmax // p.p_col
+    print("c driver: 16x12x16 slab 2 of 2", flush=True)
+    r = subprocess.run([DRIVER, str(tmp_path), str(tmp_path / "field.bin"), str(tmp_path / "out.bin"),
+                        str(p.imax), str(p.jmax), str(p.kmax), str(p.p_col), str(kloc + 1), str(p.photmax),
+                        str(p.nVar), str(p.calcwalls), str(p.seed), "2"], capture_output=True, text=True, timeout=120)
+    assert r.returncode == 0, r.stderr
+    n = (p.imax + 2) * (p.jmax + 2) * (kloc + 2)
+    out = np.fromfile(tmp_path / "out.bin", np.float32)
+    res, var = out[:n].reshape(kloc + 2, p.jmax + 2, p.imax + 2), out[n:].reshape(kloc + 2, p.jmax + 2, p.imax + 2)
+    t2 = R.Tables.read(str(tmp_path), tab.nB, tab.nT, tab.nQ, 0.5)
+    s = R.Solver(p, t2)
+    s.set_fields([T], zstart=kloc + 1)
+    s.solve(R.VARIANT_BATCHED)
+    mean, sd = s.results()
+    wm, _, _ = s.surface()
+    assert np.array_equal(res[1:kloc + 1, 1:p.jmax + 1, 1:p.imax + 1], mean)
+    assert np.array_equal(var[1:kloc + 1, 1:p.jmax + 1, 1:p.imax + 1], sd)
+    assert np.array_equal(res[1:kloc + 1, 1:p.jmax + 1, 0], wm[0])
+    assert np.array_equal(res[1:kloc + 1, 1:p.jmax + 1, p.imax + 1], wm[1])
+    s.close()
