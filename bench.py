@@ -199,8 +199,8 @@ def eps_scaling(p, tab, T, sched, eps, bf16: bool):
     """eps at 4x photmax on the same field (one solve; 2x where 4x would pass
     the 16384 rays per cell that the band sort holds): the projection to
     eps = 1% assumes eps ~ rays^(-1/2), so the ratio should be ~mult^(-1/2).
-    (The time is not 4x: above 256 rays per cell the band keys are sorted in
-    LDS rather than in registers.)"""
+    (The time is not exactly 4x: above 256 rays per cell the band keys are
+    sorted by radix passes through HBM rather than in registers.)"""
     from radchan_amd import VARIANT_BATCHED
     import copy
     mult = next((m for m in (4, 2) if p.srt != 1 or m * p.photmax <= 16384), None)

@@ -126,7 +126,7 @@ int check_params(const rc_params &p) {
     if ((int64_t)p.jmax * p.kmax >= (1 << 24) || (int64_t)p.imax * p.jmax * p.kmax >= ((int64_t)1 << 31))
         return fail("grid too large: jmax kmax must stay below 2^24 and imax jmax kmax below 2^31 (cell indices)");
     if (p.srt == 1 && p.photmax > 16384)
-        return fail("srt: photmax above 16384 (the band keys of a cell's rays are sorted in one workgroup's LDS)");
+        return fail("srt: photmax above 16384 (the plain variant sorts a cell's band keys in one workgroup's LDS)");
     if (!(p.Lx > 0) || !(p.Ly > 0) || !(p.Lz > 0)) return fail("domain lengths must be positive");
     for (int f = 0; f < 4; ++f)
         if (p.bd[f] < 1 || p.bd[f] > 3) return fail("x and y faces take bd 1 (periodic), 2 (black) or 3 (specular)");
