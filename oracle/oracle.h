@@ -14,7 +14,9 @@
  * GPU parity tests): FP32 arithmetic with no contraction (-ffp-contract=off),
  * explicit fmaf where a fused op is wanted, correctly rounded division and
  * sqrt, and project-written exp/expm1/sincos(2*pi*u) built only from those
- * operations, so that CPU and GPU produce identical bits.
+ * operations, so that CPU and GPU produce identical bits.  The full-FP64
+ * mode (precision 1, the library's floatpoint 1) does the same in double,
+ * with its own project-written expm1/exp and sin/cos of 2 pi u.
  */
 #ifndef RADCHAN_ORACLE_H
 #define RADCHAN_ORACLE_H
@@ -53,7 +55,8 @@ typedef struct orc_params {
     float   abscoeff, Tmins, wvmax;  /* NarrowBand.h:38-40 (grey block)        */
     uint64_t seed;                   /* Philox key                             */
     int32_t sampler;                 /* 1: Sobol fluid directions, 0: Philox   */
-    int32_t pad_;
+    int32_t fp64;                    /* the library's floatpoint switch; the   */
+                                     /* oracle takes the precision per call    */
 } orc_params;
 
 /* Spectral tables as read by readT (read_tables.cpp:21-95).  Arrays are in

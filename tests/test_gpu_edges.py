@@ -1,0 +1,60 @@
+"""Edge cases of the solver against the CPU oracle on the GPU: the smallest
+grid and ray count (one run: variance 0), odd grid sizes with wall fluxes,
+and the largest photmax the band sort takes (16384 keys: the radix sort
+and the plain variant's LDS sort at their limits)."""
+import numpy as np
+import pytest
+
+from radchan_amd import VARIANT_BATCHED, VARIANT_PLAIN, Solver
+from radchan_amd import fields as F
+from radchan_amd import params as P
+from oracle import oracle as O
+
+pytestmark = pytest.mark.gpu
+
+
+def _both(p, tab, T, surface=False):
+    o = O.Oracle(p, tab, T)
+    cells = O.all_wall_cells(p) if surface else O.all_cells(p)
+    de32, de32d, _, _ = o.solve(cells, surface=surface)
+    assert np.abs(de32d).max() > 0
+    for variant in (VARIANT_PLAIN, VARIANT_BATCHED):
+        print(f"edge: {p.imax}x{p.jmax}x{p.kmax} photmax {p.photmax} nVar {p.nVar} variant {variant} "
+              f"surface {surface}", flush=True)
+        s = Solver(p, tab)
+        s.set_fields([T])
+        s.solve(variant)
+        got = s.surface()[2].reshape(-1, p.nVar) if surface else s.tallies()
+        if variant == VARIANT_PLAIN:
+            assert np.array_equal(got, de32.astype(np.float64))
+        else:
+            assert np.abs(got - de32d).max() <= 1e-12 * max(np.abs(de32d).max(), 1e-300)
+        if not surface:
+            mean, sd = s.results()
+            assert np.all(np.isfinite(mean)) and np.all(np.isfinite(sd))
+            if p.nVar == 1:
+                assert np.all(sd == 0.0)
+        s.close()
+    o.close()
+
+
+def test_smallest_grid_one_ray_one_run(nb_tables):
+    """2x1x1 cells, one ray per cell, one run (seed 38: the ray of cell 1
+    leaves its cell, so the tally is not trivially 0)."""
+    p = P.base_params(2, 1, 1, photmax=1, nVar=1, fact_mesh=1.5, seed=38)
+    T = F.channel_field(p, 400.0, 1200.0, noise=0.05, seed=1)
+    _both(p, nb_tables, T)
+
+
+def test_odd_sizes_with_walls(nb_tables):
+    p = P.base_params(6, 3, 5, photmax=17, nVar=3, fact_mesh=1.2, seed=32)
+    p.calcwalls = 1
+    T = F.channel_field(p, 350.0, 1300.0, noise=0.04, seed=2)
+    _both(p, nb_tables, T)
+    _both(p, nb_tables, T, surface=True)
+
+
+def test_largest_sorted_photmax(nb_tables):
+    p = P.base_params(2, 2, 2, photmax=16384, nVar=1, fact_mesh=1.5, seed=33)
+    T = F.channel_field(p, 400.0, 1400.0, noise=0.03, seed=3)
+    _both(p, nb_tables, T)
