@@ -39,10 +39,11 @@ def _both(p, tab, T, surface=False):
 
 
 def test_smallest_grid_one_ray_one_run(nb_tables):
-    """2x1x1 cells, one ray per cell, one run (seed 38: the ray of cell 1
-    leaves its cell, so the tally is not trivially 0)."""
-    p = P.base_params(2, 1, 1, photmax=1, nVar=1, fact_mesh=1.5, seed=38)
-    T = F.channel_field(p, 400.0, 1200.0, noise=0.05, seed=1)
+    """2x1x1 cells, one ray per cell, one run (seed 31: a ray leaves its
+    cell, so the tally is not trivially 0).  Wall and centre temperatures
+    close together keep the extrapolated ghost planes positive."""
+    p = P.base_params(2, 1, 1, photmax=1, nVar=1, fact_mesh=1.5, seed=31)
+    T = F.channel_field(p, 700.0, 900.0, noise=0.05, seed=1)
     _both(p, nb_tables, T)
 
 
@@ -56,5 +57,5 @@ def test_odd_sizes_with_walls(nb_tables):
 
 def test_largest_sorted_photmax(nb_tables):
     p = P.base_params(2, 2, 2, photmax=16384, nVar=1, fact_mesh=1.5, seed=33)
-    T = F.channel_field(p, 400.0, 1400.0, noise=0.03, seed=3)
+    T = F.channel_field(p, 800.0, 1000.0, noise=0.03, seed=3)
     _both(p, nb_tables, T)
