@@ -266,6 +266,23 @@ def field_legs(args, lib, p, tab, T, bf16: bool, steps: int, warmup: int, plain_
                "ray_tally_bytes": int(sd["rays"]) * 4}
         log(f"batched, deterministic reduction: {1e3 * dwall:.2f} ms/step")
 
+    sobol = None
+    if p.sampler == 0 and not p.grey and not args.no_secondary:
+        # param.h's default direction sampler (random 1): Sobol points shifted per (cell, run)
+        q = p.copy()
+        q.sampler = 1
+        s1 = new_session(q, tab, [T])
+        s1.set_table_precision(bf16)
+        if sched:
+            s1.set_schedule(sched["wg_per_cu"], sched["threshold"], sched["steps_per_iter"])
+        s1.set_graph(not args.no_graph)
+        swall, ss, _ = timed_leg(s1, lib, VARIANT_BATCHED, 2, 1)
+        eps_s, _ = std_error(s1.tallies())
+        s1.close()
+        sobol = {"ms_per_step": 1e3 * swall, "rays_per_s": ss["rays"] / swall, "eps": eps_s,
+                 "time_to_1pct_s": swall * (eps_s / 0.01) ** 2, "eps_vs_philox": eps_s / head["eps"], "timed_solves": 2}
+        log(f"batched, Sobol directions: {1e3 * swall:.2f} ms/step, eps {eps_s:.4g} ({eps_s / head['eps']:.3f} of Philox)")
+
     plain = None
     if not args.no_plain and plain_steps > 0:
         s.set_table_precision(False)
@@ -321,7 +338,8 @@ def field_legs(args, lib, p, tab, T, bf16: bool, steps: int, warmup: int, plain_
                        f"{p.nT} T, photmax {p.photmax}, nVar {p.nVar}, srt {p.srt}, grid_num {p.grid_num}"
                        if not p.grey else f"grey OERM {p.imax}x{p.jmax}x{p.kmax}, photmax {p.photmax}, nVar {p.nVar}",
            "cells": ncell, "schedule": sched, "head": head, "batched": legs, "plain": plain,
-           "plain_one_launch_per_band": bands, "plain_fp64": fp64, "deterministic_reduction": det, "roofline": roof,
+           "plain_one_launch_per_band": bands, "plain_fp64": fp64, "deterministic_reduction": det,
+           "sobol_directions": sobol, "roofline": roof,
            "speedup_vs_plain": (head["rays_per_s"] / plain["rays_per_s"]) if plain else None,
            "time_to_1pct_speedup_vs_plain": (plain["time_to_1pct_s"] / head["time_to_1pct_s"]) if plain else None,
            "time_to_1pct_speedup_vs_plain_per_band": (bands["time_to_1pct_s"] / head["time_to_1pct_s"])
@@ -714,6 +732,7 @@ def main() -> int:
         "time_to_1pct_speedup_vs_plain": main_leg["time_to_1pct_speedup_vs_plain"],
         "plain_one_launch_per_band": main_leg["plain_one_launch_per_band"],
         "deterministic_reduction": main_leg["deterministic_reduction"],
+        "sobol_directions": main_leg["sobol_directions"],
         "time_to_1pct_speedup_vs_plain_per_band": main_leg["time_to_1pct_speedup_vs_plain_per_band"],
         "eps_scaling": main_leg.get("eps_scaling"),
         "time_to_1pct_measured": main_leg.get("time_to_1pct_measured"),
