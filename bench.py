@@ -211,7 +211,7 @@ def eps_scaling(p, tab, T, sched, eps, bf16: bool):
     s = new_session(q, tab, [T])
     s.set_table_precision(bf16)
     if sched:
-        s.set_schedule(sched["wg_per_cu"], sched["threshold"], sched["steps_per_iter"])
+        s.set_schedule(sched["wg_per_cu"], sched["threshold"], sched["steps_per_iter"], sched.get("refill", -1))
     s.set_graph(False)
     s.solve(VARIANT_BATCHED)
     st = s.solve(VARIANT_BATCHED)
@@ -274,7 +274,7 @@ def field_legs(args, lib, p, tab, T, bf16: bool, steps: int, warmup: int, plain_
         s1 = new_session(q, tab, [T])
         s1.set_table_precision(bf16)
         if sched:
-            s1.set_schedule(sched["wg_per_cu"], sched["threshold"], sched["steps_per_iter"])
+            s1.set_schedule(sched["wg_per_cu"], sched["threshold"], sched["steps_per_iter"], sched.get("refill", -1))
         s1.set_graph(not args.no_graph)
         swall, ss, _ = timed_leg(s1, lib, VARIANT_BATCHED, 2, 1)
         eps_s, _ = std_error(s1.tallies())
@@ -367,7 +367,7 @@ def confirm_leg(lib, p, tab, T, sched, eps: float, ms: float):
     q.nVar = int(math.ceil(p.nVar * (eps / 0.01) ** 2 * 1.1))
     s = new_session(q, tab, [T])
     if sched:
-        s.set_schedule(sched["wg_per_cu"], sched["threshold"], sched["steps_per_iter"])
+        s.set_schedule(sched["wg_per_cu"], sched["threshold"], sched["steps_per_iter"], sched.get("refill", -1))
     s.set_graph(False)
     lib.rc_device_synchronize()
     t0 = time.perf_counter()
@@ -511,7 +511,7 @@ def bench_c4(args) -> int:
     from radchan_amd import Group
     group = Group([s for _, _, _, s in shapes])
     # the schedule the autotuner picks for a single 256x128x256 field
-    group.set_schedule(32, 4, 4)
+    group.set_schedule(32, 4, 8, 1)
 
     def step(variant):
         if variant == VARIANT_BATCHED:
@@ -539,7 +539,7 @@ def bench_c4(args) -> int:
     for _, _, _, s in shapes:
         s.set_table_precision(True)
     group = Group([s for _, _, _, s in shapes])
-    group.set_schedule(32, 4, 4)
+    group.set_schedule(32, 4, 8, 1)
     bwall, _ = timed(VARIANT_BATCHED, args.steps, args.warmup)
     beps = [std_error(shapes[0][3].tallies(i))[0] for i in range(shapes[0][2])]
     bf16 = {"ms_per_step": 1e3 * bwall, "rays_per_s": rays / bwall, "eps_large_max": max(beps),
@@ -564,7 +564,7 @@ def bench_c4(args) -> int:
         "data": "synthetic (H2O-like tables, laminar 300-1800 K channel T fields with 5% smoothed noise)",
         "config": {"workload": "snapshot ensemble: 8 x 64x96x64 + 4 x 256x128x256, photmax 100, nVar %d" % args.nvar,
                    "rays_per_step": int(rays), "batched_launches_per_step": 1,
-                   "schedule": {"wg_per_cu": 32, "threshold": 4, "steps_per_iter": 4}},
+                   "schedule": {"wg_per_cu": 32, "threshold": 4, "steps_per_iter": 8, "refill": 1}},
         "eps_large": eps_l, "time_to_1pct_s": t1,
         "batched_bf16_tables": bf16,
         "plain": plain,

@@ -277,12 +277,19 @@ int rc_trace(rc_ctx *c, int surface, const int32_t *rays, int n, rc_ray *out);
 int rc_info(rc_ctx *c, int s, float *f5, int32_t *i2);
 /* schedule of the batched solver: persistent workgroups (one wavefront
  * each) per compute unit, the number of idle lanes that triggers a
- * refill, and march steps per loop iteration (rays that end on an earlier
- * step wait for the next refill); 0 selects the default, (32, 8, 3) for a
- * single grid and (20, 8, 4) for several levels.
+ * refill, and march steps per loop iteration; 0 selects the default.
  * Chosen by the autotuner. */
 int rc_set_schedule(rc_ctx *c, int wg_per_cu, int threshold, int steps_per_iter);
 int rc_get_schedule(rc_ctx *c, int32_t *wg_per_cu, int32_t *threshold, int32_t *steps_per_iter);
+/* refills of the batched solver (no counterpart in the reference): 0 =
+ * once per loop iteration (rays that end on an earlier step wait for it),
+ * 1 = also between the iteration's march steps, -1 = the default (1 for a
+ * single grid and for photmax above 256, 0 for several levels).  Defaults
+ * of the schedule with it: (32, 4, 8) single grid; (20, 4, 4) several
+ * levels; (20, 8, 2) several levels above 256 rays per cell.  Chosen by
+ * the autotuner; rc_get_refill returns the mode set (-1: default). */
+int rc_set_refill(rc_ctx *c, int mode);
+int rc_get_refill(rc_ctx *c);
 /* storage of the k tables on the device: 0 = FP32 (default), 1 = BF16 k
  * pairs with a power-of-two scale per band (Ib stays FP32); rebuilds the
  * loaded tables */
@@ -320,7 +327,9 @@ int rc_set_estimator(rc_ctx *c, int mode);
 int rc_set_reduction(rc_ctx *c, int mode);
 /* time the batched solve of the loaded snapshots over schedules (steps per
  * iteration, then workgroups per CU, then refill threshold, each at the
- * best of the previous) and keep the fastest */
+ * best of the previous; then refills between the steps with steps per
+ * iteration 2..8 at threshold 4 and, if they win, threshold 8) and keep
+ * the fastest, refill mode included (rc_get_refill) */
 int rc_autotune(rc_ctx *c, int32_t *best_wg, int32_t *best_th, int32_t *best_spi, float *best_ms);
 /* groups (no counterpart in the reference): the loaded snapshots of
  * several sessions -- grids of different sizes -- solved by one batched
@@ -335,6 +344,7 @@ typedef struct rc_group rc_group;
 rc_group *rc_group_create(rc_ctx *const *sessions, int n);
 void      rc_group_destroy(rc_group *g);
 int       rc_group_set_schedule(rc_group *g, int wg_per_cu, int threshold, int steps_per_iter);
+int       rc_group_set_refill(rc_group *g, int mode);
 /* launch (asynchronous): one batched kernel over every snapshot (and one
  * over their walls) */
 int       rc_group_run(rc_group *g);

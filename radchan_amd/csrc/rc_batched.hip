@@ -645,30 +645,34 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                 qcount += ng;
                 cur_h += ng;
             }
-            /* (d) idle lanes take queued rays */
-            const unsigned long long idle = ~live;
-            const int nidle = __popcll(idle);
-            if (qcount > 0 && nidle >= min(threshold, qcount)) {
-                const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32), __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
-                const int npop = min(nidle, qcount);
-                if (!active && rank < npop) {
-                    q_get<SURF>(rq, qwrap(qhead + rank), sl, cm, snaps[snap], r, e, runit, rtid, rh);
-                    active = true;
-                    dacc = 0.f;
-                    lev = 0;
-                    counter = 0;
-                    kt = -1;
-                    wall = 0;
-                    if (MULTI) {
-                        ll = lane_level(llev[0]);
-                        cell_walk_setup(ll, r, cw);
-                    } else {
-                        cell_walk_setup(level_view(&snaps[snap].lev[0]), r, cw);
+            /* (d) idle lanes take queued rays, once `threshold` lanes
+             * are idle (or the queue holds fewer rays) */
+            auto take_rays = [&](unsigned long long idle) {
+                const int nidle = __popcll(idle);
+                if (qcount > 0 && nidle >= min(threshold, qcount)) {
+                    const int rank = __builtin_amdgcn_mbcnt_hi((uint32_t)(idle >> 32),
+                                                               __builtin_amdgcn_mbcnt_lo((uint32_t)idle, 0u));
+                    const int npop = min(nidle, qcount);
+                    if (!active && rank < npop) {
+                        q_get<SURF>(rq, qwrap(qhead + rank), sl, cm, snaps[snap], r, e, runit, rtid, rh);
+                        active = true;
+                        dacc = 0.f;
+                        lev = 0;
+                        counter = 0;
+                        kt = -1;
+                        wall = 0;
+                        if (MULTI) {
+                            ll = lane_level(llev[0]);
+                            cell_walk_setup(ll, r, cw);
+                        } else {
+                            cell_walk_setup(level_view(&snaps[snap].lev[0]), r, cw);
+                        }
                     }
+                    qhead = qwrap(qhead + npop);
+                    qcount -= npop;
                 }
-                qhead = qwrap(qhead + npop);
-                qcount -= npop;
-            }
+            };
+            take_rays(~live);
             const bool drained = qdone && nxt < 0 && cur_h >= P && qcount == 0;
             if (drained && (__ballot(active) == 0ull || xfit)) break;   /* to the drain (f) */
             if (__ballot(active) != 0ull) {
@@ -724,6 +728,9 @@ __device__ __forceinline__ void batched_body(const DevCommon &cm, const DevSnap 
                 };
 #pragma unroll 1
                 for (int q = 0; q < spi; ++q) {
+                    /* refill between the steps too: a lane whose ray ended
+                     * idles for less of the iteration */
+                    if (cm.refill && q > 0) take_rays(~__ballot(active));
                     if (MULTI) substep(ll, ll.xoff, ll.poff);
                     else substep(l0, 0u, 0u);
                 }
@@ -920,11 +927,23 @@ void launch_zero(hipStream_t st, double *p, size_t n) {
     hipLaunchKernelGGL(k_zero_f64, dim3(blocks), dim3(256), 0, st, p, n);
 }
 
-/* defaults: what the autotuner picks on 64^3 (single grid) and on the
- * three-level 192x128x192 case (multigrid: fewer, longer-stepping
- * wavefronts) */
+/* defaults, what the autotuner picks on the benchmark fields: a single
+ * grid (256x128x256, 64^3) 32 workgroups per CU, refill at 4 idle lanes,
+ * 8 steps per iteration with refills between them; several levels
+ * (192x128x192: short rays) 20, 4, 4 without refills between the steps,
+ * and with the radix-sorted keys of large photmax (param.h's settings)
+ * 20, 8, 2 with them.  A refill mode < 0 selects the default. */
 static int default_wg_per_cu(const DevCommon &cm, int wg_per_cu) {
     return wg_per_cu > 0 ? wg_per_cu : (cm.grid_num > 1 ? 20 : 32);
+}
+
+void batched_schedule(const DevCommon &cm, int &wg_per_cu, int &threshold, int &spi, int &refill) {
+    const bool mg = cm.grid_num > 1;
+    const bool radix = cm.srt && cm.photmax > 256;
+    if (refill < 0) refill = (!mg || radix) ? 1 : 0;
+    wg_per_cu = default_wg_per_cu(cm, wg_per_cu);
+    if (threshold <= 0) threshold = (mg && radix) ? 8 : 4;
+    if (spi <= 0) spi = !mg ? (refill ? 8 : 3) : (radix ? 2 : 4);
 }
 
 int batched_grid(const DevCommon &cm, long nunit_cells, int wg_per_cu) {
@@ -944,12 +963,11 @@ size_t batched_srt_keys(const DevCommon &cm, long nunit_cells, int wg_per_cu) {
     return (size_t)batched_grid(cm, nunit_cells, wg_per_cu) * 2 * (size_t)((cm.photmax + 63) & ~63);
 }
 
-void launch_batched(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int nsnap, int nunit_cells, int surface,
-                    int snaps_bf16, double *de, unsigned long long *stats, int *queue, int wg_per_cu, int threshold,
-                    int spi) {
-    const bool mg = cm.grid_num > 1;
-    if (threshold <= 0) threshold = 8;
-    if (spi <= 0) spi = mg ? 4 : 3;
+void launch_batched(hipStream_t st, const DevCommon &cm0, const DevSnap *snaps, int nsnap, int nunit_cells,
+                    int surface, int snaps_bf16, double *de, unsigned long long *stats, int *queue, int wg_per_cu,
+                    int threshold, int spi) {
+    DevCommon cm = cm0;
+    batched_schedule(cm, wg_per_cu, threshold, spi, cm.refill);
     int n2 = 1;
     while (n2 < cm.photmax) n2 <<= 1;
     const size_t lds = (size_t)batched_lds_words(n2, cm.srt, cm.grid_num > 1) * 4;

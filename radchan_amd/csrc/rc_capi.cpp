@@ -146,6 +146,7 @@ struct rc_ctx {
     rc_stats last{};
     bool launched = false;
     int wg_per_cu = 0, threshold = 0, spi = 0;   /* batched schedule; 0 = default */
+    int refill = -1;                        /* refills between march steps: 1 on, 0 off, -1 default */
     bool use_graph = false;                 /* replay the batched run as a hipGraph */
     int table_bf16 = 0;                     /* k tables stored as BF16 */
     int graph_launches = 0;
@@ -381,6 +382,7 @@ DevCommon make_common(const rc_ctx *c) {
     cm.k0 = (uint32_t)p.seed; cm.k1 = (uint32_t)(p.seed >> 32);
     cm.ray_de = nullptr;
     cm.sampler = p.sampler == 1 ? 1 : 0;
+    cm.refill = c->refill;
     return cm;
 }
 
@@ -1004,26 +1006,41 @@ int rc_autotune(rc_ctx *c, int32_t *best_wg, int32_t *best_th, int32_t *best_spi
     static const int spis[] = {1, 2, 3, 4};
     static const int wgs[] = {8, 12, 16, 20, 24, 32, 40};
     static const int ths[] = {4, 8, 16, 32};
-    const int keep_wg = c->wg_per_cu, keep_th = c->threshold, keep_spi = c->spi;
+    static const int rspis[] = {2, 4, 6, 8};
+    static const int rths[] = {4, 8};
+    const int keep_wg = c->wg_per_cu, keep_th = c->threshold, keep_spi = c->spi, keep_rf = c->refill;
     float best = 1e30f;
-    int bw = 32, bt = 8, bs = 1;
-    auto trial = [&](int wg, int th, int sp) -> int {
-        c->wg_per_cu = wg; c->threshold = th; c->spi = sp;
+    int bw = 32, bt = 8, bs = 1, br = 0;
+    auto trial = [&](int wg, int th, int sp, int rf) -> int {
+        c->wg_per_cu = wg; c->threshold = th; c->spi = sp; c->refill = rf;
         c->drop_graph();
         if (int e = rc_run(c, RC_VARIANT_BATCHED)) return e;
         rc_stats st;
         if (int e = rc_sync(c, &st)) return e;
-        if ((float)st.march_ms < best) { best = (float)st.march_ms; bw = wg; bt = th; bs = sp; }
+        if ((float)st.march_ms < best) { best = (float)st.march_ms; bw = wg; bt = th; bs = sp; br = rf; }
         return 0;
     };
+    /* refills once per iteration: steps per iteration, workgroups per CU,
+     * threshold; then refills between the steps too: steps per iteration
+     * at threshold 4 and the best workgroup count, then threshold 8 */
     int e = 0;
-    for (int sp : spis) if (!e) e = trial(bw, bt, sp);
+    for (int sp : spis) if (!e) e = trial(bw, bt, sp, 0);
     const int s1 = bs;
-    for (int wg : wgs) if (!e) e = trial(wg, bt, s1);
+    for (int wg : wgs) if (!e) e = trial(wg, bt, s1, 0);
     const int w1 = bw;
-    for (int th : ths) if (!e) e = trial(w1, th, s1);
-    if (e) { c->wg_per_cu = keep_wg; c->threshold = keep_th; c->spi = keep_spi; c->drop_graph(); return e; }
-    c->wg_per_cu = bw; c->threshold = bt; c->spi = bs;
+    for (int th : ths) if (!e) e = trial(w1, th, s1, 0);
+    const int w2 = bw;
+    for (int sp : rspis) if (!e) e = trial(w2, rths[0], sp, 1);
+    if (!e && br == 1) {
+        const int s3 = bs;
+        for (int th : rths) if (!e && th != rths[0]) e = trial(w2, th, s3, 1);
+    }
+    if (e) {
+        c->wg_per_cu = keep_wg; c->threshold = keep_th; c->spi = keep_spi; c->refill = keep_rf;
+        c->drop_graph();
+        return e;
+    }
+    c->wg_per_cu = bw; c->threshold = bt; c->spi = bs; c->refill = br;
     c->drop_graph();
     if (best_wg) *best_wg = bw;
     if (best_th) *best_th = bt;
@@ -1092,6 +1109,18 @@ int rc_set_schedule(rc_ctx *c, int wg_per_cu, int threshold, int steps_per_iter)
     c->spi = steps_per_iter;
     c->drop_graph();
     return 0;
+}
+
+int rc_set_refill(rc_ctx *c, int mode) {
+    if (!c || mode < -1 || mode > 1) { set_error("rc_set_refill: mode -1 (default), 0 or 1"); return -1; }
+    c->refill = mode;
+    c->drop_graph();
+    return 0;
+}
+
+int rc_get_refill(rc_ctx *c) {
+    if (!c) { set_error("null context"); return -1; }
+    return c->refill;
 }
 
 int rc_get_schedule(rc_ctx *c, int32_t *wg_per_cu, int32_t *threshold, int32_t *steps_per_iter) {
@@ -1168,7 +1197,7 @@ struct rc_group {
     hipEvent_t ev0 = nullptr, ev1 = nullptr, evm0 = nullptr, evm1 = nullptr;
     std::vector<DevSnap> h;
     DevBuf dsnap, de, ds, stats, queue, srtk;
-    int wg_per_cu = 0, threshold = 0, spi = 0;
+    int wg_per_cu = 0, threshold = 0, spi = 0, refill = -1;
     rc_stats last{};
     bool launched = false;
 };
@@ -1232,6 +1261,12 @@ int rc_group_set_schedule(rc_group *g, int wg_per_cu, int threshold, int steps_p
     return 0;
 }
 
+int rc_group_set_refill(rc_group *g, int mode) {
+    if (!g || mode < -1 || mode > 1) { set_error("rc_group_set_refill: mode -1 (default), 0 or 1"); return -1; }
+    g->refill = mode;
+    return 0;
+}
+
 int rc_group_run(rc_group *g) {
     const Range range_("radchan: group launch");
     if (!g) { set_error("null group"); return -1; }
@@ -1271,6 +1306,7 @@ int rc_group_run(rc_group *g) {
     if (int e = g->stats.ensure(8 * sizeof(unsigned long long))) return e;
     if (int e = g->queue.ensure(32 * sizeof(int))) return e;
     DevCommon cm = make_common(c0);
+    cm.refill = g->refill;
     const size_t nk = std::max(batched_srt_keys(cm, cells, g->wg_per_cu),
                                walls ? batched_srt_keys(cm, wcells, g->wg_per_cu) : 0);
     if (nk)

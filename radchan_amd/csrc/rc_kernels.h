@@ -29,7 +29,8 @@ struct DevCommon {
      * with compensation; nullptr: FP64 atomics, the default */
     float *ray_de;
     int sampler;                   /* fluid directions: 1 Sobol points shifted per (cell, run), 0 Philox */
-    int pad_s;
+    int refill;                    /* batched: 1 also refill idle lanes between an iteration's march steps, */
+                                   /* 0 only once per iteration, < 0 the default (batched_schedule)      */
     /* batched variant, srt above 256 keys: the radix sort's scratch, 2
      * rows of photmax (rounded up to 64) keys per workgroup of the launch
      * (batched_srt_keys) */
@@ -151,6 +152,9 @@ void launch_reduce_rays(hipStream_t st, const float *ray, size_t nunit, int phot
 /* workgroups of a batched launch over nunit_cells emitting (wall) cells,
  * and the 16-bit keys of radix-sort scratch it needs (0: none) */
 int batched_grid(const DevCommon &cm, long nunit_cells, int wg_per_cu);
+/* the batched schedule with its defaults filled in (values <= 0, refill
+ * < 0: the defaults) */
+void batched_schedule(const DevCommon &cm, int &wg_per_cu, int &threshold, int &spi, int &refill);
 size_t batched_srt_keys(const DevCommon &cm, long nunit_cells, int wg_per_cu);
 void launch_zero(hipStream_t st, double *p, size_t n);
 void launch_zero_u64(hipStream_t st, unsigned long long *p, int n);

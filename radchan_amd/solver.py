@@ -99,8 +99,10 @@ class Solver:
                                   out.ctypes.data_as(C.POINTER(L.Ray))), "rc_trace")
         return out
 
-    def set_schedule(self, wg_per_cu: int = 0, threshold: int = 0, steps_per_iter: int = 0) -> None:
+    def set_schedule(self, wg_per_cu: int = 0, threshold: int = 0, steps_per_iter: int = 0,
+                     refill: int = -1) -> None:
         L.check(self.lib.rc_set_schedule(self.ctx, wg_per_cu, threshold, steps_per_iter), "rc_set_schedule")
+        L.check(self.lib.rc_set_refill(self.ctx, refill), "rc_set_refill")
 
     def set_table_precision(self, bf16: bool) -> None:
         L.check(self.lib.rc_set_table_precision(self.ctx, int(bf16)), "rc_set_table_precision")
@@ -126,7 +128,8 @@ class Solver:
     def autotune(self) -> dict:
         wg, th, sp, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_float()
         L.check(self.lib.rc_autotune(self.ctx, C.byref(wg), C.byref(th), C.byref(sp), C.byref(ms)), "rc_autotune")
-        return {"wg_per_cu": wg.value, "threshold": th.value, "steps_per_iter": sp.value, "march_ms": ms.value}
+        return {"wg_per_cu": wg.value, "threshold": th.value, "steps_per_iter": sp.value,
+                "refill": self.lib.rc_get_refill(self.ctx), "march_ms": ms.value}
 
     def info(self, s: int = 0) -> dict:
         f = np.zeros(5, np.float32)
@@ -200,8 +203,10 @@ class Group:
         except Exception:
             pass
 
-    def set_schedule(self, wg_per_cu: int = 0, threshold: int = 0, steps_per_iter: int = 0) -> None:
+    def set_schedule(self, wg_per_cu: int = 0, threshold: int = 0, steps_per_iter: int = 0,
+                     refill: int = -1) -> None:
         L.check(self.lib.rc_group_set_schedule(self.g, wg_per_cu, threshold, steps_per_iter), "rc_group_set_schedule")
+        L.check(self.lib.rc_group_set_refill(self.g, refill), "rc_group_set_refill")
 
     def run(self) -> None:
         L.check(self.lib.rc_group_run(self.g), "rc_group_run")

@@ -257,7 +257,8 @@ def test_autotune_keeps_results(nb_tables):
     ref = s.tallies()
     best = s.autotune()
     assert best["wg_per_cu"] in (8, 12, 16, 20, 24, 32, 40) and best["threshold"] in (4, 8, 16, 32)
-    assert best["steps_per_iter"] in (1, 2, 3, 4)
+    assert best["steps_per_iter"] in (1, 2, 3, 4, 6, 8) and best["refill"] in (0, 1)
+    assert best["refill"] == 1 or best["steps_per_iter"] <= 4
     s.solve(VARIANT_BATCHED)
     assert np.abs(s.tallies() - ref).max() <= 1e-12 * np.abs(ref).max()
     s.close()
@@ -312,4 +313,26 @@ def test_plain_one_launch_per_band_equals_plain(nb_tables):
         assert np.array_equal(s.tallies(), a) and np.array_equal(s.surface()[2], aw)
         assert st1["rays"] == st0["rays"] and st1["steps"] == st0["steps"]
         assert st1["launches"] > p.nB * p.nVar
+        s.close()
+
+
+@pytest.mark.parametrize("refill", [0, 1])
+def test_refill_modes(nb_tables, refill):
+    """Refills once per iteration and between the march steps, on a single
+    grid and on three levels with walls: the tallies equal the oracle's
+    (FP64 sums of its per-ray FP32 tallies) either way."""
+    for p, tab, T in (cases.nb_case(nb_tables, n=12, photmax=64), cases.mg_case(nb_tables)):
+        print(f"refill {refill}: {p.imax}x{p.jmax}x{p.kmax} grid_num {p.grid_num}", flush=True)
+        s = Solver(p, tab)
+        s.set_schedule(16, 4, 6, refill)
+        s.set_fields([T])
+        s.solve(VARIANT_BATCHED)
+        got = s.tallies()
+        o = O.Oracle(p, tab, T)
+        _, de32d, _, _ = o.solve(O.all_cells(p))
+        assert np.abs(got - de32d).max() <= 1e-12 * np.abs(de32d).max()
+        if p.calcwalls:
+            _, w32d, _, _ = o.solve(O.all_wall_cells(p), surface=True)
+            gw = s.surface()[2].reshape(-1, p.nVar)
+            assert np.abs(gw - w32d).max() <= 1e-12 * np.abs(w32d).max()
         s.close()
