@@ -577,14 +577,100 @@ def bench_c4(args) -> int:
     return 0
 
 
+def bench_slab(args) -> int:
+    """One GPU's share of param.h's production run as it stands: the whole
+    406x576x2304 field resident (a laminar 300-1500 K profile with a 3%
+    sin/cos modulation), the 48-plane z slab of rank 24 of p_col = 48 solved
+    (406x576x48 cells), photmax 6000, nine levels with maxs 5, bd 4 z walls,
+    Sobol directions, NarrowBand.h's table sizes, nVar 2 (param.h has 1; two
+    runs give the std-error).  Legs: the batched solver at its default
+    schedule (no autotune: a solve takes seconds), BF16 tables, and the
+    plain solver, each timed once after (batched) one warm-up solve."""
+    from radchan_amd import VARIANT_BATCHED, VARIANT_PLAIN, Solver, config
+    from radchan_amd import fields as F
+    from radchan_amd import lib as L
+    from radchan_amd.tables import synthetic_nb
+
+    p = config("slab406")
+    p.nVar = args.nvar if args.nvar_given else 2
+    rank = p.p_col // 2
+    kloc = p.kmax // p.p_col
+    z0 = rank * kloc + 1
+    lib = L.load()
+    log(f"bench: workload slab406 {p.imax}x{p.jmax}x{p.kmax} field, z slab {z0}..{z0 + kloc - 1} of rank {rank} "
+        f"(p_col {p.p_col}), photmax {p.photmax}, nVar {p.nVar}")
+    t0 = time.perf_counter()
+    T = F.wavy_channel_field(p, 300.0, 1500.0, amp=0.03)
+    tab = synthetic_nb(p.nB, p.nT, p.nQ)
+    log(f"field and tables built in {time.perf_counter() - t0:.1f} s")
+    s = Solver(p, tab)
+    s.set_fields([T], zstart=z0)
+    # the library's defaults for a multigrid field above 256 rays per cell
+    # (the refill mode, threshold and steps the autotuner picks at param.h's
+    # settings on 28x96x128), set here so that the line names what ran
+    sched = {"wg_per_cu": 20, "threshold": 8, "steps_per_iter": 2, "refill": 1}
+    s.set_schedule(sched["wg_per_cu"], sched["threshold"], sched["steps_per_iter"], sched["refill"])
+    steps = max(1, min(args.steps, 2))
+    legs = {}
+    for name, use_bf16 in (("fp32_tables", False), ("bf16_tables", True)):
+        s.set_table_precision(use_bf16)
+        wall, st, march = timed_leg(s, lib, VARIANT_BATCHED, steps, 1)
+        eps, p95 = std_error(s.tallies())
+        legs[name] = {"rays_per_s": st["rays"] / wall, "ms_per_step": 1e3 * wall, "march_ms": march, "eps": eps,
+                      "eps_p95_cell": p95, "time_to_1pct_s": wall * (eps / 0.01) ** 2, "rays": st["rays"],
+                      "steps": st["steps"], "timed_solves": steps, **lane_use(st)}
+        log(f"batched {name}: {1e3 * wall:.1f} ms/step, {st['rays'] / wall / 1e9:.3f} Grays/s, "
+            f"{st['steps'] / st['rays']:.2f} steps/ray, eps {eps:.4g}, bulk active lanes "
+            f"{legs[name]['bulk_active_lane_frac']:.3f}")
+    head = legs["fp32_tables"]
+    s.set_table_precision(False)
+    plain = None
+    if not args.no_plain:
+        pwall, sp, _ = timed_leg(s, lib, VARIANT_PLAIN, 1, 0)
+        eps_p, _ = std_error(s.tallies())
+        plain = {"rays_per_s": sp["rays"] / pwall, "ms_per_step": 1e3 * pwall, "eps": eps_p,
+                 "time_to_1pct_s": pwall * (eps_p / 0.01) ** 2, "timed_solves": 1}
+        log(f"plain:   {1e3 * pwall:.1f} ms/step, {sp['rays'] / pwall / 1e9:.3f} Grays/s, eps {eps_p:.4g}")
+    ncell = s.ncell
+    s.close()
+    march_ms = float(np.mean(head["march_ms"]))
+    alg_bytes = head["steps"] * B_STEP + head["rays"] * B_RAY + ncell * B_CELL
+    achieved = alg_bytes / (march_ms * 1e-3) / 1e9
+    roof = {"bound": "hbm", "achieved": round(achieved, 2), "peak": HBM_PEAK_GBS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBS, 4), "traffic": None,
+            "kernel": "k_batched_mg<false, 0, true> (persistent march, FP32 k-tables)",
+            "kernel_ms": round(march_ms, 4), "algorithmic_bytes": int(alg_bytes),
+            "steps_per_launch": int(head["steps"]), "rays_per_launch": int(head["rays"]),
+            "bytes_per_step": B_STEP, "bytes_per_ray": B_RAY, "bytes_per_cell": B_CELL}
+    out = {
+        "metric": "rays/s (batched solver) and time to 1% std-error of div q_r; batched vs plain, 1 MI355X",
+        "value": head["rays_per_s"], "unit": "rays/s", "n_gpus": 1, "steps": steps, "warmup": 1,
+        "ms_per_step": head["ms_per_step"], "higher_is_better": True, "scaling": "weak", "vs_baseline": None,
+        "dtype": "fp32 march, fp32 k-tables, fp64 tallies",
+        "data": "synthetic (H2O-like tables of NarrowBand.h's sizes, laminar 300-1500 K channel T field, 3% "
+                "sin/cos modulation)",
+        "config": {"workload": f"param.h production share: {p.imax}x{p.jmax}x{p.kmax} field, z slab of rank {rank} "
+                               f"of p_col {p.p_col} ({p.imax}x{p.jmax}x{kloc} cells), photmax {p.photmax}, nVar "
+                               f"{p.nVar}, grid_num {p.grid_num}, Sobol directions, {p.nB} bands x {p.nQ} g x "
+                               f"{p.nT} T", "zstart": z0, "cells": ncell, "rays_per_step": int(head["rays"]),
+                   "steps_per_ray": head["steps"] / head["rays"], "schedule": sched},
+        "eps": head["eps"], "time_to_1pct_s": head["time_to_1pct_s"],
+        "batched": legs, "plain": plain,
+        "speedup_vs_plain": (head["rays_per_s"] / plain["rays_per_s"]) if plain else None,
+        "roofline": roof,
+    }
+    print(json.dumps(out), flush=True)
+    return 0
+
+
 def main() -> int:
     ap = argparse.ArgumentParser(description=__doc__, formatter_class=argparse.RawDescriptionHelpFormatter)
     ap.add_argument("--gpus", type=int, default=1)
     ap.add_argument("--steps", type=int, default=5)
     ap.add_argument("--warmup", type=int, default=1)
     ap.add_argument("--workload", default="big256", choices=["nb64", "nb128", "big256", "grey32", "mg192", "prod28",
-                                                                 "c4", "ckdist"])
-    ap.add_argument("--nvar", type=int, default=8)
+                                                                 "c4", "ckdist", "slab406"])
+    ap.add_argument("--nvar", type=int, default=None, help="runs per cell (default 8; slab406: 2)")
     ap.add_argument("--tables", default="fp32", choices=["bf16", "fp32"],
                     help="k-table storage of the headline (batched) leg; both are measured")
     ap.add_argument("--plain-steps", type=int, default=None,
@@ -610,6 +696,9 @@ def main() -> int:
                     help="no timing: one elementwise copy of 512 MiB (counter calibration) and one batched "
                          "solve, for hardware-counter runs")
     args = ap.parse_args()
+    args.nvar_given = args.nvar is not None
+    if args.nvar is None:
+        args.nvar = 8
     global RESTRICTION
     RESTRICTION = 1 if args.restriction == "spline" else 0
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) != 1:
@@ -619,6 +708,8 @@ def main() -> int:
 
     if args.workload == "c4":
         return bench_c4(args)
+    if args.workload == "slab406":
+        return bench_slab(args)
     if args.workload == "ckdist":
         ck = ckdist_leg(not args.no_cpu, reps=args.steps)
         print(json.dumps({"metric": "correlated-k distributions per second (rc_ckdist), 1 MI355X",

@@ -66,3 +66,29 @@ def pad_field(p: L.Params, inner: np.ndarray, T_wall: float) -> np.ndarray:
     T[0, :, :] = T[km, :, :]
     T[km + 1, :, :] = T[1, :, :]
     return np.ascontiguousarray(T.reshape(-1))
+
+
+def wavy_channel_field(p: L.Params, T_wall: float, T_centre: float, amp: float = 0.03) -> np.ndarray:
+    """T(x, y, z) = [Tw + (Tc - Tw) (1 - (2x/Lx - 1)^2)] (1 + amp sin(2 pi j/jmax)
+    cos(2 pi k/kmax)), built 64 z planes at a time in FP32 (for fields of
+    hundreds of millions of cells, where the noisy channel_field's FP64
+    temporaries do not fit comfortably); ghost planes as pad_field."""
+    im, jm, km = p.imax, p.jmax, p.kmax
+    x = centres_x(p)[1:im + 1].astype(np.float64)
+    prof = (T_wall + (T_centre - T_wall) * (1.0 - (2.0 * x / p.Lx - 1.0) ** 2)).astype(np.float32)
+    sy = (amp * np.sin(2.0 * np.pi * np.arange(1, jm + 1) / jm)).astype(np.float32)
+    cz = np.cos(2.0 * np.pi * np.arange(1, km + 1) / km).astype(np.float32)
+    T = np.empty((km + 2, jm + 2, im + 2), np.float32)
+    tw2 = np.float32(2.0 * T_wall)
+    for k0 in range(0, km, 64):
+        k1 = min(km, k0 + 64)
+        blk = T[k0 + 1:k1 + 1]
+        np.multiply(prof[None, None, :], (np.float32(1.0) + sy[None, :] * cz[k0:k1, None])[:, :, None],
+                    out=blk[:, 1:jm + 1, 1:im + 1])
+        blk[:, 1:jm + 1, 0] = tw2 - blk[:, 1:jm + 1, 1]
+        blk[:, 1:jm + 1, im + 1] = tw2 - blk[:, 1:jm + 1, im]
+        blk[:, 0] = blk[:, jm]
+        blk[:, jm + 1] = blk[:, 1]
+    T[0] = T[km]
+    T[km + 1] = T[1]
+    return T.reshape(-1)

@@ -97,6 +97,8 @@ def config(name: str) -> L.Params:
         p.nB, p.nT, p.nQ = r.nB, r.nT, r.nQ   # NarrowBand.h's table sizes
         p.sampler = r.sampler                 # param.h random 1: Sobol directions
         return p
+    if name == "slab406":         # param.h as it stands: 406x576x2304 in 48 z slabs, one GPU's slab solved
+        return reference_params()
     if name == "ens64x96":        # the small snapshots of the ensemble
         return base_params(64, 96, 64, photmax=100, fact_mesh=1.5)
     raise KeyError(name)

@@ -14,7 +14,9 @@ multi-node run, against the CPU oracle on the same Philox streams.
   solves the whole field, the oracle sampled cells.
 * The reference's production parameters (param.h: photmax 6000, nine
   grid levels, 5 steps per level but the last, bd 4 z walls) on a reduced
-  grid with the same coarsening pattern.
+  grid with the same coarsening pattern, and as they stand: one GPU's z
+  slab (rank 24 of p_col = 48) of the 406x576x2304 field, batched solver
+  against sampled oracle cells.
 
 Bar (as the parity tests): plain tallies equal to the oracle's FP32 sums
 bit for bit, batched FP64 tallies to 1e-12 of the largest.
@@ -240,3 +242,36 @@ def test_reference_production_parameters(nb_tables):
         _check(variant, s.tallies()[idx], de32, de32d)
         assert st["rays"] == len(cells) * 6000 * 2
         s.close()
+
+
+def test_production_share_at_full_size():
+    """param.h unchanged: the 406x576x2304 field (544 M cells with ghosts,
+    2.2 GB) resident, the 48-plane z slab of rank 24 of p_col = 48 solved
+    (11.2 M cells x 6000 rays, nine levels, Sobol directions, 999 bands x
+    135 T x 16 g), batched solver against 24 sampled oracle cells.  The
+    sizes reach the library's 32-bit index ranges (table positions of all
+    levels 0.63 G of the 2^31 it accepts) and 64-bit ray and step counts.
+    The plain variant is checked at these settings on the reduced grid
+    above (here its sorted keys alone would take 135 GB)."""
+    from radchan_amd.tables import synthetic_nb
+    p = P.reference_params()
+    assert (p.imax, p.jmax, p.kmax, p.p_col, p.photmax, p.grid_num) == (406, 576, 2304, 48, 6000, 9)
+    kloc = p.kmax // p.p_col
+    z0 = 24 * kloc + 1
+    print("production share: field and tables", flush=True)
+    T = F.wavy_channel_field(p, 300.0, 1500.0, amp=0.03)
+    tab = synthetic_nb(p.nB, p.nT, p.nQ)
+    print("production share: oracle set-up (all nine levels of the whole field)", flush=True)
+    o = O.Oracle(p, tab, T, zstart=z0)
+    cells = O.all_cells(p, z0)
+    idx = _sample(len(cells), 24, 7)
+    de32, de32d, _, _ = o.solve(cells[idx], nthreads=8)
+    o.close()
+    print(f"production share: batched solve of {len(cells)} cells", flush=True)
+    s = Solver(p, tab)
+    s.set_fields([T], zstart=z0)
+    st = s.solve(VARIANT_BATCHED)
+    _check(VARIANT_BATCHED, s.tallies()[idx], de32, de32d)
+    assert st["rays"] == len(cells) * p.photmax * p.nVar
+    assert st["steps"] > 2 ** 32
+    s.close()
