@@ -605,11 +605,19 @@ def bench_slab(args) -> int:
     log(f"field and tables built in {time.perf_counter() - t0:.1f} s")
     s = Solver(p, tab)
     s.set_fields([T], zstart=z0)
-    # the library's defaults for a multigrid field above 256 rays per cell
-    # (the refill mode, threshold and steps the autotuner picks at param.h's
-    # settings on 28x96x128), set here so that the line names what ran
-    sched = {"wg_per_cu": 20, "threshold": 8, "steps_per_iter": 2, "refill": 1}
+    # the library's defaults on several levels (refills once per iteration,
+    # steps per iteration = the finest level's cap, maxs 5: the fastest of
+    # the schedules timed on this share), set here so that the line names
+    # what ran
+    sched = {"wg_per_cu": 20, "threshold": 4, "steps_per_iter": p.maxs[0], "refill": 0}
     s.set_schedule(sched["wg_per_cu"], sched["threshold"], sched["steps_per_iter"], sched["refill"])
+    if args.probe:
+        # no timing: one batched solve, for hardware-counter runs
+        st = s.solve(VARIANT_BATCHED)
+        print(json.dumps({"probe": "slab406", "nVar": p.nVar, "rays": st["rays"], "steps": st["steps"],
+                          "march_ms": st["march_ms"], "schedule": sched}), flush=True)
+        s.close()
+        return 0
     steps = max(1, min(args.steps, 2))
     legs = {}
     for name, use_bf16 in (("fp32_tables", False), ("bf16_tables", True)):

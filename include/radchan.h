@@ -284,10 +284,11 @@ int rc_get_schedule(rc_ctx *c, int32_t *wg_per_cu, int32_t *threshold, int32_t *
 /* refills of the batched solver (no counterpart in the reference): 0 =
  * once per loop iteration (rays that end on an earlier step wait for it),
  * 1 = also between the iteration's march steps, -1 = the default (1 for a
- * single grid and for photmax above 256, 0 for several levels).  Defaults
- * of the schedule with it: (32, 4, 8) single grid; (20, 4, 4) several
- * levels; (20, 8, 2) several levels above 256 rays per cell.  Chosen by
- * the autotuner; rc_get_refill returns the mode set (-1: default). */
+ * single grid, 0 for several levels).  Defaults of the schedule with it:
+ * (32, 4, 8) single grid; (20, 4, maxs[0]) several levels, the finest
+ * level's step cap (clamped to 1..16), so that rays picked up together
+ * hand off to the next level together.  Chosen by the autotuner;
+ * rc_get_refill returns the mode set (-1: default). */
 int rc_set_refill(rc_ctx *c, int mode);
 int rc_get_refill(rc_ctx *c);
 /* storage of the k tables on the device: 0 = FP32 (default), 1 = BF16 k
@@ -326,8 +327,9 @@ int rc_set_estimator(rc_ctx *c, int mode);
  * groups use mode 0. */
 int rc_set_reduction(rc_ctx *c, int mode);
 /* time the batched solve of the loaded snapshots over schedules (steps per
- * iteration, then workgroups per CU, then refill threshold, each at the
- * best of the previous; then refills between the steps with steps per
+ * iteration 1..4 and, on several levels, the finest level's step cap, then
+ * workgroups per CU, then refill threshold, each at the best of the
+ * previous; then refills between the steps with steps per
  * iteration 2..8 at threshold 4 and, if they win, threshold 8) and keep
  * the fastest, refill mode included (rc_get_refill) */
 int rc_autotune(rc_ctx *c, int32_t *best_wg, int32_t *best_th, int32_t *best_spi, float *best_ms);

@@ -383,6 +383,7 @@ DevCommon make_common(const rc_ctx *c) {
     cm.ray_de = nullptr;
     cm.sampler = p.sampler == 1 ? 1 : 0;
     cm.refill = c->refill;
+    cm.maxs0 = p.maxs[0];
     return cm;
 }
 
@@ -1003,7 +1004,10 @@ int rc_set_graph(rc_ctx *c, int on) {
 int rc_autotune(rc_ctx *c, int32_t *best_wg, int32_t *best_th, int32_t *best_spi, float *best_ms) {
     const Range range_("radchan: autotune");
     if (!c || c->snaps.empty()) { set_error("rc_autotune before rc_set_fields"); return -1; }
-    static const int spis[] = {1, 2, 3, 4};
+    /* steps per iteration 1..4, and on several levels the finest level's
+     * step cap (rays picked up together hand off together) */
+    std::vector<int> spis = {1, 2, 3, 4};
+    if (c->p.grid_num > 1 && c->p.maxs[0] > 4 && c->p.maxs[0] <= 16) spis.push_back(c->p.maxs[0]);
     static const int wgs[] = {8, 12, 16, 20, 24, 32, 40};
     static const int ths[] = {4, 8, 16, 32};
     static const int rspis[] = {2, 4, 6, 8};
@@ -1100,8 +1104,8 @@ int rc_surface_results(rc_ctx *c, int s, float *mean, float *sdev, double *de) {
 
 int rc_set_schedule(rc_ctx *c, int wg_per_cu, int threshold, int steps_per_iter) {
     if (!c || wg_per_cu < 0 || wg_per_cu > 64 || threshold < 0 || threshold > 64 || steps_per_iter < 0 ||
-        steps_per_iter > 8) {
-        set_error("schedule: wg_per_cu in 0..64, threshold in 0..64, steps_per_iter in 0..8 (0 = default)");
+        steps_per_iter > 16) {
+        set_error("schedule: wg_per_cu in 0..64, threshold in 0..64, steps_per_iter in 0..16 (0 = default)");
         return -1;
     }
     c->wg_per_cu = wg_per_cu;
@@ -1253,8 +1257,8 @@ void rc_group_destroy(rc_group *g) {
 
 int rc_group_set_schedule(rc_group *g, int wg_per_cu, int threshold, int steps_per_iter) {
     if (!g) { set_error("null group"); return -1; }
-    if (wg_per_cu < 0 || wg_per_cu > 64 || threshold < 0 || threshold > 64 || steps_per_iter < 0 || steps_per_iter > 8) {
-        set_error("schedule out of range (wg_per_cu 0..64, threshold 0..64, steps_per_iter 0..8)");
+    if (wg_per_cu < 0 || wg_per_cu > 64 || threshold < 0 || threshold > 64 || steps_per_iter < 0 || steps_per_iter > 16) {
+        set_error("schedule out of range (wg_per_cu 0..64, threshold 0..64, steps_per_iter 0..16)");
         return -2;
     }
     g->wg_per_cu = wg_per_cu; g->threshold = threshold; g->spi = steps_per_iter;

@@ -31,6 +31,7 @@ struct DevCommon {
     int sampler;                   /* fluid directions: 1 Sobol points shifted per (cell, run), 0 Philox */
     int refill;                    /* batched: 1 also refill idle lanes between an iteration's march steps, */
                                    /* 0 only once per iteration, < 0 the default (batched_schedule)      */
+    int maxs0;                     /* step cap of the finest level (the default schedule's steps)        */
     /* batched variant, srt above 256 keys: the radix sort's scratch, 2
      * rows of photmax (rounded up to 64) keys per workgroup of the launch
      * (batched_srt_keys) */
