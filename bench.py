@@ -231,7 +231,7 @@ def field_legs(args, lib, p, tab, T, bf16: bool, steps: int, warmup: int, plain_
     s = new_session(p, tab, [T])
     sched = None
     if not args.no_autotune:
-        sched = s.autotune()                       # untimed: picks wavefronts per CU and refill threshold
+        sched = s.autotune()                       # untimed: picks the schedule and refill mode
         log(f"autotune {p.imax}x{p.jmax}x{p.kmax}: {sched}")
     legs = {}
     for name, use_bf16 in (("fp32_tables", False), ("bf16_tables", True)):

@@ -927,18 +927,16 @@ void launch_zero(hipStream_t st, double *p, size_t n) {
     hipLaunchKernelGGL(k_zero_f64, dim3(blocks), dim3(256), 0, st, p, n);
 }
 
-/* defaults, what the autotuner picks on the benchmark fields: a single
- * grid (256x128x256, 64^3) 32 workgroups per CU, refill at 4 idle lanes,
- * 8 steps per iteration with refills between them; several levels
- * (192x128x192: short rays) 20, 4, 4 without refills between the steps,
- * and with the radix-sorted keys of large photmax (param.h's settings)
- * 20, 8, 2 with them.  A refill mode < 0 selects the default. */
+/* workgroups per CU by default: 32 on a single grid, 20 on several levels
+ * (what the autotuner picks on 256x128x256 and 192x128x192) */
 static int default_wg_per_cu(const DevCommon &cm, int wg_per_cu) {
     return wg_per_cu > 0 ? wg_per_cu : (cm.grid_num > 1 ? 20 : 32);
 }
 
-/* Defaults of the schedule.  Single grid: refills between the steps, 8
- * steps per iteration.  Several levels: refills once per iteration and as
+/* Defaults of the schedule (a refill mode < 0, a threshold or steps per
+ * iteration <= 0).  Single grid: refills at 4 idle lanes, also between
+ * the steps, 8 steps per iteration (the autotuner's pick at 256x128x256).
+ * Several levels: refills at 4 idle lanes once per iteration and as
  * many steps per iteration as the finest level's step cap, so that the
  * rays a wavefront picks up together reach that cap, and hand off to the
  * next level, on the same step: one pass through the hand-off serves all

@@ -1000,7 +1000,8 @@ int rc_set_graph(rc_ctx *c, int on) {
 
 /* coordinate search over schedules on the loaded snapshots (batched
  * variant, device-event timing of the march): steps per iteration at the
- * default occupancy, then workgroups per CU, then the refill threshold */
+ * default occupancy, then workgroups per CU, then the refill threshold,
+ * refilling once per iteration; then refills between the steps too */
 int rc_autotune(rc_ctx *c, int32_t *best_wg, int32_t *best_th, int32_t *best_spi, float *best_ms) {
     const Range range_("radchan: autotune");
     if (!c || c->snaps.empty()) { set_error("rc_autotune before rc_set_fields"); return -1; }
