@@ -285,7 +285,7 @@ int rc_get_schedule(rc_ctx *c, int32_t *wg_per_cu, int32_t *threshold, int32_t *
  * once per loop iteration (rays that end on an earlier step wait for it),
  * 1 = also between the iteration's march steps, -1 = the default (1 for a
  * single grid, 0 for several levels).  Defaults of the schedule with it:
- * (32, 4, 8) single grid; (20, 4, maxs[0]) several levels, the finest
+ * (32, 4, 6) single grid; (20, 4, maxs[0]) several levels, the finest
  * level's step cap (clamped to 1..16), so that rays picked up together
  * hand off to the next level together.  Chosen by the autotuner;
  * rc_get_refill returns the mode set (-1: default). */

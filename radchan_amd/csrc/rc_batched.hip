@@ -935,7 +935,10 @@ static int default_wg_per_cu(const DevCommon &cm, int wg_per_cu) {
 
 /* Defaults of the schedule (a refill mode < 0, a threshold or steps per
  * iteration <= 0).  Single grid: refills at 4 idle lanes, also between
- * the steps, 8 steps per iteration (the autotuner's pick at 256x128x256).
+ * the steps, 6 steps per iteration: within 1 % of the autotuner's pick at
+ * 256x128x256 (8 steps) and of the best at 64^3, where rays of 6 steps end
+ * faster than the one round of 64 new rays per iteration replaces them at
+ * 8 steps (0.60 of the lanes active, 10 % slower).
  * Several levels: refills at 4 idle lanes once per iteration and as
  * many steps per iteration as the finest level's step cap, so that the
  * rays a wavefront picks up together reach that cap, and hand off to the
@@ -947,7 +950,7 @@ void batched_schedule(const DevCommon &cm, int &wg_per_cu, int &threshold, int &
     if (refill < 0) refill = mg ? 0 : 1;
     wg_per_cu = default_wg_per_cu(cm, wg_per_cu);
     if (threshold <= 0) threshold = 4;
-    if (spi <= 0) spi = !mg ? (refill ? 8 : 3) : std::max(1, std::min(16, cm.maxs0));
+    if (spi <= 0) spi = !mg ? (refill ? 6 : 3) : std::max(1, std::min(16, cm.maxs0));
 }
 
 int batched_grid(const DevCommon &cm, long nunit_cells, int wg_per_cu) {

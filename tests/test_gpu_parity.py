@@ -336,3 +336,26 @@ def test_refill_modes(nb_tables, refill):
             gw = s.surface()[2].reshape(-1, p.nVar)
             assert np.abs(gw - w32d).max() <= 1e-12 * np.abs(w32d).max()
         s.close()
+
+
+@pytest.mark.parametrize("maxs0", [12, 40])
+def test_long_iterations_on_several_levels(nb_tables, maxs0):
+    """Steps per iteration above 8: the default on several levels is the
+    finest level's step cap (12; 40 clamped to 16), and 16 steps with
+    refills between them; tallies equal the oracle's either way."""
+    p, tab, T = cases.mg_case(nb_tables)
+    p.maxs[0] = maxs0
+    o = O.Oracle(p, tab, T)
+    _, de32d, _, _ = o.solve(O.all_cells(p))
+    _, w32d, _, _ = o.solve(O.all_wall_cells(p), surface=True)
+    o.close()
+    for sched in ((0, 0, 0, -1), (20, 4, 16, 1)):
+        print(f"maxs[0] {maxs0}, schedule {sched}", flush=True)
+        s = Solver(p, tab)
+        s.set_schedule(*sched)
+        s.set_fields([T])
+        s.solve(VARIANT_BATCHED)
+        assert np.abs(s.tallies() - de32d).max() <= 1e-12 * np.abs(de32d).max()
+        gw = s.surface()[2].reshape(-1, p.nVar)
+        assert np.abs(gw - w32d).max() <= 1e-12 * np.abs(w32d).max()
+        s.close()
