@@ -288,9 +288,9 @@ int rc_get_schedule(rc_ctx *c, int32_t *wg_per_cu, int32_t *threshold, int32_t *
  * (32, 4, 6) single grid; (20, 4, maxs[0]) several levels, the finest
  * level's step cap (clamped to 1..16), so that rays picked up together
  * hand off to the next level together.  Chosen by the autotuner;
- * rc_get_refill returns the mode set (-1: default). */
+ * rc_get_refill stores the mode set (-1: default) in *mode. */
 int rc_set_refill(rc_ctx *c, int mode);
-int rc_get_refill(rc_ctx *c);
+int rc_get_refill(rc_ctx *c, int32_t *mode);
 /* storage of the k tables on the device: 0 = FP32 (default), 1 = BF16 k
  * pairs with a power-of-two scale per band (Ib stays FP32); rebuilds the
  * loaded tables */

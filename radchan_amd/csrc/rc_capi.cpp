@@ -1123,9 +1123,10 @@ int rc_set_refill(rc_ctx *c, int mode) {
     return 0;
 }
 
-int rc_get_refill(rc_ctx *c) {
-    if (!c) { set_error("null context"); return -1; }
-    return c->refill;
+int rc_get_refill(rc_ctx *c, int32_t *mode) {
+    if (!c || !mode) { set_error("rc_get_refill: null context or output"); return -1; }
+    *mode = c->refill;
+    return 0;
 }
 
 int rc_get_schedule(rc_ctx *c, int32_t *wg_per_cu, int32_t *threshold, int32_t *steps_per_iter) {

@@ -152,7 +152,7 @@ def load() -> C.CDLL:
         "rc_set_schedule": (C.c_int, [vp, C.c_int, C.c_int, C.c_int]),
         "rc_get_schedule": (C.c_int, [vp, P(C.c_int32), P(C.c_int32), P(C.c_int32)]),
         "rc_set_refill": (C.c_int, [vp, C.c_int]),
-        "rc_get_refill": (C.c_int, [vp]),
+        "rc_get_refill": (C.c_int, [vp, C.POINTER(C.c_int32)]),
         "rc_group_set_refill": (C.c_int, [vp, C.c_int]),
         "rc_set_graph": (C.c_int, [vp, C.c_int]),
         "rc_set_table_precision": (C.c_int, [vp, C.c_int]),

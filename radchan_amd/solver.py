@@ -128,8 +128,10 @@ class Solver:
     def autotune(self) -> dict:
         wg, th, sp, ms = C.c_int32(), C.c_int32(), C.c_int32(), C.c_float()
         L.check(self.lib.rc_autotune(self.ctx, C.byref(wg), C.byref(th), C.byref(sp), C.byref(ms)), "rc_autotune")
+        rf = C.c_int32()
+        L.check(self.lib.rc_get_refill(self.ctx, C.byref(rf)), "rc_get_refill")
         return {"wg_per_cu": wg.value, "threshold": th.value, "steps_per_iter": sp.value,
-                "refill": self.lib.rc_get_refill(self.ctx), "march_ms": ms.value}
+                "refill": rf.value, "march_ms": ms.value}
 
     def info(self, s: int = 0) -> dict:
         f = np.zeros(5, np.float32)

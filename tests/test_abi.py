@@ -92,6 +92,18 @@ def test_get_results_without_launch_fails_cleanly():
     assert np.all(res == 3.0) and "without a preceding mc_gpu_" in L.last_error()
 
 
+def test_schedule_setters_reject_null_contexts():
+    """The schedule and refill setters and getters of the batched solver
+    return an error with a message for a null context (no GPU needed)."""
+    lib = L.load()
+    assert lib.rc_set_schedule(None, 20, 4, 5) != 0 and "schedule" in L.last_error()
+    assert lib.rc_set_refill(None, 0) != 0 and "rc_set_refill" in L.last_error()
+    mode = C.c_int32(7)
+    assert lib.rc_get_refill(None, C.byref(mode)) != 0 and mode.value == 7
+    assert lib.rc_group_set_schedule(None, 20, 4, 5) != 0
+    assert lib.rc_group_set_refill(None, 1) != 0
+
+
 def test_table_reader_errors(tmp_path):
     with pytest.raises(RuntimeError, match="cannot open"):
         R.Tables.read(str(tmp_path / "none"), 4, 3, 16, 0.5)
