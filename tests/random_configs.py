@@ -1,0 +1,63 @@
+"""Seeded random solver configurations within the space the reference's
+param.h spans (grid sizes and levels, step caps, photmax across the
+register and radix sorts, nVar, sorted or unsorted narrow bands, banded
+grey, x walls adiabatic or at a fixed temperature, periodic / specular /
+black y walls, periodic / specular / bd 4 / bd 5 / black z walls, wall
+fluxes, Philox or Sobol directions, the exact estimator), small enough for
+the CPU oracle to solve every cell in about a second."""
+import numpy as np
+
+from radchan_amd import fields as F
+from radchan_amd import params as P
+
+
+def _levels(rng, n):
+    """grid_num 1..3: each coarser level divides the finer one per axis"""
+    nlev = int(rng.integers(1, 4))
+    sizes = [n]
+    for _ in range(nlev - 1):
+        prev = sizes[-1]
+        nxt = []
+        for a in prev:
+            divs = [d for d in (1, 2, 3) if a % d == 0 and a // d >= 1]
+            nxt.append(a // int(rng.choice(divs)))
+        sizes.append(tuple(nxt))
+    maxs = [int(rng.integers(2, 9)) for _ in range(nlev - 1)] + [100000]
+    return sizes, maxs
+
+
+def random_case(i: int, tables):
+    """configuration i (deterministic): (params, tables or None, T field)"""
+    rng = np.random.default_rng(1000 + i)
+    n = tuple(int(rng.choice([2, 3, 4, 6, 8, 9, 12, 16])) for _ in range(3))
+    sizes, maxs = _levels(rng, n)
+    grey = bool(rng.random() < 0.2)
+    photmax = int(rng.choice([1, 7, 33, 64, 100, 200, 257, 300]))
+    nVar = int(rng.integers(1, 4))
+    p = P.base_params(*n, photmax=photmax, nVar=nVar, fact_mesh=float(rng.choice([0.0, 1.2, 1.6])),
+                      grey=grey, srt=int(rng.random() < 0.75), seed=int(rng.integers(0, 1 << 20)),
+                      levels=sizes, maxs=maxs)
+    # x walls: black, adiabatic or at a fixed temperature (each side)
+    p.adiabw, p.adiabe = int(rng.random() < 0.6), int(rng.random() < 0.6)
+    p.Tw[0], p.Tw[1] = float(rng.uniform(350, 450)), float(rng.uniform(350, 450))
+    # y walls: periodic pair, or each specular / black at a fixed temperature
+    if rng.random() < 0.5:
+        p.bd[2] = p.bd[3] = 1
+    else:
+        p.bd[2], p.bd[3] = int(rng.choice([2, 3])), int(rng.choice([2, 3]))
+        p.Tw[2], p.Tw[3] = float(rng.uniform(350, 450)), float(rng.uniform(350, 450))
+    # z walls: periodic pair, or each specular / black / bd 4 / bd 5
+    if rng.random() < 0.4:
+        p.bd[4] = p.bd[5] = 1
+    else:
+        p.bd[4], p.bd[5] = int(rng.choice([2, 3, 4, 5])), int(rng.choice([2, 3, 4, 5]))
+        p.Tw[4], p.Tw[5] = float(rng.uniform(350, 450)), float(rng.uniform(350, 450))
+    p.calcwalls = int((p.adiabw == 1 or p.adiabe == 1) and rng.random() < 0.4)
+    p.sampler = int(rng.random() < 0.5)
+    estimator = int(rng.random() < 0.25)
+    # wall and centre temperatures close enough that the extrapolated x
+    # ghost planes (2 Tw - T of the first cell) stay positive, as the
+    # library requires
+    tw = float(rng.uniform(500, 800))
+    T = F.channel_field(p, tw, tw * float(rng.uniform(1.2, 1.7)), noise=0.03, seed=i)
+    return p, (None if grey else tables), T, estimator
