@@ -281,8 +281,8 @@ int snapshot_scalars(const rc_params &p, const Tables &t, SnapState &s) {
  * emission 4 pi sum K B w dnu exactly.  Walls: p(b) from B_b(Tmax) dnu_b and
  * Ibmax_s = eps pi Zs / photmax, the banded emission rather than eps sigma
  * T^4.  Sums in FP64 in band order, rounded once. */
-int snapshot_exact(const rc_params &p, const Tables &t, SnapState &s, std::vector<float> &cb, std::vector<float> &cg,
-                   std::vector<float> &cs) {
+int snapshot_exact(const rc_params &p, const Tables &t, const std::vector<float> &kq, SnapState &s,
+                   std::vector<float> &cb, std::vector<float> &cg, std::vector<float> &cs) {
     const int nB = t.nB, nQ = t.nQ, nT = t.nT, tm = s.tm;
     const double f = ((double)s.Tmax - (double)t.Tnb[tm]) / ((double)t.Tnb[tm + 1] - (double)t.Tnb[tm]);
     std::vector<double> qb(nB), qbg((size_t)nB * nQ), bs(nB);
@@ -293,7 +293,7 @@ int snapshot_exact(const rc_params &p, const Tables &t, SnapState &s, std::vecto
         const double dnu = ((double)t.wvr[b] - (double)t.wvl[b]) * 100.0;
         double sb = 0.0;
         for (int g = 0; g < nQ; ++g) {
-            const double k0 = t.kq[((size_t)b * nT + tm) * nQ + g], k1 = t.kq[((size_t)b * nT + tm + 1) * nQ + g];
+            const double k0 = kq[((size_t)b * nT + tm) * nQ + g], k1 = kq[((size_t)b * nT + tm + 1) * nQ + g];
             const double q = (k0 + f * (k1 - k0)) * B * (double)t.wq[g] * dnu;
             qbg[(size_t)b * nQ + g] = q;
             sb += q;
@@ -333,6 +333,30 @@ int snapshot_exact(const rc_params &p, const Tables &t, SnapState &s, std::vecto
     return 0;
 }
 
+/* the k-distribution values as the march reads them: with BF16 tables the
+ * values build_tableset stores (per band a power-of-two scale 2^(e-1) of
+ * the band's largest k, the scaled k rounded to BF16 nearest-even), so that
+ * the exact estimator's emission uses the k of the march */
+std::vector<float> kq_as_marched(const rc_ctx *c, const Tables &t) {
+    if (!c->table_bf16) return t.kq;
+    const int nB = t.nB, nT = t.nT, nQ = t.nQ;
+    std::vector<float> out(t.kq.size());
+    for (int b = 0; b < nB; ++b) {
+        float mx = 0.f;
+        for (size_t i = (size_t)b * nT * nQ; i < (size_t)(b + 1) * nT * nQ; ++i) mx = std::max(mx, t.kq[i]);
+        int ex = 0;
+        (void)std::frexp(mx > 0.f ? mx : 1.f, &ex);
+        const float sc = std::ldexp(1.0f, ex - 1), inv = std::ldexp(1.0f, 1 - ex);
+        for (size_t i = (size_t)b * nT * nQ; i < (size_t)(b + 1) * nT * nQ; ++i) {
+            const uint32_t u = (uint32_t)to_bf16(t.kq[i] * inv) << 16;
+            float v;
+            std::memcpy(&v, &u, 4);
+            out[i] = v * sc;
+        }
+    }
+    return out;
+}
+
 int snapshot_rows(rc_ctx *c, const Tables &t, SnapState &s) {
     const rc_params &p = c->p;
     const int nB = t.nB, nQ = t.nQ, ts = s.tstar;
@@ -350,7 +374,7 @@ int snapshot_rows(rc_ctx *c, const Tables &t, SnapState &s) {
     }
     for (int b = 0; b < nB; ++b) cs[b] = cs[b] / acc;
     if (c->estimator == 1 && !p.grey)
-        if (int e = snapshot_exact(p, t, s, cb, cg, cs)) return e;
+        if (int e = snapshot_exact(p, t, kq_as_marched(c, t), s, cb, cg, cs)) return e;
     /* guide tables: g[m] = first index with cdf > m/M, m = 0..M */
     auto guide = [&](const std::vector<float> &cdf) {
         const int M = kGuideM;
@@ -917,6 +941,11 @@ int rc_set_table_precision(rc_ctx *c, int bf16) {
             std::vector<int32_t> idx, ro;
             if (int e = build_tableset(c, sp->greyhost, *sp->grey, idx, ro)) return e;
         }
+    /* the exact estimator's emission follows the k the march reads */
+    if (c->estimator == 1 && !c->p.grey)
+        for (auto &sp : c->snaps)
+            if (sp)
+                if (int e = snapshot_rows(c, c->tab, *sp)) return e;
     if (!c->snaps.empty()) return push_snaps(c);
     return 0;
 }

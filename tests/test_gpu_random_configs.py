@@ -65,3 +65,48 @@ def test_random_configuration(nb_tables, i):
         o64.close()
         got, _ = _solve(q, tab, T, est, VARIANT_PLAIN)
         assert np.array_equal(got, want)
+
+
+@pytest.mark.parametrize("i", range(16))
+def test_random_session_features(nb_tables, i):
+    """Random configurations with the session's options drawn too: one z
+    slab of p_col = 1..3 (zstart of a random rank), 1..3 snapshots, the
+    spline restriction, the deterministic reduction, BF16 k tables (the
+    oracle fed the dequantized tables) and hipGraph replays; batched
+    against the oracle snapshot by snapshot."""
+    from radchan_amd import fields as F
+    from radchan_amd.tables import bf16_dequantized
+    p, tab, T0, est = random_case(200 + i, nb_tables)
+    rng = np.random.default_rng(5000 + i)
+    p.p_col = int(rng.choice([d for d in (1, 2, 3) if p.kmax % d == 0]))
+    kloc = p.kmax // p.p_col
+    z0 = int(rng.integers(0, p.p_col)) * kloc + 1
+    nsnap = int(rng.integers(1, 4))
+    tw = float(T0[(p.imax + 2) * (p.jmax + 2) + 1] + T0[(p.imax + 2) * (p.jmax + 2)]) / 2
+    Ts = [T0] + [F.channel_field(p, tw, tw * 1.5, noise=0.03, seed=700 + 10 * i + s) for s in range(1, nsnap)]
+    restriction = int(p.grid_num > 1 and min(p.imax, p.jmax, p.kmax) >= 2 and rng.random() < 0.5)
+    reduction = int(rng.random() < 0.5)
+    bf16 = bool(tab is not None and rng.random() < 0.5)
+    graph = bool(rng.random() < 0.5)
+    print(f"session features {i}: {p.imax}x{p.jmax}x{p.kmax} grid_num {p.grid_num} photmax {p.photmax} "
+          f"nVar {p.nVar} p_col {p.p_col} zstart {z0} snapshots {nsnap} restriction {restriction} "
+          f"reduction {reduction} bf16 {bf16} graph {graph} estimator {est}", flush=True)
+    s = Solver(p, tab)
+    if est:
+        s.set_estimator(est)
+    if restriction:
+        s.set_restriction(1)
+    s.set_fields(Ts, zstart=z0)
+    s.set_table_precision(bf16)
+    s.set_reduction(reduction)
+    s.set_graph(graph)
+    otab = bf16_dequantized(tab) if bf16 else tab
+    for rep in range(2 if graph else 1):
+        s.solve(VARIANT_BATCHED)
+        for k, T in enumerate(Ts):
+            o = O.Oracle(p, otab, T, zstart=z0, snapshot=k, restriction=restriction, estimator=est)
+            _, de32d, _, _ = o.solve(O.all_cells(p, z0))
+            o.close()
+            got = s.tallies(k)
+            assert np.abs(got - de32d).max() <= 1e-12 * np.abs(de32d).max(), (rep, k)
+    s.close()
