@@ -61,3 +61,27 @@ def random_case(i: int, tables):
     tw = float(rng.uniform(500, 800))
     T = F.channel_field(p, tw, tw * float(rng.uniform(1.2, 1.7)), noise=0.03, seed=i)
     return p, (None if grey else tables), T, estimator
+
+
+def regrid(p, rng):
+    """the same parameters on another random grid with the same number of
+    levels (fresh level sizes, step caps, stretching, wall temperatures):
+    what a session group may mix"""
+    q = p.copy()
+    while True:
+        n = tuple(int(rng.choice([2, 3, 4, 6, 8, 9, 12, 16])) for _ in range(3))
+        sizes, maxs = _levels(rng, n)
+        if len(sizes) == p.grid_num:
+            break
+    q.imax, q.jmax, q.kmax = n
+    P._levels(q, sizes, maxs)
+    q.fact_mesh = float(rng.choice([0.0, 1.2, 1.6]))
+    for f in range(6):
+        q.Tw[f] = float(rng.uniform(350, 450))
+    return q
+
+
+def field_for(p, rng, seed):
+    """a channel field whose extrapolated x ghost planes stay positive"""
+    tw = float(rng.uniform(500, 800))
+    return F.channel_field(p, tw, tw * float(rng.uniform(1.2, 1.7)), noise=0.03, seed=seed)
