@@ -214,3 +214,24 @@ def test_random_dropin(tmp_path, nb_tables, i):
             if on:
                 assert np.abs(r3[1:-1, 1:-1, ii] - wm[b]).max() <= 1e-6 * np.abs(wm[b]).max()
     assert np.all(r3[0] == -7.0) and np.all(r3[-1] == -7.0)
+
+
+@pytest.mark.parametrize("i", range(8))
+def test_random_radix_widths(nb_tables, i):
+    """Random configurations at random band-sort widths above 256 keys (the
+    radix sort through HBM), up to param.h's 6000: both variants against
+    the oracle on every cell."""
+    p, tab, T, est = random_case(500 + i, nb_tables)
+    rng = np.random.default_rng(9000 + i)
+    p.photmax = int(rng.integers(257, 6001))
+    if p.imax * p.jmax * p.kmax * p.photmax * p.nVar > 4_000_000:
+        p.nVar = 1
+    print(f"radix width {i}: {p.imax}x{p.jmax}x{p.kmax} grid_num {p.grid_num} photmax {p.photmax} nVar {p.nVar} "
+          f"srt {p.srt} grey {p.grey} sampler {p.sampler}", flush=True)
+    o = O.Oracle(p, tab, T, estimator=est)
+    de32, de32d, _, _ = o.solve(O.all_cells(p))
+    o.close()
+    got, _ = _solve(p, tab, T, est, VARIANT_PLAIN)
+    assert np.array_equal(got, de32.astype(np.float64))
+    got, _ = _solve(p, tab, T, est, VARIANT_BATCHED)
+    assert np.abs(got - de32d).max() <= 1e-12 * np.abs(de32d).max()
