@@ -119,14 +119,13 @@ __device__ __forceinline__ void set_dir64(Ray64 &r, double sx, double sy, double
     r.isx = 1.0 / sx; r.isy = 1.0 / sy; r.isz = 1.0 / sz;
 }
 
-/* one grid level of march_ray in double; returns the steps taken on it.
- * T holds the level's cell temperatures then its black walls' (the layout
- * of DevLevel.P) */
-__device__ inline int march_level64(const DevCommon &cm, const DevSnap &sn, const DevLevel &L, const float *T,
-                                    Ray64 &r, const En64 &e, float tol, double &de, int &steps, int &status) {
-    const double tl = (double)tol;
-    int counter = 0;
-    while (r.tra > tl && counter < L.maxs) {
+/* one step of march_ray in double on level L (one iteration of the loop
+ * at mc_gpu_noint.cu:252-654).  T holds the level's cell temperatures then
+ * its black walls' (the layout of DevLevel.P).  A ray that ends here is
+ * left with tra = 0 (absorbed below tl, at a black wall, or lost). */
+__device__ __forceinline__ void step64(const DevCommon &cm, const DevSnap &sn, const DevLevel &L, const float *T,
+                                       Ray64 &r, const En64 &e, double tl, double &de, int &steps, int &status) {
+    {
         const double fx = (double)L.xu[r.ic - r.f0];
         const double fy = (double)((float)(r.jc - r.f1) * L.dy);
         const double fz = (double)((float)(r.kc - r.f2) * L.dz);
@@ -212,9 +211,28 @@ __device__ inline int march_level64(const DevCommon &cm, const DevSnap &sn, cons
                 status = RC_RAY_LOST;
             }
         }
+    }
+}
+
+/* one grid level of march_ray in double; returns the steps taken on it */
+__device__ inline int march_level64(const DevCommon &cm, const DevSnap &sn, const DevLevel &L, const float *T,
+                                    Ray64 &r, const En64 &e, float tol, double &de, int &steps, int &status) {
+    const double tl = (double)tol;
+    int counter = 0;
+    while (r.tra > tl && counter < L.maxs) {
+        step64(cm, sn, L, T, r, e, tl, de, steps, status);
         counter += 1;
     }
     return counter;
+}
+
+/* the indices on level g + 1 of a ray leaving level g (:215-223) */
+__device__ __forceinline__ void coarsen64(const DevSnap &sn, int g, Ray64 &r) {
+    const DevLevel &L = sn.lev[g], &M = sn.lev[g + 1];
+    const int ri = L.im / M.im, rj = L.jm / M.jm, rk = L.km / M.km;
+    r.ic = (r.ic + ri - 1) / ri;
+    r.jc = (r.jc + rj - 1) / rj;
+    r.kc = (r.kc + rk - 1) / rk;
 }
 
 /* the multigrid loop (:210-224): a ray still alive after a level's step
@@ -230,11 +248,7 @@ __device__ inline void march_all64(const DevCommon &cm, const DevSnap &sn, Ray64
             if (n >= L.maxs) status = RC_RAY_CAPPED;
             return;
         }
-        const DevLevel &M = sn.lev[g + 1];
-        const int ri = L.im / M.im, rj = L.jm / M.jm, rk = L.km / M.km;
-        r.ic = (r.ic + ri - 1) / ri;
-        r.jc = (r.jc + rj - 1) / rj;
-        r.kc = (r.kc + rk - 1) / rk;
+        coarsen64(sn, g, r);
     }
 }
 
@@ -249,9 +263,11 @@ __device__ __forceinline__ void energy64(const DevCommon &cm, const DevSnap &sn,
     e.W = surface ? (bound ? sn.Ibmax_e64 : sn.Ibmax_w64) * (Bi / Bm) : sn.Ibmax64 * (Bi / Bm) * (Ki / Km);
 }
 
-/* one fluid ray from the centre of cell (i,j,k) (kernel_fluid :115-224) */
-__device__ inline double fluid_ray64(const DevCommon &cm, const DevSnap &sn, int i, int j, int k, uint32_t cid, int v,
-                                     int h, uint32_t key, bool have_key, float Ti, int &steps) {
+/* a fluid ray's start at the centre of cell (i,j,k): direction, position
+ * and per-ray constants (kernel_fluid :115-186) */
+__device__ __forceinline__ void fluid_setup64(const DevCommon &cm, const DevSnap &sn, int i, int j, int k,
+                                              uint32_t cid, int v, int h, uint32_t key, bool have_key, float Ti,
+                                              Ray64 &r, En64 &e) {
     const uint32_t tag = TAG_FLUID + 16u * (uint32_t)sn.snap;
     U4 u{0.f, 0.f, 0.f, 0.f};
     if (!have_key || !cm.sampler) u = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)v, cid, tag);
@@ -268,13 +284,19 @@ __device__ inline double fluid_ray64(const DevCommon &cm, const DevSnap &sn, int
     if (sx == 0.0) sx = (double)1e-10f;
     if (sy == 0.0) sy = (double)1e-10f;
     if (sz == 0.0) sz = (double)1e-10f;
-    Ray64 r;
     set_dir64(r, sx, sy, sz);
     r.ic = i; r.jc = j; r.kc = k;
     r.xp = (double)sn.x0[i]; r.yp = (double)sn.y0[j]; r.zp = (double)sn.z0[k];
     r.tra = 1.0;
-    En64 e;
     energy64(cm, sn, key, Ti, false, 0, e);
+}
+
+/* one fluid ray from the centre of cell (i,j,k) (kernel_fluid :115-224) */
+__device__ inline double fluid_ray64(const DevCommon &cm, const DevSnap &sn, int i, int j, int k, uint32_t cid, int v,
+                                     int h, uint32_t key, bool have_key, float Ti, int &steps) {
+    Ray64 r;
+    En64 e;
+    fluid_setup64(cm, sn, i, j, k, cid, v, h, key, have_key, Ti, r, e);
     double de = 0.0;
     int status;
     march_all64(cm, sn, r, e, cm.toll, de, steps, status);
