@@ -94,7 +94,7 @@ def test_mc_gpu_with_exact_estimator_and_spline_restriction(tmp_path, nb_tables)
 DRIVER = os.path.join(os.path.dirname(os.path.abspath(__file__)), "dropin", "dropin_driver")
 
 
-@pytest.mark.skipif(not os.path.exists(DRIVER), reason="C driver not built (python -c 'import __graft_entry__ as g; g.build()')")
+@pytest.mark.skipif(not os.path.exists(DRIVER), reason="C driver not built (radchan_amd.build.build_dropin_driver())")
 def test_c_driver_as_the_dns_calls(tmp_path):
     """A C program (tests/dropin/dropin_driver.c) calls get_results_ before
     any launch (outputs untouched, error reported), then mc_gpu_ /
