@@ -66,9 +66,12 @@ def test_random_configuration(nb_tables, i):
         q.fp64 = 1
         o64 = O.Oracle(q, tab, T, estimator=est)
         _, want, _, _ = o64.solve(O.all_cells(q), precision=1)
+        wwant = o64.solve(O.all_wall_cells(q), surface=True, precision=1)[1] if q.calcwalls else None
         o64.close()
-        got, _ = _solve(q, tab, T, est, VARIANT_PLAIN)
+        got, wall = _solve(q, tab, T, est, VARIANT_PLAIN)
         assert np.array_equal(got, want)
+        if q.calcwalls:
+            assert np.array_equal(wall[solved], wwant[solved]) and np.all(wall[~solved] == 0.0)
 
 
 @pytest.mark.parametrize("i", range(16))
@@ -235,3 +238,16 @@ def test_random_radix_widths(nb_tables, i):
     assert np.array_equal(got, de32.astype(np.float64))
     got, _ = _solve(p, tab, T, est, VARIANT_BATCHED)
     assert np.abs(got - de32d).max() <= 1e-12 * np.abs(de32d).max()
+
+
+@pytest.mark.parametrize("i", range(16))
+def test_random_ray_traces(nb_tables, i):
+    """Single rays of random configurations traced on the GPU and by the
+    oracle: band, g, end cell, level, step count, end status and the hash
+    of the visited cells equal; direction, final transmissivity and FP32
+    tally bit for bit; fluid rays, and wall rays when wall fluxes are on."""
+    from tests.test_gpu_parity import _check_trace
+    p, tab, T, _ = random_case(600 + i, nb_tables)
+    _check_trace(p, tab, T, n=256, seed=i)
+    if p.calcwalls:
+        _check_trace(p, tab, T, surface=True, n=128, seed=100 + i)
