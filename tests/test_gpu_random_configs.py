@@ -222,8 +222,9 @@ def test_random_dropin(tmp_path, nb_tables, i):
 @pytest.mark.parametrize("i", range(8))
 def test_random_radix_widths(nb_tables, i):
     """Random configurations at random band-sort widths above 256 keys (the
-    radix sort through HBM), up to param.h's 6000: both variants against
-    the oracle on every cell."""
+    radix sort through HBM), up to param.h's 6000, sorted bands and wall
+    fluxes on wherever an x wall is adiabatic: both variants against the
+    oracle on every cell and wall cell."""
     p, tab, T, est = random_case(500 + i, nb_tables)
     rng = np.random.default_rng(9000 + i)
     p.photmax = int(rng.integers(257, 6001))
@@ -231,13 +232,25 @@ def test_random_radix_widths(nb_tables, i):
         p.nVar = 1
     print(f"radix width {i}: {p.imax}x{p.jmax}x{p.kmax} grid_num {p.grid_num} photmax {p.photmax} nVar {p.nVar} "
           f"srt {p.srt} grey {p.grey} sampler {p.sampler}", flush=True)
+    if tab is not None and p.srt == 0:
+        p.srt = 1                      # the radix sort is the sorted narrow band's
+    p.calcwalls = int(p.adiabw == 1 or p.adiabe == 1)
     o = O.Oracle(p, tab, T, estimator=est)
     de32, de32d, _, _ = o.solve(O.all_cells(p))
+    w32 = w32d = None
+    if p.calcwalls:
+        w32, w32d, _, _ = o.solve(O.all_wall_cells(p), surface=True)
+        half = w32.shape[0] // 2
+        solved = np.repeat(np.array([p.adiabw == 1, p.adiabe == 1]), half)
     o.close()
-    got, _ = _solve(p, tab, T, est, VARIANT_PLAIN)
+    got, wall = _solve(p, tab, T, est, VARIANT_PLAIN)
     assert np.array_equal(got, de32.astype(np.float64))
-    got, _ = _solve(p, tab, T, est, VARIANT_BATCHED)
+    if p.calcwalls:
+        assert np.array_equal(wall[solved], w32[solved].astype(np.float64))
+    got, wall = _solve(p, tab, T, est, VARIANT_BATCHED)
     assert np.abs(got - de32d).max() <= 1e-12 * np.abs(de32d).max()
+    if p.calcwalls:
+        assert np.abs(wall[solved] - w32d[solved]).max() <= 1e-12 * np.abs(w32d[solved]).max()
 
 
 @pytest.mark.parametrize("i", range(16))
