@@ -350,10 +350,13 @@ __global__ void __launch_bounds__(128) k_surface_plain(DevCommon cm, const DevSn
 
 /* ------------------------------------------------------------------------ */
 /* full-FP64 variant (floatpoint 1): the reference's launch shape, a thread  */
-/* per emitting (wall) cell, rays in sequence, FP64 per-cell sums             */
-__global__ void __launch_bounds__(128) k_fluid_f64(DevCommon cm, const DevSnap *__restrict__ snaps, int s, int v,
-                                                   const uint16_t *__restrict__ keys, double *__restrict__ de,
-                                                   unsigned long long *__restrict__ stats) {
+/* per emitting (wall) cell, rays in sequence, FP64 per-cell sums.  Both     */
+/* kernels are held to 128 VGPRs (4 wavefronts per SIMD, a few spills): 7 %  */
+/* faster at 192x128x192 than with the 151 registers and 3 wavefronts they   */
+/* take unbounded.                                                          */
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_fluid_f64(DevCommon cm, const DevSnap *__restrict__ snaps, int s, int v, const uint16_t *__restrict__ keys,
+            double *__restrict__ de, unsigned long long *__restrict__ stats) {
     const DevSnap &sn = snaps[s];
     int64_t steps = 0, rays = 0;
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < cm.ncell; n += gridDim.x * blockDim.x) {
@@ -381,9 +384,9 @@ __global__ void __launch_bounds__(128) k_fluid_f64(DevCommon cm, const DevSnap *
     }
 }
 
-__global__ void __launch_bounds__(128) k_surface_f64(DevCommon cm, const DevSnap *__restrict__ snaps, int s, int v,
-                                                     const uint16_t *__restrict__ keys, double *__restrict__ de,
-                                                     unsigned long long *__restrict__ stats) {
+__global__ void __launch_bounds__(128) __attribute__((amdgpu_waves_per_eu(4, 8)))
+k_surface_f64(DevCommon cm, const DevSnap *__restrict__ snaps, int s, int v, const uint16_t *__restrict__ keys,
+              double *__restrict__ de, unsigned long long *__restrict__ stats) {
     const DevSnap &sn = snaps[s];
     int64_t steps = 0, rays = 0;
     for (int n = blockIdx.x * blockDim.x + threadIdx.x; n < 2 * cm.nsurf; n += gridDim.x * blockDim.x) {
