@@ -138,6 +138,7 @@ struct rc_ctx {
     DevBuf dsnap;
     DevBuf de, ds, mean, sdev, smean, ssdev;
     DevBuf keys, stats, queue;
+    DevBuf scan;                            /* a field's check flag and Tmax bits (rc_set_fields) */
     DevBuf bacc;                            /* per-band plain variant: FP32 cell sums */
     int reduction = 0;                      /* batched tallies: 0 FP64 atomics, 1 deterministic */
     DevBuf rayf, rayw;                      /* deterministic reduction: per-ray tallies */
@@ -708,12 +709,19 @@ int rc_set_fields(rc_ctx *c, int nsnap, const float *const *T, int zstart) {
     for (int s = 0; s < nsnap; ++s) {
         if (!c->snaps[s]) c->snaps[s] = std::make_unique<SnapState>();
         SnapState &S = *c->snaps[s];
-        for (size_t q = 0; q < nT0; ++q)
-            if (!std::isfinite(T[s][q]) || !(T[s][q] > 0.f)) {
-                set_error("temperature field " + std::to_string(s) + " holds a non-positive or non-finite value");
-                return -4;
-            }
-        S.Tmax = field_tmax(p, T[s]);
+        /* the field to the device, checked and its Tmax found there */
+        if (int e = S.T0.ensure(nT0 * 4)) return e;
+        if (int e = c->scan.ensure(2 * sizeof(uint32_t))) return e;
+        RC_HIP(hipMemcpyAsync(S.T0.p, T[s], nT0 * 4, hipMemcpyHostToDevice, c->stream));
+        launch_field_scan(c->stream, S.T0.as<float>(), p.imax, p.jmax, p.kmax, c->scan.as<uint32_t>());
+        uint32_t scan[2] = {0u, 0u};
+        RC_HIP(hipMemcpyAsync(scan, c->scan.p, sizeof scan, hipMemcpyDeviceToHost, c->stream));
+        RC_HIP(hipStreamSynchronize(c->stream));
+        if (scan[0]) {
+            set_error("temperature field " + std::to_string(s) + " holds a non-positive or non-finite value");
+            return -4;
+        }
+        std::memcpy(&S.Tmax, &scan[1], 4);
         const Tables *tb = &c->tab;
         if (p.grey) {
             S.greyhost.resize(p.nB, p.nT, 1);
@@ -725,8 +733,6 @@ int rc_set_fields(rc_ctx *c, int nsnap, const float *const *T, int zstart) {
         }
         if (int e = snapshot_scalars(p, *tb, S)) return e;
         if (int e = snapshot_rows(c, *tb, S)) return e;
-        if (int e = S.T0.ensure(nT0 * 4)) return e;
-        RC_HIP(hipMemcpyAsync(S.T0.p, T[s], nT0 * 4, hipMemcpyHostToDevice, c->stream));
         const TableSet &tset = S.grey ? *S.grey : c->nbset;
         if (int e = S.Pall.ensure(c->npos * 4)) return e;
         const bool f64 = p.fp64 == 1;

@@ -39,7 +39,6 @@ struct Grid {
 
 float planck(float T, float nu);
 void build_grid(const rc_params &p, int level, Grid &g);
-float field_tmax(const rc_params &p, const float *T);
 int read_tables(const std::string &dir, double chanHeight, Tables &t);
 int grey_tables(const rc_params &p, float Tmax, Tables &t);
 int write_tables(const std::string &dir, const Tables &t, double chanHeight);

@@ -159,6 +159,8 @@ void batched_schedule(const DevCommon &cm, int &wg_per_cu, int &threshold, int &
 size_t batched_srt_keys(const DevCommon &cm, long nunit_cells, int wg_per_cu);
 void launch_zero(hipStream_t st, double *p, size_t n);
 void launch_zero_u64(hipStream_t st, unsigned long long *p, int n);
+/* input check and Tmax of a temperature field on the device (out: 2 words) */
+void launch_field_scan(hipStream_t st, const float *T, int imax, int jmax, int kmax, uint32_t *out);
 
 }  // namespace rc
 

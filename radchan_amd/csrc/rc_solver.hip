@@ -507,6 +507,50 @@ void launch_surface_plain(hipStream_t st, const DevCommon &cm, const DevSnap *sn
     hipLaunchKernelGGL(k_surface_plain, dim3(blocks), dim3(128), 0, st, cm, snaps, s, v, keys, de, stats);
 }
 
+/* Input check and Tmax of a temperature field (imax+2)(jmax+2)(kmax+2) on
+ * the device: out[0] |= 1 when an entry (ghosts included) is not a
+ * positive finite number; out[1] = the largest interior temperature and
+ * x-wall midpoint (T_0 + T_1)/2, (T_imax + T_imax+1)/2 of the interior rows
+ * (mc_gpu_noint.cu:696-710), as float bits: positive floats order as their
+ * bits.  out zero on entry. */
+__global__ void __launch_bounds__(256) k_field_scan(const float *__restrict__ T, int imax, int jmax, int kmax,
+                                                    uint32_t *__restrict__ out) {
+    const uint32_t nx = (uint32_t)imax + 2u, ny = (uint32_t)jmax + 2u;
+    const size_t n = (size_t)nx * ny * (size_t)(kmax + 2);
+    uint32_t bad = 0u, m = 0u;
+    for (size_t q = (size_t)blockIdx.x * blockDim.x + threadIdx.x; q < n; q += (size_t)gridDim.x * blockDim.x) {
+        const float t = T[q];
+        bad |= (t > 0.f && t <= 3.40282347e38f) ? 0u : 1u;
+        const uint32_t row = (uint32_t)(q / nx), i = (uint32_t)(q - (size_t)row * nx);
+        const uint32_t j = row % ny, k = row / ny;
+        if (j >= 1u && j <= (uint32_t)jmax && k >= 1u && k <= (uint32_t)kmax && i >= 1u && i <= (uint32_t)imax) {
+            float c = t;
+            if (i == 1u) c = fmaxf(c, (T[q - 1] + t) * 0.5f);
+            if (i == (uint32_t)imax) c = fmaxf(c, (t + T[q + 1]) * 0.5f);
+            m = max(m, __float_as_uint(c));
+        }
+    }
+    for (int o = 32; o > 0; o >>= 1) {
+        bad |= (uint32_t)__shfl_xor((int)bad, o);
+        m = max(m, (uint32_t)__shfl_xor((int)m, o));
+    }
+    if ((threadIdx.x & 63) == 0) {
+        if (bad) atomicOr(&out[0], 1u);
+        atomicMax(&out[1], m);
+    }
+}
+
+__global__ void k_zero_u32x2(uint32_t *p) {
+    if (threadIdx.x < 2) p[threadIdx.x] = 0u;
+}
+
+void launch_field_scan(hipStream_t st, const float *T, int imax, int jmax, int kmax, uint32_t *out) {
+    const size_t n = (size_t)(imax + 2) * (jmax + 2) * (kmax + 2);
+    const int blocks = (int)std::max<size_t>(1, std::min<size_t>((n + 255) / 256, 8192));
+    hipLaunchKernelGGL(k_zero_u32x2, dim3(1), dim3(64), 0, st, out);
+    hipLaunchKernelGGL(k_field_scan, dim3(blocks), dim3(256), 0, st, T, imax, jmax, kmax, out);
+}
+
 void launch_fluid_f64(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int s, int v, const uint16_t *keys,
                       double *de, unsigned long long *stats) {
     const int blocks = std::max(1, std::min((cm.ncell + 127) / 128, 1 << 20));

@@ -185,20 +185,6 @@ void build_grid(const rc_params &p, int level, Grid &g) {
     g.x[im + 1] = g.xu[im] + (g.xu[im] - g.x[im]);
 }
 
-/* hottest interior cell or x-wall midpoint (mc_gpu_noint.cu:696-710) */
-float field_tmax(const rc_params &p, const float *T) {
-    const size_t nx = (size_t)p.imax + 2, ny = (size_t)p.jmax + 2;
-    float m = 0.f;
-    for (int k = 1; k <= p.kmax; ++k)
-        for (int j = 1; j <= p.jmax; ++j) {
-            const float *row = T + nx * ((size_t)j + ny * (size_t)k);
-            for (int i = 1; i <= p.imax; ++i) m = std::max(m, row[i]);
-            m = std::max(m, (row[0] + row[1]) * 0.5f);
-            m = std::max(m, (row[p.imax] + row[p.imax + 1]) * 0.5f);
-        }
-    return m;
-}
-
 namespace {
 
 /* whitespace-separated numeric tokens of a text file */

@@ -59,3 +59,30 @@ def test_largest_sorted_photmax(nb_tables):
     p = P.base_params(2, 2, 2, photmax=16384, nVar=1, fact_mesh=1.5, seed=33)
     T = F.channel_field(p, 800.0, 1000.0, noise=0.03, seed=3)
     _both(p, nb_tables, T)
+
+
+def test_field_check_and_tmax_on_the_device(nb_tables):
+    """rc_set_fields checks the field and finds Tmax on the GPU: a zero in a
+    ghost plane, a negative or a NaN anywhere is refused with the message;
+    Tmax (interior cells and x-wall midpoints) equals the oracle's, also
+    when the hottest value is an x-wall midpoint."""
+    p = P.base_params(5, 3, 4, photmax=8, nVar=2, fact_mesh=1.2, seed=7)
+    T = F.channel_field(p, 700.0, 900.0, noise=0.03, seed=7)
+    for q, val in ((0, 0.0), (T.size - 1, -1.0), (T.size // 2, np.nan), (17, np.inf)):
+        bad = T.copy()
+        bad[q] = val
+        s = Solver(p, nb_tables)
+        with pytest.raises(RuntimeError, match="non-positive or non-finite"):
+            s.set_fields([bad])
+        s.close()
+    A = T.reshape(p.kmax + 2, p.jmax + 2, p.imax + 2).copy()
+    A[2, 2, 0] = 1500.0              # a hot west ghost: the midpoint (T_0 + T_1)/2 is the maximum
+    hot = A.reshape(-1)
+    for field in (T, hot):
+        s = Solver(p, nb_tables)
+        s.set_fields([field])
+        o = O.Oracle(p, nb_tables, field)
+        assert s.info()["Tmax"] == o.info()["Tmax"]
+        o.close()
+        s.close()
+    assert O.Oracle(p, nb_tables, hot).info()["Tmax"] == (np.float32(1500.0) + A[2, 2, 1]) * np.float32(0.5)
