@@ -85,3 +85,35 @@ def field_for(p, rng, seed):
     """a channel field whose extrapolated x ghost planes stay positive"""
     tw = float(rng.uniform(500, 800))
     return F.channel_field(p, tw, tw * float(rng.uniform(1.2, 1.7)), noise=0.03, seed=seed)
+
+
+def deep_case(i: int, tables):
+    """configuration i with three to six levels coarsened by ratios up to 7
+    per axis (param.h coarsens x by 29 and 7, which the index map's
+    multiplicative inverses must hold), steps caps 2..6, photmax 8..64"""
+    rng = np.random.default_rng(3000 + i)
+    n = tuple(int(rng.choice([12, 20, 28, 30])) for _ in range(3))
+    nlev = int(rng.integers(3, 7))
+    sizes = [n]
+    for _ in range(nlev - 1):
+        nxt = []
+        for a in sizes[-1]:
+            divs = [d for d in (1, 2, 3, 5, 7) if a % d == 0]
+            nxt.append(a // int(rng.choice(divs)))
+        sizes.append(tuple(nxt))
+    maxs = [int(rng.integers(2, 7)) for _ in range(nlev - 1)] + [100000]
+    # banded grey at a small absorption coefficient: rays long enough to
+    # reach the coarse levels
+    grey = bool(rng.random() < 0.6)
+    p = P.base_params(*n, photmax=int(rng.integers(8, 65)), nVar=2, fact_mesh=float(rng.choice([0.0, 1.6])),
+                      grey=grey, srt=int(rng.random() < 0.75), seed=int(rng.integers(0, 1 << 20)), levels=sizes,
+                      maxs=maxs)
+    if grey:
+        p.abscoeff = float(rng.choice([0.02, 0.1, 0.5]))
+    p.adiabw, p.adiabe = int(rng.random() < 0.7), int(rng.random() < 0.7)
+    p.Tw[0], p.Tw[1] = float(rng.uniform(350, 450)), float(rng.uniform(350, 450))
+    if rng.random() < 0.5:
+        p.bd[4] = p.bd[5] = int(rng.choice([2, 3, 4]))
+    p.calcwalls = int((p.adiabw == 1 or p.adiabe == 1) and rng.random() < 0.5)
+    p.sampler = int(rng.random() < 0.5)
+    return p, (None if grey else tables), field_for(p, rng, 40 + i)

@@ -15,7 +15,7 @@ import pytest
 from radchan_amd import VARIANT_BATCHED, VARIANT_PLAIN, VARIANT_PLAIN_BANDS, Group, Solver
 from radchan_amd.tables import bf16_dequantized
 from oracle import oracle as O
-from tests.random_configs import field_for, random_case, regrid
+from tests.random_configs import deep_case, field_for, random_case, regrid
 
 pytestmark = pytest.mark.gpu
 
@@ -264,3 +264,24 @@ def test_random_ray_traces(nb_tables, i):
     _check_trace(p, tab, T, n=256, seed=i)
     if p.calcwalls:
         _check_trace(p, tab, T, surface=True, n=128, seed=100 + i)
+
+
+@pytest.mark.parametrize("i", range(12))
+def test_random_deep_levels(nb_tables, i):
+    """Three to six grid levels coarsened by ratios up to 7 per axis (the
+    index map's multiplicative inverses), tight step caps, optically thin
+    banded grey among them so that rays reach the coarse levels: single-ray
+    traces (level, cell sequence, steps, tally bits) and both variants'
+    tallies against the oracle."""
+    from tests.test_gpu_parity import _check_trace
+    p, tab, T = deep_case(i, nb_tables)
+    print(f"deep levels {i}: {p.imax}x{p.jmax}x{p.kmax} grid_num {p.grid_num} "
+          f"levels {[(p.maxi[g], p.maxj[g], p.maxk[g]) for g in range(p.grid_num)]} grey {p.grey}", flush=True)
+    _check_trace(p, tab, T, n=256, seed=i)
+    o = O.Oracle(p, tab, T)
+    de32, de32d, _, _ = o.solve(O.all_cells(p))
+    o.close()
+    got, _ = _solve(p, tab, T, 0, VARIANT_PLAIN)
+    assert np.array_equal(got, de32.astype(np.float64))
+    got, _ = _solve(p, tab, T, 0, VARIANT_BATCHED)
+    assert np.abs(got - de32d).max() <= 1e-12 * np.abs(de32d).max()
