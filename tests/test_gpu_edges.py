@@ -86,3 +86,22 @@ def test_field_check_and_tmax_on_the_device(nb_tables):
         o.close()
         s.close()
     assert O.Oracle(p, nb_tables, hot).info()["Tmax"] == (np.float32(1500.0) + A[2, 2, 1]) * np.float32(0.5)
+
+
+def test_many_runs_per_cell(nb_tables):
+    """nVar 37 runs of 5 rays per cell (units are taken a cell's runs at a
+    time): both variants against the oracle, and the scattered mean and
+    the reference's std-error formula from the tallies."""
+    p = P.base_params(6, 4, 5, photmax=5, nVar=37, fact_mesh=1.2, seed=41)
+    T = F.channel_field(p, 700.0, 1000.0, noise=0.03, seed=41)
+    _both(p, nb_tables, T)
+    s = Solver(p, nb_tables)
+    s.set_fields([T])
+    s.solve(VARIANT_BATCHED)
+    de = s.tallies()
+    mean, sd = s.results()
+    m = de.mean(axis=1)
+    want_sd = np.sqrt(((de - m[:, None]) ** 2).sum(axis=1) / ((37 - 0.5) * 36))
+    assert np.allclose(mean.reshape(-1), m, rtol=1e-6, atol=1e-6 * np.abs(m).max())
+    assert np.allclose(sd.reshape(-1), want_sd, rtol=1e-5, atol=1e-6 * np.abs(m).max())
+    s.close()
