@@ -39,6 +39,7 @@
 #include "rc_internal.h"
 #include "rc_kernels.h"
 #include "rc_march.h"
+#include "rc_radix.h"
 #include "rc_march64.h"
 
 using namespace rcd;
@@ -180,30 +181,77 @@ __global__ void __launch_bounds__(256) k_pos_from_T(const float *__restrict__ Tl
 /* ------------------------------------------------------------------------ */
 /* the srt enumeration                                                       */
 
+/* the radix sort's LDS: photmax keys (rounded up to 64) twice as 16 bits,
+ * then two 256-bucket histograms; within 64 KiB up to 15 872 keys */
+__host__ __device__ __forceinline__ bool sort_radix(int P) { return P > 256 && ((P + 63) & ~63) * 4 + 2048 <= 65536; }
+
+/* the Philox stream of sort unit u: cell id, run and tag (explicit
+ * (cell id, run, tag) triples, else fluid cells or the two walls' cells of
+ * the slab) */
+__device__ __forceinline__ void unit_stream(const DevCommon &cm, int u, int surface, const int32_t *ulist,
+                                            uint32_t &cid, int &vv, uint32_t &tag) {
+    if (ulist) {
+        cid = (uint32_t)ulist[3 * u]; vv = ulist[3 * u + 1]; tag = (uint32_t)ulist[3 * u + 2];
+    } else if (!surface) {
+        cid = (uint32_t)u + (uint32_t)cm.imax * cm.jmax * (cm.zstart - 1); tag = TAG_FLUID;
+    } else {
+        const int b = u / cm.nsurf;
+        cid = (uint32_t)(u % cm.nsurf) + (uint32_t)cm.jmax * (cm.zstart - 1); tag = b ? TAG_EAST : TAG_WEST;
+    }
+}
+
 /* One workgroup (one wavefront) per (cell, run): draws photmax keys and
- * bitonic-sorts them in LDS.  The sorted sequence is the one the reference's
- * histogram enumeration produces (kernel_find :1023-1049 + wave_find
- * :1234-1267): ascending sorted-band rank, then g; equal keys are identical
- * draws, so any correct sort gives the same sequence. */
+ * sorts them in LDS, by a bitonic network up to 256 keys (and above the
+ * radix sort's LDS, 15 872 keys), else by the two-pass LSD radix sort the
+ * batched kernel uses (8-bit digits; O(photmax) like the reference's
+ * histogram enumeration, where the network's stages grow as log^2).  The
+ * sorted sequence is the one the reference's histogram enumeration
+ * produces (kernel_find :1023-1049 + wave_find :1234-1267): ascending
+ * sorted-band rank, then g; equal keys are identical draws, so any correct
+ * sort gives the same sequence. */
 __global__ void __launch_bounds__(64) k_sort_bands(DevCommon cm, const DevSnap *__restrict__ snaps, int s, int v,
                                                    int surface, int nunit, const int32_t *__restrict__ ulist,
                                                    uint16_t *__restrict__ keys) {
     extern __shared__ uint32_t lds[];
     const DevSnap &sn = snaps[s];
     const int P = cm.photmax;
+    const int lane = threadIdx.x;
+    if (sort_radix(P)) {
+        /* keys in draw order at ka, after the low-byte pass at kb, the two
+         * digit histograms after them */
+        const int Pr = (P + 63) & ~63;
+        uint16_t *const ka = reinterpret_cast<uint16_t *>(lds);
+        uint16_t *const kb = ka + Pr;
+        uint32_t *const hist = lds + Pr;
+        for (int u = blockIdx.x; u < nunit; u += gridDim.x) {
+            uint32_t cid, tag;
+            int vv = v;
+            unit_stream(cm, u, surface, ulist, cid, vv, tag);
+            const uint32_t tagx = tag + 16u * (uint32_t)sn.snap;
+            for (int w = lane; w < 512; w += 64) hist[w] = 0u;
+            __syncthreads();
+            for (int h = lane; h < P; h += 64) {
+                const U4 r = draw4(cm.k0, cm.k1, (uint32_t)h, (uint32_t)vv, cid, tagx);
+                const uint32_t key = band_key(cm, sn, tag != TAG_FLUID, r.c, r.d);
+                ka[h] = (uint16_t)key;
+                atomicAdd(&hist[key & 255u], 1u);
+                atomicAdd(&hist[256u + (key >> 8)], 1u);
+            }
+            __syncthreads();
+            radix_offsets(hist, lane);
+            radix_pass(ka, kb, hist, P, 0, lane);
+            radix_pass(kb, ka, hist + 256, P, 8, lane);
+            for (int h = lane; h < P; h += 64) keys[(size_t)u * P + h] = ka[h];
+            __syncthreads();
+        }
+        return;
+    }
     int n2 = 1;
     while (n2 < P) n2 <<= 1;
     for (int u = blockIdx.x; u < nunit; u += gridDim.x) {
         uint32_t cid, tag;
         int vv = v;
-        if (ulist) {   /* explicit (cell id, run, tag) triples */
-            cid = (uint32_t)ulist[3 * u]; vv = ulist[3 * u + 1]; tag = (uint32_t)ulist[3 * u + 2];
-        } else if (!surface) {
-            cid = (uint32_t)u + (uint32_t)cm.imax * cm.jmax * (cm.zstart - 1); tag = TAG_FLUID;
-        } else {
-            const int b = u / cm.nsurf;
-            cid = (uint32_t)(u % cm.nsurf) + (uint32_t)cm.jmax * (cm.zstart - 1); tag = b ? TAG_EAST : TAG_WEST;
-        }
+        unit_stream(cm, u, surface, ulist, cid, vv, tag);
         const uint32_t tagx = tag + 16u * (uint32_t)sn.snap;
         for (int h = threadIdx.x; h < n2; h += blockDim.x) {
             uint32_t key = 0xFFFFFFFFu;
@@ -480,8 +528,9 @@ void launch_sort(hipStream_t st, const DevCommon &cm, const DevSnap *snaps, int 
                  const int32_t *ulist, uint16_t *keys) {
     int n2 = 1;
     while (n2 < cm.photmax) n2 <<= 1;
+    const size_t lds = sort_radix(cm.photmax) ? (size_t)((cm.photmax + 63) & ~63) * 4 + 2048 : (size_t)n2 * 4;
     const int blocks = std::min(nunit, 1 << 20);
-    hipLaunchKernelGGL(k_sort_bands, dim3(blocks), dim3(64), (size_t)n2 * 4, st, cm, snaps, s, v, surface, nunit,
+    hipLaunchKernelGGL(k_sort_bands, dim3(blocks), dim3(64), lds, st, cm, snaps, s, v, surface, nunit,
                        ulist, keys);
 }
 
