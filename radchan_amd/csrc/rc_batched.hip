@@ -12,11 +12,13 @@
  * meanwhile.
  *
  * The main loop generates rays in bulk (all 64 lanes set up one ray each),
- * refills idle lanes from the queue once enough of them are idle, and
+ * refills idle lanes from the queue once enough of them are idle (at the
+ * top of the iteration, and in refill mode 1 also between its steps), and
  * advances every lane by a few steps (march_core_bulk, the loop body of
  * march_ray; a ray that meets a black wall takes the wall's emission on
- * its next step, through that step's own table lookup).  Rays keep their (sorted) order of dispatch, so neighbouring
- * lanes march neighbouring bands.  Each finished ray adds its FP32 tally to
+ * its next step, through that step's own table lookup).  Rays keep their
+ * (sorted) order of dispatch, so neighbouring lanes march neighbouring
+ * bands.  Each finished ray adds its FP32 tally to
  * its unit's FP64 total in an LDS slot; a slot's total goes to HBM when the
  * slot is reused.  When no unit is left the wavefront drains its last rays
  * (look-ahead march, then a whole-wavefront march of the longest ones).
