@@ -75,12 +75,37 @@ def test_reference_defaults():
     (lambda p: p.maxi.__setitem__(1, 7), "divide"),
     (lambda p: setattr(p, "photmax", 0), "photmax"),
     (lambda p: setattr(p, "grid_num", 10), "grid_num"),
+    (lambda p: setattr(p, "grid_num", 0), "grid_num"),
+    (lambda p: setattr(p, "imax", 0), "must be positive"),
+    (lambda p: setattr(p, "nVar", 0), "nVar must be positive"),
+    (lambda p: setattr(p, "p_col", 0), "multiple of p_col"),
+    (lambda p: p.maxi.__setitem__(0, 8), "level 0 must be the full grid"),
+    (lambda p: p.maxj.__setitem__(1, 0), "level sizes must be positive"),
+    (lambda p: p.maxs.__setitem__(1, 0), "maxs must be positive"),
+    (lambda p: [setattr(p, "jmax", 4096), setattr(p, "kmax", 4096), p.maxj.__setitem__(0, 4096),
+                p.maxk.__setitem__(0, 4096)], "grid too large"),
+    (lambda p: setattr(p, "photmax", 16385), "photmax above 16384"),
+    (lambda p: setattr(p, "Ly", 0.0), "domain lengths"),
+    (lambda p: setattr(p, "Lz", float("nan")), "domain lengths"),
+    (lambda p: p.bd.__setitem__(2, 0), "x and y faces"),
+    (lambda p: setattr(p, "nT", 1), "table sizes"),
+    (lambda p: setattr(p, "nB", 4097), "16 bits"),
+    (lambda p: setattr(p, "grey", 1), "grey runs use nQ=1"),
+    (lambda p: [setattr(p, "grey", 1), setattr(p, "grey_true", 1), setattr(p, "nQ", 1)], "true grey uses nB=1"),
+    (lambda p: setattr(p, "toll", 1.0), "toll and tollS"),
+    (lambda p: setattr(p, "tollS", 0.0), "toll and tollS"),
+    (lambda p: [setattr(p, "calcwalls", 1), setattr(p, "adiabw", 0), setattr(p, "adiabe", 0)],
+     "calcwalls needs an adiabatic x wall"),
+    (lambda p: setattr(p, "sampler", 2), "sampler"),
+    (lambda p: setattr(p, "fp64", 2), "floatpoint"),
 ])
 def test_parameter_validation(change, msg):
     p = P.base_params(16, 16, 16, levels=[(16, 16, 16), (8, 8, 8)], maxs=[5, 100])
     p.p_col = 2
+    lib = L.load()
+    assert lib.rc_set_params(C.byref(p)) == 0
     change(p)
-    rc = L.load().rc_set_params(C.byref(p))
+    rc = lib.rc_set_params(C.byref(p))
     assert rc != 0 and msg in L.last_error()
 
 
